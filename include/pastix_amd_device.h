@@ -1,0 +1,161 @@
+/*
+ * pastix_amd_device.h -- the C ABI between the host C library and the
+ * hand-written HIP engine for one MI355X (gfx950).
+ *
+ * Everything here takes plain pointers, sizes and enums; nothing exposes HIP,
+ * torch or C++ types.  Device buffers are opaque `void *` obtained from
+ * pxd_malloc.  Every call returns 0 (PASTIX_SUCCESS) or a pastix_error_t code
+ * and never falls back to host arithmetic.
+ *
+ * Which reference routine each entry point replaces:
+ *   pxd_set_structure  -- SolverMatrix upload; the reference keeps it in host
+ *                         memory (blend/solver.h:200-273) and, for its CUDA path,
+ *                         copies blok tables per task (sopalin/parsec/
+ *                         parsec_sparse_matrix.c:480-507).
+ *   pxd_set_matrix     -- bcsc values (bcsc/bcsc.c:1887, bcsc/bcsc.h:119-133).
+ *   pxd_factorize      -- coeftabInit + fill (sopalin/coeftab.c:203-250,
+ *                         kernels/cpucblk_zinit.c:392-509, cpucblk_zcinit.c:80-170)
+ *                         followed by sopalinFacto[factotype][flttype]
+ *                         (sopalin/pastix_task_sopalin.c:51-58,500-508): panel
+ *                         (kernels/core_z{potrf,getrf,sytrf}sp.c), TRSM
+ *                         (kernels/core_ztrsmsp.c:355-399) and updates
+ *                         (kernels/core_zgemmsp.c:401-472,1523-1607).
+ *   pxd_solve          -- pastix_subtask_solve_adv (sopalin/pastix_task_solve.c:
+ *                         510-677): forward/backward supernodal TRSM
+ *                         (kernels/solve_ztrsmsp.c:300-625), LDL^T diagonal
+ *                         (:629-679) and the mixed-precision RHS conversion
+ *                         (sopalin/pastix_rhs.c:146-278).
+ *   pxd_spmv           -- bcsc_zspmv (bcsc/bcsc_zspmv.c:629-678).
+ *   pxd_vec_*          -- bvec_z* (bcsc/bvec_zcompute.c: nrm2 :71, dotc :561,
+ *                         gemv :1112, axpy/scal/copy).
+ */
+#ifndef PASTIX_AMD_DEVICE_H
+#define PASTIX_AMD_DEVICE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pxd_ctx_s pxd_ctx_t;
+
+/* Column block of the solver matrix (one per cblk, plus a final sentinel). */
+typedef struct pxd_cblk_s {
+    int64_t fcolnum;   /* first column (permuted numbering)             */
+    int64_t lcolnum;   /* last column                                   */
+    int64_t fblok;     /* index of its diagonal blok                    */
+    int64_t stride;    /* number of rows of the cblk                    */
+    int64_t coefoff;   /* offset of its tiles in the coeftab (elements) */
+    int64_t brownum;   /* first entry of its incoming blocks in browtab */
+} pxd_cblk_t;
+
+/* Block (tile) of the solver matrix (one per blok, plus a final sentinel). */
+typedef struct pxd_blok_s {
+    int64_t frownum;
+    int64_t lrownum;
+    int64_t lcblknm;   /* owning cblk                                    */
+    int64_t fcblknm;   /* facing cblk                                    */
+    int64_t coefind;   /* offset of the tile inside its cblk (elements)  */
+} pxd_blok_t;
+
+/* Timing / counters of the last factorization and solves. */
+typedef struct pxd_stats_s {
+    double fact_seconds;     /* device time of the whole factorization         */
+    double fill_seconds;     /* zero + scatter of A into the coeftab             */
+    double panel_seconds;    /* sum of diagonal-block kernels                    */
+    double trsm_seconds;     /* sum of off-diagonal TRSM kernels                 */
+    double update_seconds;   /* sum of update kernels                            */
+    double solve_seconds;    /* last pxd_solve                                   */
+    int64_t levels;          /* number of dependency levels                      */
+    int64_t update_tiles;    /* output tiles of all update launches              */
+    int64_t launches;        /* kernel launches of one factorization             */
+    int64_t coef_bytes;      /* bytes of the factor storage                      */
+} pxd_stats_t;
+
+int  pxd_device_count( void );
+int  pxd_create ( pxd_ctx_t **ctx, int device );
+void pxd_destroy( pxd_ctx_t *ctx );
+int  pxd_sync   ( pxd_ctx_t *ctx );
+int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
+/* Engine knobs: key names are documented in engine.hip (e.g. "timing"). */
+int  pxd_set_option( pxd_ctx_t *ctx, const char *key, double value );
+
+int  pxd_malloc ( pxd_ctx_t *ctx, size_t bytes, void **dptr );
+int  pxd_free   ( pxd_ctx_t *ctx, void *dptr );
+int  pxd_memset0( pxd_ctx_t *ctx, void *dptr, size_t bytes );
+int  pxd_h2d    ( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes );
+int  pxd_d2h    ( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes );
+
+/*
+ * Upload the solver structure.  cblks has cblknbr+1 entries, bloks bloknbr+1,
+ * browtab bloknbr-cblknbr.  n is the number of unknowns (rows of the RHS).
+ */
+int pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
+                       const pxd_cblk_t *cblks, const pxd_blok_t *bloks,
+                       const int64_t *browtab, int64_t coefnbr );
+
+/*
+ * Upload the permuted matrix P A P^T as a full-pattern CSC (both triangles,
+ * rows sorted in every column).  flttype is the spm coefficient type of the
+ * values (SpmDouble or SpmComplex64).  For general matrices the transpose is
+ * given as a second CSC (tcolptr/trowtab/tvalues) and used by the SpMV.
+ */
+int pxd_set_matrix( pxd_ctx_t *ctx, int flttype, int mtxtype, int64_t n, int64_t nnz,
+                    const int64_t *colptr, const int64_t *rowtab, const void *values,
+                    const int64_t *tcolptr, const int64_t *trowtab, const void *tvalues );
+
+/*
+ * Numerical factorization on the device.  factotype: PastixFactPOTRF/GETRF/
+ * SYTRF.  factor_flttype: SpmDouble/SpmFloat for real matrices, SpmComplex64/
+ * SpmComplex32 for complex ones (single = mixed precision).  |pivot| <
+ * threshold is replaced by threshold (static pivoting); the number of such
+ * replacements is returned in *nbpivots.
+ */
+int pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype,
+                   double threshold, int64_t *nbpivots );
+
+/* Download one side of the factor (0: L, 1: U^T) in the factor precision. */
+int pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes );
+
+/*
+ * x <- A^{-1} x in place for nrhs columns of a device array in the vector
+ * precision (double / complex double), leading dimension ldx.  With a single
+ * precision factor the conversion to single and back is fused into the sweeps.
+ */
+int pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx );
+
+/* y <- alpha A x + beta y (nrhs columns). alpha/beta are real. */
+int pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy,
+              int64_t nrhs, int64_t ld );
+
+/*
+ * Block vector operations on n x nrhs device arrays (vector precision).
+ * Scalars per column live in device memory (dalpha: nrhs entries, complex as
+ * pairs) so that a Krylov iteration runs without host round trips.
+ *   dot:   dres[c]  = x(:,c)^H y(:,c)
+ *   nrm2:  dres[c]  = ||x(:,c)||_2            (real output)
+ *   axpy:  y(:,c)  += sign * dalpha[c] * x(:,c)
+ *   scal:  x(:,c)  *= dalpha[c]              (or 1/dalpha[c] when inv)
+ *   copy:  y <- x
+ *   gemv:  x(:,c) += sum_j V_j(:,c) * dy[j*nrhs + c]   (V_j blocks of n x nrhs, ld n)
+ *   maxabs: dres[c] = ||x(:,c)||_inf
+ */
+int pxd_vec_dot   ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, const void *y, void *dres );
+int pxd_vec_nrm2  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, double *dres );
+int pxd_vec_maxabs( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, double *dres );
+int pxd_vec_axpy  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, double sign, const void *dalpha,
+                    const void *x, void *y );
+int pxd_vec_scal  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *dalpha, int inv, void *x );
+int pxd_vec_copy  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, void *y );
+int pxd_vec_gemv  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, int64_t m, const void *V,
+                    const void *dy, void *x );
+/* Residual-norm helpers: ||A||_inf of the uploaded matrix. */
+int pxd_matrix_norm_inf( pxd_ctx_t *ctx, double *norm );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASTIX_AMD_DEVICE_H */

@@ -1,0 +1,557 @@
+/*
+ * oracle.c -- CPU restatement of the PaStiX sopalin path (fill, static-pivot
+ * Cholesky, TRSM, 2D updates, static scheduler, supernodal solve, SpMV,
+ * GMRES) on a solver matrix given as plain arrays.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this library: it is the checker for the device engine
+ * and the CPU baseline timed beside it, never part of the product.
+ *
+ * Each routine follows a reference file:
+ *   orc_fill           kernels/cpucblk_zinit.c:392-509 (cpucblk_zfillin_fr)
+ *   potf2 / potrf      kernels/core_zpotrfsp.c:67-189 (64-column blocks)
+ *   trsm               kernels/core_ztrsmsp.c:148-183 (2D: one dtrsm per blok)
+ *   update             kernels/core_zgemmsp.c:401-472 (core_zgemmsp_2d2d)
+ *   cblk task          kernels/core_zpotrfsp.c:333-367, dependency counters of
+ *                      kernels/cpucblk_zmpi_coeftab.c:541-622
+ *   static scheduler   sopalin/sequential_zpotrf.c:102-147 (per-thread task
+ *                      lists in a simulated order, spin on ctrbcnt)
+ *   solve              kernels/solve_ztrsmsp.c:74-625
+ *   spmv               bcsc/bcsc_zspmv.c:57-76
+ *   gmres              refinement/z_refine_gmres.c:48-371
+ *
+ * Structure arrays: cblk[6*(cblknbr+1)] = {fcolnum, lcolnum, fblok, stride,
+ * coefoff, brownum}, blok[5*bloknbr] = {frownum, lrownum, lcblknm, fcblknm,
+ * coefind}, browtab[bloknbr-cblknbr].
+ */
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <stdatomic.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "oracle_blas.h"
+
+#define CB( k, f ) ( cblk[6 * ( k ) + ( f )] )
+#define BK( b, f ) ( blok[5 * ( b ) + ( f )] )
+enum { C_FCOL, C_LCOL, C_FBLOK, C_STRIDE, C_COEFOFF, C_BROWNUM };
+enum { B_FROW, B_LROW, B_LCBLK, B_FCBLK, B_COEFIND };
+
+static double
+wtime( void )
+{
+    struct timespec ts;
+    clock_gettime( CLOCK_MONOTONIC, &ts );
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+const char *
+orc_blas( void )
+{
+    orc_blas_init();
+    return orc_blas_name();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Typed kernels, instantiated for double and float.                        */
+
+#define ORC_TYPED( T, SFX, GEMM, TRSM, SQRT, FABS )                                                 \
+    static void fill_##SFX( int64_t n, const int64_t *cblk, const int64_t *blok,                    \
+                            const int64_t *colptr, const int64_t *rowtab, const double *val,        \
+                            const int64_t *col2cblk, T *L )                                         \
+    {                                                                                               \
+        for ( int64_t j = 0; j < n; j++ ) {                                                         \
+            int64_t k = col2cblk[j], ncol = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                  \
+            int64_t b = CB( k, C_FBLOK ), lb = CB( k + 1, C_FBLOK );                                \
+            for ( int64_t e = colptr[j]; e < colptr[j + 1]; e++ ) {                                 \
+                int64_t i = rowtab[e];                                                              \
+                if ( i < j ) continue;                                                              \
+                while ( b < lb && ( BK( b, B_LROW ) < i || BK( b, B_FROW ) > i ) ) b++;             \
+                if ( b >= lb ) continue;                                                            \
+                int64_t rows = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                               \
+                L[CB( k, C_COEFOFF ) + BK( b, B_COEFIND ) + ( i - BK( b, B_FROW ) ) +               \
+                  ( j - CB( k, C_FCOL ) ) * rows] = (T)val[e];                                      \
+            }                                                                                       \
+            (void)ncol;                                                                             \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* unblocked Cholesky with static pivoting (core_zpotf2sp) */                                   \
+    static void potf2_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
+    {                                                                                               \
+        for ( int64_t k = 0; k < n; k++ ) {                                                         \
+            T *akk = A + k * ( lda + 1 );                                                           \
+            if ( FABS( *akk ) < (T)tau ) {                                                          \
+                *akk = (T)tau;                                                                      \
+                ( *npiv )++;                                                                        \
+            }                                                                                       \
+            *akk = SQRT( *akk );                                                                    \
+            T a  = (T)1 / *akk;                                                                     \
+            for ( int64_t i = k + 1; i < n; i++ ) A[i + k * lda] *= a;                              \
+            for ( int64_t j = k + 1; j < n; j++ ) {                                                 \
+                T f = A[j + k * lda];                                                               \
+                for ( int64_t i = j; i < n; i++ ) A[i + j * lda] -= A[i + k * lda] * f;             \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* blocked (64) Cholesky of the diagonal tile (core_zpotrfsp) */                                \
+    static void potrf_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
+    {                                                                                               \
+        for ( int64_t k = 0; k < n; k += 64 ) {                                                     \
+            int64_t bs = ( n - k < 64 ) ? n - k : 64;                                               \
+            T      *a  = A + k * ( lda + 1 );                                                       \
+            potf2_##SFX( bs, a, lda, npiv, tau );                                                   \
+            if ( k + bs < n ) {                                                                     \
+                int64_t m  = n - k - bs;                                                            \
+                T      *a1 = a + bs;                                                                \
+                T      *a2 = a1 + bs * lda;                                                         \
+                TRSM( m, bs, a, lda, a1, lda );                                                     \
+                /* lower part of a2 -= a1 a1^T, by 64-column slabs */                               \
+                for ( int64_t j = 0; j < m; j += 64 ) {                                             \
+                    int64_t w = ( m - j < 64 ) ? m - j : 64;                                        \
+                    GEMM( ORC_TRANS, m - j, w, bs, (T)-1, a1 + j, lda, a1 + j, lda, (T)1,           \
+                          a2 + j + j * lda, lda );                                                  \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    static void trsm_cblk_##SFX( const int64_t *cblk, const int64_t *blok, int64_t k, T *L )        \
+    {                                                                                               \
+        int64_t n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                          \
+        T      *A = L + CB( k, C_COEFOFF );                                                         \
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {                   \
+            int64_t m = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                      \
+            TRSM( m, n, A, n, A + BK( b, B_COEFIND ), m );                                          \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* C[f(i)] -= A_i A_b^T for every blok i >= b (core_zgemmsp_2d2d) */                            \
+    static void update_##SFX( const int64_t *cblk, const int64_t *blok, int64_t k, int64_t b,       \
+                              T *L, pthread_mutex_t *locks )                                        \
+    {                                                                                               \
+        int64_t K  = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                         \
+        int64_t F  = BK( b, B_FCBLK );                                                              \
+        int64_t N  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                         \
+        int64_t fb = CB( F, C_FBLOK );                                                              \
+        const T *Bm = L + CB( k, C_COEFOFF ) + BK( b, B_COEFIND );                                  \
+        for ( int64_t i = b; i < CB( k + 1, C_FBLOK ); i++ ) {                                      \
+            while ( !( BK( i, B_FROW ) >= BK( fb, B_FROW ) && BK( i, B_LROW ) <= BK( fb, B_LROW ) ) ) \
+                fb++;                                                                               \
+            int64_t M   = BK( i, B_LROW ) - BK( i, B_FROW ) + 1;                                    \
+            int64_t ldc = BK( fb, B_LROW ) - BK( fb, B_FROW ) + 1;                                  \
+            T *C = L + CB( F, C_COEFOFF ) + BK( fb, B_COEFIND ) + ( BK( i, B_FROW ) - BK( fb, B_FROW ) ) + \
+                   ( BK( b, B_FROW ) - CB( F, C_FCOL ) ) * ldc;                                     \
+            if ( locks ) pthread_mutex_lock( locks + F );                                           \
+            GEMM( ORC_TRANS, M, N, K, (T)-1, L + CB( k, C_COEFOFF ) + BK( i, B_COEFIND ), M, Bm, N, \
+                  (T)1, C, ldc );                                                                   \
+            if ( locks ) pthread_mutex_unlock( locks + F );                                         \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* forward and backward sweeps (solve_cblk_ztrsmsp_forward/backward) */                         \
+    static void solve_##SFX( int64_t nc, const int64_t *cblk, const int64_t *blok,                  \
+                             const int64_t *browtab, const T *L, int64_t nrhs, T *X, int64_t ldx )  \
+    {                                                                                               \
+        for ( int64_t k = 0; k < nc; k++ ) {                                                        \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = 0; j < n; j++ ) {                                                 \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = 0; l < j; l++ ) s -= A[j + l * n] * xq[l];                    \
+                    xq[j] = s / A[j + j * n];                                                       \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {               \
+                int64_t m = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                  \
+                GEMM( ORC_NOTRANS, m, nrhs, n, (T)-1, A + BK( b, B_COEFIND ), m, x, ldx, (T)1,      \
+                      X + BK( b, B_FROW ), ldx );                                                   \
+            }                                                                                       \
+        }                                                                                           \
+        for ( int64_t k = nc - 1; k >= 0; k-- ) {                                                   \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = n - 1; j >= 0; j-- ) {                                            \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = j + 1; l < n; l++ ) s -= A[l + j * n] * xq[l];                \
+                    xq[j] = s / A[j + j * n];                                                       \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t e = CB( k + 1, C_BROWNUM ) - 1; e >= CB( k, C_BROWNUM ); e-- ) {          \
+                int64_t  b  = browtab[e];                                                           \
+                int64_t  s  = BK( b, B_LCBLK );                                                     \
+                int64_t  m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                \
+                int64_t  ns = CB( s, C_LCOL ) - CB( s, C_FCOL ) + 1;                                \
+                const T *Ab = L + CB( s, C_COEFOFF ) + BK( b, B_COEFIND );                          \
+                /* X_s -= A_b^T X_k(rows of b) */                                                   \
+                for ( int64_t q = 0; q < nrhs; q++ ) {                                              \
+                    const T *xk = X + BK( b, B_FROW ) + q * ldx;                                    \
+                    T       *xs = X + CB( s, C_FCOL ) + q * ldx;                                    \
+                    for ( int64_t c = 0; c < ns; c++ ) {                                            \
+                        T acc = 0;                                                                  \
+                        for ( int64_t r = 0; r < m; r++ ) acc += Ab[r + c * m] * xk[r];             \
+                        xs[c] -= acc;                                                               \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+static double dgemm_tr_wrap_dummy;
+static void dgemm_w( int tb, int64_t m, int64_t n, int64_t k, double al, const double *A, int64_t lda,
+                     const double *B, int64_t ldb, double be, double *C, int64_t ldc )
+{
+    orc_dgemm( tb, m, n, k, al, A, lda, B, ldb, be, C, ldc );
+}
+static void sgemm_w( int tb, int64_t m, int64_t n, int64_t k, float al, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float be, float *C, int64_t ldc )
+{
+    orc_sgemm( tb, m, n, k, al, A, lda, B, ldb, be, C, ldc );
+}
+static void dtrsm_w( int64_t m, int64_t n, const double *L, int64_t ldl, double *B, int64_t ldb )
+{
+    orc_dtrsm( m, n, L, ldl, B, ldb );
+}
+static void strsm_w( int64_t m, int64_t n, const float *L, int64_t ldl, float *B, int64_t ldb )
+{
+    orc_strsm( m, n, L, ldl, B, ldb );
+}
+
+ORC_TYPED( double, d, dgemm_w, dtrsm_w, sqrt, fabs )
+ORC_TYPED( float, s, sgemm_w, strsm_w, sqrtf, fabsf )
+
+/* ------------------------------------------------------------------------ */
+/* Static scheduler.                                                        */
+
+typedef struct sched_s {
+    int             nthreads;
+    int64_t         nc;
+    const int64_t  *cblk, *blok;
+    void           *L;
+    int             flt;
+    double          tau;
+    int64_t       **lists;
+    int64_t        *nlist;
+    _Atomic int64_t *ctrbcnt;
+    pthread_mutex_t *locks;
+    _Atomic int64_t  npiv;
+} sched_t;
+
+typedef struct { sched_t *s; int rank; } targ_t;
+
+static void
+cblk_task( sched_t *s, int64_t k )
+{
+    const int64_t *cblk = s->cblk, *blok = s->blok;
+    int64_t        n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1, piv = 0;
+    /* wait for the incoming contributions */
+    while ( atomic_load_explicit( s->ctrbcnt + k, memory_order_acquire ) > 0 ) {
+        sched_yield();
+    }
+    if ( s->flt == 3 ) {
+        double *L = (double *)s->L;
+        potrf_d( n, L + CB( k, C_COEFOFF ), n, &piv, s->tau );
+        trsm_cblk_d( cblk, blok, k, L );
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {
+            update_d( cblk, blok, k, b, L, s->locks );
+            atomic_fetch_sub_explicit( s->ctrbcnt + BK( b, B_FCBLK ), 1, memory_order_release );
+        }
+    }
+    else {
+        float *L = (float *)s->L;
+        potrf_s( n, L + CB( k, C_COEFOFF ), n, &piv, s->tau );
+        trsm_cblk_s( cblk, blok, k, L );
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {
+            update_s( cblk, blok, k, b, L, s->locks );
+            atomic_fetch_sub_explicit( s->ctrbcnt + BK( b, B_FCBLK ), 1, memory_order_release );
+        }
+    }
+    if ( piv ) {
+        atomic_fetch_add( &s->npiv, piv );
+    }
+}
+
+static void *
+thread_static( void *varg )
+{
+    targ_t  *a = (targ_t *)varg;
+    sched_t *s = a->s;
+    for ( int64_t i = 0; i < s->nlist[a->rank]; i++ ) {
+        cblk_task( s, s->lists[a->rank][i] );
+    }
+    return NULL;
+}
+
+static double
+cblk_cost( const int64_t *cblk, const int64_t *blok, int64_t k )
+{
+    double n = (double)( CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1 );
+    double m = (double)( CB( k, C_STRIDE ) ) - n;
+    double c = n * n * n / 3. + m * n * n;
+    for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {
+        double nb = (double)( BK( b, B_LROW ) - BK( b, B_FROW ) + 1 );
+        c += 2. * nb * m * n;
+        m -= nb;
+    }
+    return c + 1e4; /* plus a fixed task overhead */
+}
+
+/* Greedy list schedule in simulated time: a cblk becomes ready when its last
+ * contributor finishes; ready cblks go, in ready-time order, to the thread
+ * that frees first.  Every thread's list is in simulated start order, which
+ * keeps the runtime spin-waits deadlock free. */
+static void
+build_static_lists( sched_t *s )
+{
+    const int64_t *cblk = s->cblk, *blok = s->blok;
+    int64_t        nc = s->nc;
+    int            P  = s->nthreads;
+    double        *ready  = calloc( nc, sizeof( double ) );
+    double        *finish = calloc( nc, sizeof( double ) );
+    int64_t       *left   = calloc( nc, sizeof( int64_t ) );
+    double        *tfree  = calloc( P, sizeof( double ) );
+    int64_t       *heap   = malloc( ( nc > 0 ? nc : 1 ) * sizeof( int64_t ) );
+    int64_t        hn = 0;
+
+    for ( int64_t b = 0; b < CB( nc, C_FBLOK ); b++ ) {
+        if ( BK( b, B_FCBLK ) != BK( b, B_LCBLK ) ) left[BK( b, B_FCBLK )]++;
+    }
+    s->lists = malloc( P * sizeof( int64_t * ) );
+    s->nlist = calloc( P, sizeof( int64_t ) );
+    for ( int p = 0; p < P; p++ ) s->lists[p] = malloc( ( nc > 0 ? nc : 1 ) * sizeof( int64_t ) );
+
+#define HLESS( a, b ) ( ready[a] < ready[b] || ( ready[a] == ready[b] && ( a ) < ( b ) ) )
+#define HPUSH( x )                                                                                  \
+    do {                                                                                            \
+        int64_t i_ = hn++;                                                                          \
+        heap[i_]   = ( x );                                                                         \
+        while ( i_ > 0 && HLESS( heap[i_], heap[( i_ - 1 ) / 2] ) ) {                               \
+            int64_t t_ = heap[i_]; heap[i_] = heap[( i_ - 1 ) / 2]; heap[( i_ - 1 ) / 2] = t_;      \
+            i_ = ( i_ - 1 ) / 2;                                                                    \
+        }                                                                                           \
+    } while ( 0 )
+
+    for ( int64_t k = 0; k < nc; k++ ) {
+        if ( left[k] == 0 ) HPUSH( k );
+    }
+    while ( hn > 0 ) {
+        int64_t k = heap[0], i = 0;
+        heap[0] = heap[--hn];
+        for ( ;; ) {
+            int64_t l = 2 * i + 1, r = l + 1, m = i;
+            if ( l < hn && HLESS( heap[l], heap[m] ) ) m = l;
+            if ( r < hn && HLESS( heap[r], heap[m] ) ) m = r;
+            if ( m == i ) break;
+            int64_t t = heap[i]; heap[i] = heap[m]; heap[m] = t;
+            i = m;
+        }
+        int p = 0;
+        for ( int q = 1; q < P; q++ ) if ( tfree[q] < tfree[p] ) p = q;
+        double st = ( ready[k] > tfree[p] ) ? ready[k] : tfree[p];
+        finish[k] = st + cblk_cost( cblk, blok, k );
+        tfree[p]  = finish[k];
+        s->lists[p][s->nlist[p]++] = k;
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {
+            int64_t F = BK( b, B_FCBLK );
+            if ( finish[k] > ready[F] ) ready[F] = finish[k];
+            if ( --left[F] == 0 ) HPUSH( F );
+        }
+    }
+#undef HPUSH
+#undef HLESS
+    free( ready ); free( finish ); free( left ); free( tfree ); free( heap );
+}
+
+/*
+ * Fill + numerical Cholesky factorization.  flt: 3 = double, 2 = float.
+ * L (coefnbr entries of the factor type) is overwritten.  Returns 0.
+ */
+int
+orc_factorize( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+               int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const double *val,
+               double tau, void *L, int nthreads, int64_t *npiv, double *seconds )
+{
+    sched_t  s;
+    int64_t *col2cblk = malloc( ( n > 0 ? n : 1 ) * sizeof( int64_t ) );
+    double   t0;
+
+    orc_blas_init();
+    for ( int64_t k = 0; k < nc; k++ ) {
+        for ( int64_t j = CB( k, C_FCOL ); j <= CB( k, C_LCOL ); j++ ) col2cblk[j] = k;
+    }
+    memset( &s, 0, sizeof( s ) );
+    s.nthreads = nthreads > 0 ? nthreads : 1;
+    s.nc = nc;
+    s.cblk = cblk;
+    s.blok = blok;
+    s.L = L;
+    s.flt = flt;
+    s.tau = tau;
+    s.ctrbcnt = malloc( ( nc > 0 ? nc : 1 ) * sizeof( _Atomic int64_t ) );
+    s.locks   = malloc( ( nc > 0 ? nc : 1 ) * sizeof( pthread_mutex_t ) );
+    for ( int64_t k = 0; k < nc; k++ ) {
+        atomic_init( s.ctrbcnt + k, 0 );
+        pthread_mutex_init( s.locks + k, NULL );
+    }
+    for ( int64_t b = 0; b < CB( nc, C_FBLOK ); b++ ) {
+        if ( BK( b, B_FCBLK ) != BK( b, B_LCBLK ) ) atomic_fetch_add( s.ctrbcnt + BK( b, B_FCBLK ), 1 );
+    }
+    atomic_init( &s.npiv, 0 );
+    build_static_lists( &s );
+
+    t0 = wtime();
+    if ( flt == 3 ) {
+        memset( L, 0, coefnbr * sizeof( double ) );
+        fill_d( n, cblk, blok, colptr, rowtab, val, col2cblk, (double *)L );
+    }
+    else {
+        memset( L, 0, coefnbr * sizeof( float ) );
+        fill_s( n, cblk, blok, colptr, rowtab, val, col2cblk, (float *)L );
+    }
+    {
+        pthread_t *th = malloc( s.nthreads * sizeof( pthread_t ) );
+        targ_t    *ta = malloc( s.nthreads * sizeof( targ_t ) );
+        for ( int p = 0; p < s.nthreads; p++ ) {
+            ta[p].s    = &s;
+            ta[p].rank = p;
+            if ( p > 0 ) pthread_create( th + p, NULL, thread_static, ta + p );
+        }
+        thread_static( ta );
+        for ( int p = 1; p < s.nthreads; p++ ) pthread_join( th[p], NULL );
+        free( th );
+        free( ta );
+    }
+    if ( seconds ) *seconds = wtime() - t0;
+    if ( npiv ) *npiv = atomic_load( &s.npiv );
+
+    for ( int p = 0; p < s.nthreads; p++ ) free( s.lists[p] );
+    free( s.lists );
+    free( s.nlist );
+    for ( int64_t k = 0; k < nc; k++ ) pthread_mutex_destroy( s.locks + k );
+    free( s.locks );
+    free( (void *)s.ctrbcnt );
+    free( col2cblk );
+    return 0;
+}
+
+/* x <- A^{-1} x with the factor (double x, factor in flt). */
+int
+orc_solve( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+           const int64_t *browtab, const void *L, int64_t nrhs, double *x, int64_t ldx )
+{
+    orc_blas_init();
+    if ( flt == 3 ) {
+        solve_d( nc, cblk, blok, browtab, (const double *)L, nrhs, x, ldx );
+    }
+    else {
+        float *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
+        solve_s( nc, cblk, blok, browtab, (const float *)L, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
+    return 0;
+}
+
+/* y = alpha A x + beta y on the full-pattern CSC of a symmetric matrix
+ * (column j is row j), or on the transposed CSC of a general one. */
+int
+orc_spmv( int64_t n, const int64_t *colptr, const int64_t *rowtab, const double *val, double alpha,
+          const double *x, double beta, double *y )
+{
+    for ( int64_t j = 0; j < n; j++ ) {
+        double s = 0.;
+        for ( int64_t e = colptr[j]; e < colptr[j + 1]; e++ ) s += val[e] * x[rowtab[e]];
+        y[j] = alpha * s + ( beta == 0. ? 0. : beta * y[j] );
+    }
+    return 0;
+}
+
+static double
+nrm2( int64_t n, const double *x )
+{
+    double s = 0.;
+    for ( int64_t i = 0; i < n; i++ ) s += x[i] * x[i];
+    return sqrt( s );
+}
+
+/*
+ * GMRES(im) right-preconditioned by the factor, one right-hand side, as
+ * z_gmres_smp: modified Gram-Schmidt, Givens rotations, stop on
+ * ||r||_2/||b||_2 <= eps.  Returns the number of iterations.
+ */
+int64_t
+orc_gmres( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+           const int64_t *browtab, const void *L, const int64_t *colptr, const int64_t *rowtab,
+           const double *val, const double *b, double *x, int64_t im, int64_t itermax, double eps )
+{
+    int64_t im1 = im + 1, iters = 0, i;
+    double *V = malloc( n * im1 * sizeof( double ) ), *W = malloc( n * im * sizeof( double ) );
+    double *H = calloc( im * im1, sizeof( double ) ), *cs = calloc( im, sizeof( double ) );
+    double *sn = calloc( im, sizeof( double ) ), *g = calloc( im1, sizeof( double ) );
+    double  normb = nrm2( n, b ), resid;
+    if ( normb == 0. ) normb = 1.;
+    for ( ;; ) {
+        memcpy( V, b, n * sizeof( double ) );
+        orc_spmv( n, colptr, rowtab, val, -1., x, 1., V );
+        resid = nrm2( n, V );
+        if ( resid / normb <= eps ) break;
+        for ( int64_t k = 0; k < n; k++ ) V[k] /= resid;
+        memset( g, 0, im1 * sizeof( double ) );
+        g[0] = resid;
+        for ( i = 0; i < im && iters < itermax; ) {
+            double *vi = V + i * n, *vi1 = V + ( i + 1 ) * n, *wi = W + i * n, *Hi = H + i * im1, t;
+            memcpy( wi, vi, n * sizeof( double ) );
+            orc_solve( flt, n, nc, cblk, blok, browtab, L, 1, wi, n );
+            orc_spmv( n, colptr, rowtab, val, 1., wi, 0., vi1 );
+            for ( int64_t j = 0; j <= i; j++ ) {
+                double h = 0.;
+                for ( int64_t k = 0; k < n; k++ ) h += vi1[k] * V[j * n + k];
+                Hi[j] = h;
+                for ( int64_t k = 0; k < n; k++ ) vi1[k] -= h * V[j * n + k];
+            }
+            Hi[i + 1] = nrm2( n, vi1 );
+            if ( Hi[i + 1] > 1e-50 ) {
+                for ( int64_t k = 0; k < n; k++ ) vi1[k] /= Hi[i + 1];
+            }
+            for ( int64_t j = 0; j < i; j++ ) {
+                t         = Hi[j];
+                Hi[j]     = cs[j] * t + sn[j] * Hi[j + 1];
+                Hi[j + 1] = cs[j] * Hi[j + 1] - sn[j] * t;
+            }
+            t = sqrt( Hi[i] * Hi[i] + Hi[i + 1] * Hi[i + 1] );
+            if ( fabs( t ) <= eps ) t = eps;
+            cs[i]    = Hi[i] / t;
+            sn[i]    = Hi[i + 1] / t;
+            g[i + 1] = -sn[i] * g[i];
+            g[i]     = cs[i] * g[i];
+            Hi[i]    = cs[i] * Hi[i] + sn[i] * Hi[i + 1];
+            i++;
+            iters++;
+            if ( fabs( g[i] ) / normb <= eps ) break;
+        }
+        for ( int64_t k = i - 1; k >= 0; k-- ) {
+            double s = g[k];
+            for ( int64_t l = k + 1; l < i; l++ ) s -= H[l * im1 + k] * g[l];
+            g[k] = s / H[k * im1 + k];
+        }
+        for ( int64_t j = 0; j < i; j++ )
+            for ( int64_t k = 0; k < n; k++ ) x[k] += W[j * n + k] * g[j];
+        if ( iters >= itermax ) break;
+    }
+    free( V ); free( W ); free( H ); free( cs ); free( sn ); free( g );
+    (void)dgemm_tr_wrap_dummy;
+    return iters;
+}

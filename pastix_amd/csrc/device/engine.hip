@@ -1,0 +1,869 @@
+// engine.hip -- the single-GPU numerical engine behind pastix_amd_device.h.
+//
+// Scheduling.  The reference runs the cblk tasks of blend's static list on
+// CPU threads that spin on dependency counters (sopalin/sequential_zpotrf.c:
+// 102-147, kernels/cpucblk_zmpi_coeftab.c:541-622).  On the GPU the same task
+// graph is walked by dependency levels: level(k) = 1 + max level of the cblks
+// that update k (browtab), so every cblk of a level is ready once the previous
+// levels are done.  A level is three batched launches: diagonal panels, the
+// off-diagonal TRSM tiles, and the update tiles of all its cblks.  Updates of
+// one level that can touch the same facing-cblk columns (two sources, one
+// fcblk, overlapping column ranges) use atomic adds; all other updates write
+// with plain read-modify-write, which is what the reference's per-cblk lock
+// serializes (kernels/core_zgemmsp.c:462,470).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "kernels_solve.hpp"
+
+namespace {
+
+enum { FT_DOUBLE = 3, FT_FLOAT = 2, FT_CPLX64 = 5, FT_CPLX32 = 4 };
+enum { FACT_POTRF = 0, FACT_SYTRF = 1, FACT_GETRF = 2 };
+
+struct level_t {
+    int64_t c0, c1;     // cblk list
+    int64_t t0, t1;     // trsm tasks (32-row tiles)
+    int64_t u0, u1;     // update tasks
+    int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
+    int32_t maxn;
+};
+
+}  // namespace
+
+struct pxd_ctx_s {
+    int         device = 0;
+    hipStream_t stream = nullptr;
+    size_t      lds_max = 65536;
+
+    // structure
+    int64_t n = 0, cblknbr = 0, bloknbr = 0, coefnbr = 0, dinvnbr = 0;
+    int32_t colmax = 0;
+    std::vector<dcblk> hcb;
+    std::vector<dblok> hbk;
+    dcblk   *dcb = nullptr;
+    dblok   *dbk = nullptr;
+    int32_t *dcol2cblk = nullptr;
+
+    // schedule
+    std::vector<level_t> levels;
+    int32_t  *dlist = nullptr;
+    row_task *dtrsm = nullptr;
+    upd_task *dupd  = nullptr;
+    row_task *dsolv = nullptr;
+    int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
+
+    // matrix
+    int      mflt = 0, mtx = 0;
+    int64_t  nnz = 0;
+    int64_t *dcolptr = nullptr, *drowtab = nullptr, *dtcolptr = nullptr, *dtrowtab = nullptr;
+    void    *dval = nullptr, *dtval = nullptr;
+    std::vector<int64_t> hcolptr, hrowtab;   // kept for the fill of the U side
+    double   norm_inf = 0.;
+
+    // factor
+    int    fflt = 0, facto = -1;
+    void  *dL = nullptr, *dU = nullptr, *ddinv = nullptr, *dudinv = nullptr;
+    size_t coef_bytes = 0;
+    int   *dscratch = nullptr;     // counters (pivots, overflow)
+    void  *dvecs = nullptr;        // solve scratch in the factor precision
+    size_t dvecs_bytes = 0;
+    void  *dred = nullptr;         // reduction scratch
+
+    // rows per TRSM tile (32, or 16 when the LDS cannot hold 32 x colmax)
+    int          trsm_rt = 32;
+
+    // options / stats
+    int          timing = 0;
+    pxd_stats_t  stats{};
+};
+
+namespace {
+
+template <typename T> int flt_of();
+template <> int flt_of<double>() { return FT_DOUBLE; }
+template <> int flt_of<float>() { return FT_FLOAT; }
+
+size_t
+elt_size( int flt )
+{
+    switch ( flt ) {
+    case FT_FLOAT: return 4;
+    case FT_DOUBLE: return 8;
+    case FT_CPLX32: return 8;
+    case FT_CPLX64: return 16;
+    }
+    return 0;
+}
+
+template <typename T>
+int
+upload( T **d, const std::vector<T> &h )
+{
+    if ( *d ) {
+        PXD_CHECK( hipFree( *d ) );
+        *d = nullptr;
+    }
+    size_t bytes = std::max<size_t>( 1, h.size() ) * sizeof( T );
+    PXD_CHECK( hipMalloc( (void **)d, bytes ) );
+    if ( !h.empty() ) {
+        PXD_CHECK( hipMemcpy( *d, h.data(), h.size() * sizeof( T ), hipMemcpyHostToDevice ) );
+    }
+    return 0;
+}
+
+inline unsigned
+nblocks( int64_t n, int bs )
+{
+    return (unsigned)( ( n + bs - 1 ) / bs );
+}
+
+// Build the level schedule of the factorization and the solve.
+int
+build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks,
+                const int64_t *browtab )
+{
+    const int64_t        nc = ctx->cblknbr;
+    std::vector<int32_t> level( nc, 0 );
+    int32_t              nlev = 0;
+
+    for ( int64_t k = 0; k < nc; k++ ) {
+        int32_t l = 0;
+        for ( int64_t e = cblks[k].brownum; e < cblks[k + 1].brownum; e++ ) {
+            int64_t src = bloks[browtab[e]].lcblknm;
+            l = std::max( l, level[src] + 1 );
+        }
+        level[k] = l;
+        nlev     = std::max( nlev, l + 1 );
+    }
+
+    std::vector<std::vector<int32_t>> bylev( nlev );
+    for ( int64_t k = 0; k < nc; k++ ) {
+        bylev[level[k]].push_back( (int32_t)k );
+    }
+
+    std::vector<int32_t>  list;
+    std::vector<row_task> trsm, solv;
+    std::vector<upd_task> upd;
+    ctx->levels.assign( nlev, level_t{} );
+
+    struct iv { int32_t F, f, l; int64_t first, last; };
+    std::vector<iv> ivs;
+
+    for ( int32_t L = 0; L < nlev; L++ ) {
+        level_t &lv = ctx->levels[L];
+        lv.c0   = list.size();
+        lv.t0   = trsm.size();
+        lv.u0   = upd.size();
+        lv.f0   = solv.size();
+        lv.maxn = 0;
+        ivs.clear();
+        for ( int32_t k : bylev[L] ) {
+            const dcblk &c  = ctx->hcb[k];
+            int32_t      nk = c.lcolnum - c.fcolnum + 1;
+            lv.maxn = std::max( lv.maxn, nk );
+            list.push_back( k );
+            for ( int32_t r0 = nk; r0 < c.stride; r0 += ctx->trsm_rt ) {
+                trsm.push_back( row_task{ k, r0 } );
+            }
+            for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
+                solv.push_back( row_task{ k, r0 } );
+            }
+            int32_t lb = ctx->hcb[k + 1].fblok;
+            for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
+                const dblok &bb = ctx->hbk[b];
+                int32_t      N  = bb.lrownum - bb.frownum + 1;
+                int32_t      M  = c.stride - bb.rowoff;
+                int64_t      first = upd.size();
+                for ( int32_t tn = 0; tn * 64 < N; tn++ ) {
+                    for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
+                        upd.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
+                    }
+                }
+                ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, first, (int64_t)upd.size() } );
+            }
+        }
+        // atomic flags for updates whose facing columns overlap another source's
+        std::sort( ivs.begin(), ivs.end(), []( const iv &a, const iv &b ) {
+            return ( a.F != b.F ) ? ( a.F < b.F ) : ( a.f < b.f );
+        } );
+        for ( size_t i = 0; i < ivs.size(); ) {
+            size_t j = i;
+            while ( j < ivs.size() && ivs[j].F == ivs[i].F ) {
+                j++;
+            }
+            int32_t maxl = -1;
+            for ( size_t p = i; p < j; p++ ) {
+                bool conflict = ( maxl >= ivs[p].f ) || ( p + 1 < j && ivs[p + 1].f <= ivs[p].l );
+                maxl = std::max( maxl, ivs[p].l );
+                if ( conflict ) {
+                    for ( int64_t u = ivs[p].first; u < ivs[p].last; u++ ) {
+                        upd[u].flags |= 1;
+                    }
+                }
+            }
+            i = j;
+        }
+        lv.c1 = list.size();
+        lv.t1 = trsm.size();
+        lv.u1 = upd.size();
+        lv.f1 = solv.size();
+    }
+    ctx->nupd  = upd.size();
+    ctx->ntrsm = trsm.size();
+    ctx->nsolv = solv.size();
+    ctx->stats.levels       = nlev;
+    ctx->stats.update_tiles = upd.size();
+    if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
+         upload( &ctx->dsolv, solv ) ) {
+        return 6;
+    }
+    return 0;
+}
+
+// Elapsed milliseconds between two events (the stream must be synchronized).
+float
+ev_ms( hipEvent_t a, hipEvent_t b )
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime( &ms, a, b );
+    return ms;
+}
+
+template <typename T>
+size_t
+trsm_lds( int32_t n, int rt )
+{
+    return (size_t)n * ( rt + 1 ) * sizeof( T );
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+int
+factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
+{
+    T *L  = (T *)ctx->dL;
+    T *DI = (T *)ctx->ddinv;
+    hipStream_t s = ctx->stream;
+    hipEvent_t  e0, e1, ea, eb;
+    PXD_CHECK( hipEventCreate( &e0 ) );
+    PXD_CHECK( hipEventCreate( &e1 ) );
+    PXD_CHECK( hipEventCreate( &ea ) );
+    PXD_CHECK( hipEventCreate( &eb ) );
+    double t_panel = 0., t_trsm = 0., t_upd = 0.;
+    int64_t launches = 0;
+
+    PXD_CHECK( hipMemsetAsync( ctx->dscratch, 0, 4 * sizeof( int ), s ) );
+    PXD_CHECK( hipEventRecord( e0, s ) );
+
+    // fill: zero + scatter of the lower triangle of P A P^T
+    PXD_CHECK( hipMemsetAsync( L, 0, ctx->coefnbr * sizeof( T ), s ) );
+    double ovf = ( sizeof( T ) == 4 ) ? (double)FLT_MAX : DBL_MAX;
+    hipLaunchKernelGGL( ( k_fill<T, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
+                        ctx->n, ctx->dcolptr, ctx->drowtab, (const double *)ctx->dval, ctx->dcol2cblk,
+                        ctx->dcb, ctx->dbk, L, 0, ovf, ctx->dscratch + 1 );
+    launches += 2;
+    if ( ctx->timing ) {
+        PXD_CHECK( hipEventRecord( ea, s ) );
+        PXD_CHECK( hipEventSynchronize( ea ) );
+        ctx->stats.fill_seconds = 1e-3 * ev_ms( e0, ea );
+    }
+
+    for ( const level_t &lv : ctx->levels ) {
+        unsigned np = (unsigned)( lv.c1 - lv.c0 );
+        unsigned nt = (unsigned)( lv.t1 - lv.t0 );
+        unsigned nu = (unsigned)( lv.u1 - lv.u0 );
+        if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
+        hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
+                            ctx->dcb, L, DI, thresh, ctx->dscratch );
+        launches++;
+        if ( ctx->timing ) {
+            PXD_CHECK( hipEventRecord( eb, s ) );
+            PXD_CHECK( hipEventSynchronize( eb ) );
+            t_panel += 1e-3 * ev_ms( ea, eb );
+        }
+        if ( nt ) {
+            if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
+            if ( ctx->trsm_rt == 32 ) {
+                hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ),
+                                    s, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_trsm<T, 16> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 16 ),
+                                    s, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+            }
+            launches++;
+            if ( ctx->timing ) {
+                PXD_CHECK( hipEventRecord( eb, s ) );
+                PXD_CHECK( hipEventSynchronize( eb ) );
+                t_trsm += 1e-3 * ev_ms( ea, eb );
+            }
+        }
+        if ( nu ) {
+            if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
+            hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s, ctx->dupd + lv.u0,
+                                ctx->dcb, ctx->dbk, L );
+            launches++;
+            if ( ctx->timing ) {
+                PXD_CHECK( hipEventRecord( eb, s ) );
+                PXD_CHECK( hipEventSynchronize( eb ) );
+                t_upd += 1e-3 * ev_ms( ea, eb );
+            }
+        }
+    }
+    PXD_CHECK( hipGetLastError() );
+    PXD_CHECK( hipEventRecord( e1, s ) );
+    PXD_CHECK( hipEventSynchronize( e1 ) );
+    ctx->stats.fact_seconds   = 1e-3 * ev_ms( e0, e1 );
+    ctx->stats.panel_seconds  = t_panel;
+    ctx->stats.trsm_seconds   = t_trsm;
+    ctx->stats.update_seconds = t_upd;
+    ctx->stats.launches       = launches;
+    int h[4];
+    PXD_CHECK( hipMemcpy( h, ctx->dscratch, 4 * sizeof( int ), hipMemcpyDeviceToHost ) );
+    *nbpivots = h[0];
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    (void)hipEventDestroy( ea );
+    (void)hipEventDestroy( eb );
+    if ( h[1] > 0 ) {
+        fprintf( stderr, "pastix_amd device: %d matrix entries overflow the factor precision\n", h[1] );
+        return 7; /* PASTIX_ERR_BADPARAMETER */
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+int
+solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
+{
+    const T    *L  = (const T *)ctx->dL;
+    const T    *DI = (const T *)ctx->ddinv;
+    hipStream_t s  = ctx->stream;
+    const int   nl = (int)ctx->levels.size();
+
+    // forward: L y = b
+    for ( int li = 0; li < nl; li++ ) {
+        const level_t &lv = ctx->levels[li];
+        unsigned       np = (unsigned)( lv.c1 - lv.c0 );
+        unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+        size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
+        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
+        if ( nf ) {
+            hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+        }
+    }
+    // backward: L^T x = y
+    for ( int li = nl - 1; li >= 0; li-- ) {
+        const level_t &lv = ctx->levels[li];
+        unsigned       np = (unsigned)( lv.c1 - lv.c0 );
+        unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+        size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
+        if ( nf ) {
+            hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+        }
+        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+template <typename T>
+void
+set_lds_attributes( size_t lds )
+{
+    (void)hipFuncSetAttribute( (const void *)k_trsm<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int
+pxd_device_count( void )
+{
+    int n = 0;
+    if ( hipGetDeviceCount( &n ) != hipSuccess ) {
+        return 0;
+    }
+    return n;
+}
+
+int
+pxd_create( pxd_ctx_t **pctx, int device )
+{
+    int ndev = pxd_device_count();
+    if ( ndev <= 0 ) {
+        fprintf( stderr, "pastix_amd device: no GPU visible; the numerical steps run only on the GPU\n" );
+        return 3; /* PASTIX_ERR_NOTIMPLEMENTED */
+    }
+    if ( device < 0 || device >= ndev ) {
+        return 7;
+    }
+    pxd_ctx_t *ctx = new pxd_ctx_t();
+    ctx->device = device;
+    PXD_CHECK( hipSetDevice( device ) );
+    PXD_CHECK( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
+    hipDeviceProp_t prop;
+    PXD_CHECK( hipGetDeviceProperties( &prop, device ) );
+    ctx->lds_max = prop.sharedMemPerBlock;
+    PXD_CHECK( hipMalloc( (void **)&ctx->dscratch, 64 * sizeof( int ) ) );
+    PXD_CHECK( hipMalloc( &ctx->dred, 4096 ) );
+    set_lds_attributes<double>( ctx->lds_max );
+    set_lds_attributes<float>( ctx->lds_max );
+    *pctx = ctx;
+    return 0;
+}
+
+void
+pxd_destroy( pxd_ctx_t *ctx )
+{
+    if ( ctx == nullptr ) {
+        return;
+    }
+    (void)hipSetDevice( ctx->device );
+    (void)hipStreamSynchronize( ctx->stream );
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dsolv,
+                     ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
+                     ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
+    for ( void *p : ptrs ) {
+        if ( p ) {
+            (void)hipFree( p );
+        }
+    }
+    (void)hipStreamDestroy( ctx->stream );
+    delete ctx;
+}
+
+int
+pxd_sync( pxd_ctx_t *ctx )
+{
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats )
+{
+    *stats = ctx->stats;
+    return 0;
+}
+
+int
+pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
+{
+    std::string k( key );
+    if ( k == "timing" ) {
+        ctx->timing = ( value != 0. );
+        return 0;
+    }
+    return 7;
+}
+
+int
+pxd_malloc( pxd_ctx_t *ctx, size_t bytes, void **dptr )
+{
+    (void)ctx;
+    PXD_CHECK( hipMalloc( dptr, std::max<size_t>( bytes, 1 ) ) );
+    return 0;
+}
+
+int
+pxd_free( pxd_ctx_t *ctx, void *dptr )
+{
+    (void)ctx;
+    if ( dptr ) {
+        PXD_CHECK( hipFree( dptr ) );
+    }
+    return 0;
+}
+
+int
+pxd_memset0( pxd_ctx_t *ctx, void *dptr, size_t bytes )
+{
+    PXD_CHECK( hipMemsetAsync( dptr, 0, bytes, ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_h2d( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes )
+{
+    PXD_CHECK( hipMemcpyAsync( dst, src, bytes, hipMemcpyHostToDevice, ctx->stream ) );
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_d2h( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes )
+{
+    PXD_CHECK( hipMemcpyAsync( dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream ) );
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
+                   const pxd_cblk_t *cblks, const pxd_blok_t *bloks, const int64_t *browtab,
+                   int64_t coefnbr )
+{
+    if ( n >= INT32_MAX || cblknbr >= INT32_MAX || bloknbr >= INT32_MAX ) {
+        return 3;
+    }
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    ctx->n       = n;
+    ctx->cblknbr = cblknbr;
+    ctx->bloknbr = bloknbr;
+    ctx->coefnbr = coefnbr;
+    ctx->hcb.resize( cblknbr + 1 );
+    ctx->hbk.resize( bloknbr + 1 );
+    ctx->colmax = 0;
+    int64_t dinv = 0;
+    std::vector<int32_t> col2cblk( std::max<int64_t>( n, 1 ) );
+    for ( int64_t k = 0; k <= cblknbr; k++ ) {
+        dcblk &c = ctx->hcb[k];
+        c.fcolnum = (int32_t)cblks[k].fcolnum;
+        c.lcolnum = (int32_t)cblks[k].lcolnum;
+        c.fblok   = (int32_t)cblks[k].fblok;
+        c.stride  = (int32_t)cblks[k].stride;
+        c.coefoff = cblks[k].coefoff;
+        c.dinvoff = dinv;
+        if ( k < cblknbr ) {
+            int32_t nk = c.lcolnum - c.fcolnum + 1;
+            ctx->colmax = std::max( ctx->colmax, nk );
+            dinv += (int64_t)( ( nk + PXD_NB - 1 ) / PXD_NB ) * PXD_NB * PXD_NB;
+            for ( int32_t j = c.fcolnum; j <= c.lcolnum; j++ ) {
+                col2cblk[j] = (int32_t)k;
+            }
+        }
+    }
+    ctx->dinvnbr = dinv;
+    for ( int64_t b = 0; b <= bloknbr; b++ ) {
+        dblok &bb  = ctx->hbk[b];
+        bb.frownum = (int32_t)bloks[b].frownum;
+        bb.lrownum = (int32_t)bloks[b].lrownum;
+        bb.fcblknm = (int32_t)bloks[b].fcblknm;
+        if ( b < bloknbr ) {
+            int64_t lc = bloks[b].lcblknm;
+            int64_t nk = cblks[lc].lcolnum - cblks[lc].fcolnum + 1;
+            bb.rowoff = (int32_t)( bloks[b].coefind / nk );
+        }
+        else {
+            bb.rowoff = 0;
+        }
+    }
+    // the trsm tile keeps RT rows x n columns in LDS (static LDS of the kernel ~9 KB)
+    ctx->trsm_rt = ( trsm_lds<double>( ctx->colmax, 32 ) + 12288 <= ctx->lds_max ) ? 32 : 16;
+    if ( trsm_lds<double>( ctx->colmax, 16 ) + 12288 > ctx->lds_max ) {
+        fprintf( stderr, "pastix_amd device: cblk width %d exceeds the LDS budget of the TRSM tile\n",
+                 ctx->colmax );
+        return 3;
+    }
+    if ( upload( &ctx->dcb, ctx->hcb ) || upload( &ctx->dbk, ctx->hbk ) || upload( &ctx->dcol2cblk, col2cblk ) ) {
+        return 6;
+    }
+    return build_schedule( ctx, cblks, bloks, browtab );
+}
+
+int
+pxd_set_matrix( pxd_ctx_t *ctx, int flttype, int mtxtype, int64_t n, int64_t nnz,
+                const int64_t *colptr, const int64_t *rowtab, const void *values,
+                const int64_t *tcolptr, const int64_t *trowtab, const void *tvalues )
+{
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    if ( flttype != FT_DOUBLE && flttype != FT_CPLX64 ) {
+        return 7;
+    }
+    if ( n != ctx->n ) {
+        return 7;
+    }
+    ctx->mflt = flttype;
+    ctx->mtx  = mtxtype;
+    ctx->nnz  = nnz;
+    size_t es = elt_size( flttype );
+    void *ptrs[] = { ctx->dcolptr, ctx->drowtab, ctx->dval, ctx->dtcolptr, ctx->dtrowtab, ctx->dtval };
+    for ( void *p : ptrs ) {
+        if ( p ) PXD_CHECK( hipFree( p ) );
+    }
+    ctx->dcolptr = ctx->drowtab = ctx->dtcolptr = ctx->dtrowtab = nullptr;
+    ctx->dval = ctx->dtval = nullptr;
+    PXD_CHECK( hipMalloc( (void **)&ctx->dcolptr, ( n + 1 ) * sizeof( int64_t ) ) );
+    PXD_CHECK( hipMalloc( (void **)&ctx->drowtab, std::max<int64_t>( nnz, 1 ) * sizeof( int64_t ) ) );
+    PXD_CHECK( hipMalloc( &ctx->dval, std::max<int64_t>( nnz, 1 ) * es ) );
+    PXD_CHECK( hipMemcpy( ctx->dcolptr, colptr, ( n + 1 ) * sizeof( int64_t ), hipMemcpyHostToDevice ) );
+    PXD_CHECK( hipMemcpy( ctx->drowtab, rowtab, nnz * sizeof( int64_t ), hipMemcpyHostToDevice ) );
+    PXD_CHECK( hipMemcpy( ctx->dval, values, nnz * es, hipMemcpyHostToDevice ) );
+    if ( tcolptr ) {
+        PXD_CHECK( hipMalloc( (void **)&ctx->dtcolptr, ( n + 1 ) * sizeof( int64_t ) ) );
+        PXD_CHECK( hipMalloc( (void **)&ctx->dtrowtab, std::max<int64_t>( nnz, 1 ) * sizeof( int64_t ) ) );
+        PXD_CHECK( hipMalloc( &ctx->dtval, std::max<int64_t>( nnz, 1 ) * es ) );
+        PXD_CHECK( hipMemcpy( ctx->dtcolptr, tcolptr, ( n + 1 ) * sizeof( int64_t ), hipMemcpyHostToDevice ) );
+        PXD_CHECK( hipMemcpy( ctx->dtrowtab, trowtab, nnz * sizeof( int64_t ), hipMemcpyHostToDevice ) );
+        PXD_CHECK( hipMemcpy( ctx->dtval, tvalues, nnz * es, hipMemcpyHostToDevice ) );
+    }
+    // ||A||_inf on the host copy (rows of P A P^T = columns of the transposed storage)
+    {
+        std::vector<double> rs( n, 0. );
+        for ( int64_t e = 0; e < nnz; e++ ) {
+            double a;
+            if ( flttype == FT_DOUBLE ) {
+                a = std::fabs( ( (const double *)values )[e] );
+            }
+            else {
+                const double *z = (const double *)values + 2 * e;
+                a = std::hypot( z[0], z[1] );
+            }
+            rs[rowtab[e]] += a;
+        }
+        ctx->norm_inf = 0.;
+        for ( double v : rs ) ctx->norm_inf = std::max( ctx->norm_inf, v );
+    }
+    return 0;
+}
+
+int
+pxd_matrix_norm_inf( pxd_ctx_t *ctx, double *norm )
+{
+    *norm = ctx->norm_inf;
+    return 0;
+}
+
+int
+pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double threshold, int64_t *nbpivots )
+{
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    if ( ctx->dval == nullptr || ctx->dcb == nullptr ) {
+        return 7;
+    }
+    if ( factotype != FACT_POTRF ) {
+        return 3;
+    }
+    if ( factor_flttype != FT_DOUBLE && factor_flttype != FT_FLOAT ) {
+        return 3;
+    }
+    size_t es    = elt_size( factor_flttype );
+    size_t bytes = std::max<int64_t>( ctx->coefnbr, 1 ) * es;
+    if ( ctx->dL == nullptr || ctx->coef_bytes != bytes || ctx->fflt != factor_flttype ) {
+        if ( ctx->dL ) PXD_CHECK( hipFree( ctx->dL ) );
+        if ( ctx->ddinv ) PXD_CHECK( hipFree( ctx->ddinv ) );
+        ctx->dL = ctx->ddinv = nullptr;
+        PXD_CHECK( hipMalloc( &ctx->dL, bytes ) );
+        PXD_CHECK( hipMalloc( &ctx->ddinv, std::max<int64_t>( ctx->dinvnbr, 1 ) * es ) );
+        ctx->coef_bytes = bytes;
+    }
+    ctx->fflt  = factor_flttype;
+    ctx->facto = factotype;
+    ctx->stats.coef_bytes = bytes;
+    if ( factor_flttype == FT_DOUBLE ) {
+        return factorize_potrf<double>( ctx, threshold, nbpivots );
+    }
+    return factorize_potrf<float>( ctx, threshold, nbpivots );
+}
+
+int
+pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes )
+{
+    void *src = ( side == 0 ) ? ctx->dL : ctx->dU;
+    if ( src == nullptr || bytes > ctx->coef_bytes ) {
+        return 7;
+    }
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    PXD_CHECK( hipMemcpy( host, src, bytes, hipMemcpyDeviceToHost ) );
+    return 0;
+}
+
+int
+pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
+{
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    if ( ctx->dL == nullptr ) {
+        return 7;
+    }
+    hipEvent_t e0, e1;
+    PXD_CHECK( hipEventCreate( &e0 ) );
+    PXD_CHECK( hipEventCreate( &e1 ) );
+    PXD_CHECK( hipEventRecord( e0, ctx->stream ) );
+    int rc = 0;
+    if ( ctx->fflt == FT_DOUBLE ) {
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
+            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+            rc = solve_sweeps<double>( ctx, (double *)dx + q0 * ldx, ldx, nq );
+        }
+    }
+    else {
+        // single precision factor: convert the vectors down, sweep, convert back
+        size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, PXD_MAXRHS ) * sizeof( float );
+        if ( ctx->dvecs_bytes < need ) {
+            if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
+            PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
+            ctx->dvecs_bytes = need;
+        }
+        float *xs = (float *)ctx->dvecs;
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
+            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+            for ( int q = 0; q < nq; q++ ) {
+                hipLaunchKernelGGL( ( k_convert<float, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0,
+                                    ctx->stream, ctx->n, (const double *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
+            }
+            rc = solve_sweeps<float>( ctx, xs, ctx->n, nq );
+            for ( int q = 0; q < nq; q++ ) {
+                hipLaunchKernelGGL( ( k_convert<double, float> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0,
+                                    ctx->stream, ctx->n, xs + q * ctx->n, (double *)dx + ( q0 + q ) * ldx );
+            }
+        }
+    }
+    PXD_CHECK( hipEventRecord( e1, ctx->stream ) );
+    PXD_CHECK( hipEventSynchronize( e1 ) );
+    ctx->stats.solve_seconds = 1e-3 * ev_ms( e0, e1 );
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    return rc;
+}
+
+int
+pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy, int64_t nrhs, int64_t ld )
+{
+    const int64_t *cp = ctx->dcolptr, *rt = ctx->drowtab;
+    const void    *vv = ctx->dval;
+    if ( ctx->mtx == 111 /* general: rows of A are the columns of the transpose */ ) {
+        cp = ctx->dtcolptr;
+        rt = ctx->dtrowtab;
+        vv = ctx->dtval;
+    }
+    if ( ctx->mflt == FT_DOUBLE ) {
+        hipLaunchKernelGGL( ( k_spmv<double, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
+                            ctx->n, cp, rt, (const double *)vv, (const double *)dx, (double *)dy, ld, (int)nrhs,
+                            alpha, beta );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_spmv<pxcd, pxcd> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
+                            ctx->n, cp, rt, (const pxcd *)vv, (const pxcd *)dx, (pxcd *)dy, ld, (int)nrhs,
+                            alpha, beta );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+static unsigned
+red_blocks( int64_t n )
+{
+    return (unsigned)std::min<int64_t>( 1024, std::max<int64_t>( 1, ( n + 255 ) / 256 ) );
+}
+
+int
+pxd_vec_dot( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, const void *y, void *dres )
+{
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_zero<pxcd> ), dim3( 1 ), dim3( 64 ), 0, ctx->stream, (pxcd *)dres, nrhs );
+        hipLaunchKernelGGL( ( k_dot<pxcd> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const pxcd *)x, (const pxcd *)y, (pxcd *)dres );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_zero<double> ), dim3( 1 ), dim3( 64 ), 0, ctx->stream, (double *)dres, nrhs );
+        hipLaunchKernelGGL( ( k_dot<double> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const double *)x, (const double *)y, (double *)dres );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+int
+pxd_vec_nrm2( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, double *dres )
+{
+    hipLaunchKernelGGL( ( k_zero<double> ), dim3( 1 ), dim3( 64 ), 0, ctx->stream, dres, nrhs );
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_nrm2sq<pxcd> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const pxcd *)x, dres );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_nrm2sq<double> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const double *)x, dres );
+    }
+    hipLaunchKernelGGL( k_sqrt_inplace, dim3( 1 ), dim3( 64 ), 0, ctx->stream, dres, (int)nrhs );
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+int
+pxd_vec_maxabs( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, double *dres )
+{
+    hipLaunchKernelGGL( ( k_zero<double> ), dim3( 1 ), dim3( 64 ), 0, ctx->stream, dres, nrhs );
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_maxabs<pxcd> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const pxcd *)x, (unsigned long long *)dres );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_maxabs<double> ), dim3( red_blocks( n ), nrhs ), dim3( 256 ), 0, ctx->stream, n,
+                            (const double *)x, (unsigned long long *)dres );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+int
+pxd_vec_axpy( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, double sign, const void *dalpha, const void *x, void *y )
+{
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_axpy<pxcd> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            sign, (const pxcd *)dalpha, (const pxcd *)x, (pxcd *)y );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_axpy<double> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            sign, (const double *)dalpha, (const double *)x, (double *)y );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+int
+pxd_vec_scal( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *dalpha, int inv, void *x )
+{
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_scal<pxcd> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            (const pxcd *)dalpha, inv, (pxcd *)x );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_scal<double> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            (const double *)dalpha, inv, (double *)x );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+int
+pxd_vec_copy( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, const void *x, void *y )
+{
+    size_t es = ( ctx->mflt == FT_CPLX64 ) ? 16 : 8;
+    PXD_CHECK( hipMemcpyAsync( y, x, n * nrhs * es, hipMemcpyDeviceToDevice, ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_vec_gemv( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, int64_t m, const void *V, const void *dy, void *x )
+{
+    if ( ctx->mflt == FT_CPLX64 ) {
+        hipLaunchKernelGGL( ( k_gemv<pxcd> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            (int)m, (const pxcd *)V, (const pxcd *)dy, (pxcd *)x );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_gemv<double> ), dim3( nblocks( n, 256 ) ), dim3( 256 ), 0, ctx->stream, n, (int)nrhs,
+                            (int)m, (const double *)V, (const double *)dy, (double *)x );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+}  // extern "C"

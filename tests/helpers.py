@@ -1,0 +1,71 @@
+"""Shared helpers of the test-suite: problem set-up through the PaStiX API."""
+import numpy as np
+
+import pastix_amd as px
+
+
+def make_problem(kind, nx, ny=None, nz=None, **kw):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    if kind == "lap7":
+        return px.Spm.laplacian7(nx, ny, nz, kw.get("diag", 6.0), kw.get("offd", 1.0))
+    if kind == "convdiff27":
+        return px.Spm.convdiff27(nx, ny, nz, 1.0, 0.5, kw.get("seed", 0))
+    if kind == "helmholtz7":
+        return px.Spm.helmholtz7(nx, ny, nz, 0.5, 0.1)
+    raise ValueError(kind)
+
+
+def analyzed(spm, nx, ny=None, nz=None, facto=None, mixed=0, extra=None):
+    """pastix_data after the grid analysis (example/ordering_grid.c flow)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    iparm, dparm = px.initParam()
+    iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
+    iparm[px.IPARM_FACTORIZATION] = px.PastixFactPOTRF if facto is None else facto
+    iparm[px.IPARM_MIXED] = mixed
+    iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
+    for k, v in (extra or {}).items():
+        iparm[k] = v
+    pd = px.init(iparm, dparm)
+    px.analyze_grid(pd, spm, nx, ny, nz)
+    return pd, iparm, dparm
+
+
+def permute_rhs(pd, b):
+    perm = px.order_export(pd)[0]
+    bp = np.zeros_like(b)
+    bp[perm] = b
+    return bp
+
+
+def unpermute(pd, xp):
+    perm = px.order_export(pd)[0]
+    return xp[perm]
+
+
+def static_pivot_threshold(dparm, dtype):
+    eps = np.finfo(dtype).eps / 2.0
+    return eps * dparm[px.DPARM_A_NORM]
+
+
+def cblk_diff(solver, La, Lb, dtype):
+    """Largest per-cblk ||La - Lb||_max / (||Lb||_F * eps) over the stored lower
+    part (cpucblk_zdiff criterion, kernels/cpucblk_zdiff.c:63-130)."""
+    cb = solver["cblk"].reshape(-1, 6)
+    bk = solver["blok"].reshape(-1, 5)
+    eps = np.finfo(dtype).eps / 2.0
+    worst = 0.0
+    for k in range(solver["cblknbr"]):
+        n = cb[k, 1] - cb[k, 0] + 1
+        off0, off1 = cb[k, 4], cb[k + 1, 4]
+        a = La[off0:off1].astype(np.float64)
+        b = Lb[off0:off1].astype(np.float64)
+        # the diagonal tile: compare its lower triangle only
+        d_a = np.tril(a[:n * n].reshape(n, n).T)
+        d_b = np.tril(b[:n * n].reshape(n, n).T)
+        diff = max(np.abs(d_a - d_b).max(), np.abs(a[n * n:] - b[n * n:]).max() if off1 - off0 > n * n else 0.0)
+        nrm = np.sqrt((d_b ** 2).sum() + (b[n * n:] ** 2).sum())
+        if nrm > 0:
+            worst = max(worst, diff / (nrm * eps))
+    return worst

@@ -1,0 +1,111 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: the oracle is the checker for the device engine and the
+CPU baseline of the benchmark; the product never loads it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_lib = None
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise ImportError("oracle library missing: run `python -m pastix_amd.build`")
+        _lib = ctypes.CDLL(ORACLE_PATH)
+        _lib.orc_factorize.restype = ctypes.c_int
+        _lib.orc_factorize.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _i64, _p, _p, _p,
+                                       ctypes.c_double, _p, ctypes.c_int, _p, _p]
+        _lib.orc_solve.restype = ctypes.c_int
+        _lib.orc_solve.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _i64, _p, _i64]
+        _lib.orc_spmv.restype = ctypes.c_int
+        _lib.orc_spmv.argtypes = [_i64, _p, _p, _p, ctypes.c_double, _p, ctypes.c_double, _p]
+        _lib.orc_gmres.restype = _i64
+        _lib.orc_gmres.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                   _i64, _i64, ctypes.c_double]
+        _lib.orc_blas.restype = ctypes.c_char_p
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_p)
+
+
+def blas_name():
+    return lib().orc_blas().decode()
+
+
+class OracleFactor:
+    """Cholesky factor computed by the oracle on an exported solver structure."""
+
+    def __init__(self, solver, bcsc, thresh, dtype=np.float64, nthreads=1):
+        self.s = solver
+        self.cp, self.rt, self.val = bcsc
+        self.dtype = dtype
+        self.flt = 3 if dtype == np.float64 else 2
+        self.n = solver["nodenbr"]
+        self.L = np.zeros(solver["coefnbr"], dtype=dtype)
+        npiv = np.zeros(1, dtype=np.int64)
+        sec = np.zeros(1)
+        lib().orc_factorize(self.flt, self.n, solver["cblknbr"], _ptr(solver["cblk"]), _ptr(solver["blok"]),
+                            solver["coefnbr"], _ptr(self.cp), _ptr(self.rt), _ptr(self.val), thresh,
+                            _ptr(self.L), nthreads, _ptr(npiv), _ptr(sec))
+        self.npiv = int(npiv[0])
+        self.seconds = float(sec[0])
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        lib().orc_solve(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
+                        _ptr(self.s["browtab"]), _ptr(self.L), nrhs, _ptr(x), self.n)
+        return x
+
+    def gmres(self, b, x0, im=25, itermax=250, eps=1e-12):
+        x = np.array(x0, dtype=np.float64, copy=True)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        it = lib().orc_gmres(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
+                             _ptr(self.s["browtab"]), _ptr(self.L), _ptr(self.cp), _ptr(self.rt),
+                             _ptr(self.val), _ptr(b), _ptr(x), im, itermax, eps)
+        return x, int(it)
+
+
+def spmv(cp, rt, val, x, alpha=1.0, beta=0.0, y=None):
+    n = len(cp) - 1
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(n) if y is None else np.array(y, dtype=np.float64, copy=True)
+    lib().orc_spmv(n, _ptr(cp), _ptr(rt), _ptr(val), alpha, _ptr(x), beta, _ptr(y))
+    return y
+
+
+def tiles(solver, L):
+    """Yield (cblk, blok, tile ndarray view) for every blok of the factor."""
+    cb = solver["cblk"].reshape(-1, 6)
+    bk = solver["blok"].reshape(-1, 5)
+    for k in range(solver["cblknbr"]):
+        n = cb[k, 1] - cb[k, 0] + 1
+        for b in range(cb[k, 2], cb[k + 1, 2]):
+            rows = bk[b, 1] - bk[b, 0] + 1
+            off = cb[k, 4] + bk[b, 4]
+            yield k, b, L[off:off + rows * n].reshape(n, rows).T
+
+
+def dense_from_factor(solver, L):
+    """Assemble the dense lower factor (permuted numbering) from the tiles."""
+    n = solver["nodenbr"]
+    D = np.zeros((n, n), dtype=L.dtype)
+    cb = solver["cblk"].reshape(-1, 6)
+    bk = solver["blok"].reshape(-1, 5)
+    for k, b, t in tiles(solver, L):
+        fc = cb[k, 0]
+        fr = bk[b, 0]
+        D[fr:fr + t.shape[0], fc:fc + t.shape[1]] = t
+    return np.tril(D)
