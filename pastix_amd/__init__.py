@@ -331,14 +331,24 @@ def initParam():
     return iparm, dparm
 
 
+# pastix_data keeps pointers to the caller's iparm/dparm arrays (the C API
+# references them, reference common/api.c:767-768): hold them until finalize.
+_LIVE_PARAMS = {}
+
+
 def init(iparm, dparm, comm=0):
+    if iparm.dtype != np.int64 or dparm.dtype != np.float64 or not iparm.flags.c_contiguous:
+        raise TypeError("iparm must be a contiguous int64 array and dparm a float64 array")
     pd = _p()
     lib.pastixInit(ctypes.byref(pd), comm, _ptr(iparm), _ptr(dparm))
+    _LIVE_PARAMS[pd.value] = (iparm, dparm)
     return pd
 
 
 def finalize(pastix_data, iparm=None, dparm=None):
+    key = pastix_data.value
     lib.pastixFinalize(ctypes.byref(pastix_data))
+    _LIVE_PARAMS.pop(key, None)
 
 
 def task_analyze(pastix_data, spm):

@@ -382,6 +382,7 @@ template <typename T>
 void
 set_lds_attributes( size_t lds )
 {
+    lds -= 16384;   // room for the kernels' static LDS
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
@@ -425,6 +426,11 @@ pxd_create( pxd_ctx_t **pctx, int device )
     PXD_CHECK( hipMalloc( &ctx->dred, 4096 ) );
     set_lds_attributes<double>( ctx->lds_max );
     set_lds_attributes<float>( ctx->lds_max );
+    (void)hipGetLastError();   // an attribute refused above must not be reported later
+    if ( getenv( "PXD_VERBOSE" ) ) {
+        fprintf( stderr, "pastix_amd device: %s, %d CUs, LDS/block %zu, L2 %d\n", prop.gcnArchName,
+                 prop.multiProcessorCount, ctx->lds_max, prop.l2CacheSize );
+    }
     *pctx = ctx;
     return 0;
 }

@@ -8,10 +8,12 @@
  * 120-635, sopalin/pastix_task_sopalin.c:92-693, sopalin/pastix_task_solve.c
  * and refinement/pastix_task_refine.c.
  */
+#define _GNU_SOURCE
 #include <complex.h>
 #include <math.h>
 #include <float.h>
 #include <unistd.h>
+#include <sched.h>
 #include "px_internal.h"
 
 int px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_int_t *iters );
@@ -93,11 +95,26 @@ pastixInitParam( pastix_int_t *iparm, double *dparm )
     dparm[DPARM_COMPRESS_MIN_RATIO] = 1.;
 }
 
+/* Host threads of the analysis: PASTIX_NUM_THREADS, else the CPUs of the
+ * affinity mask capped at 16 (the host share of one GPU). */
 static int
 px_default_threads( void )
 {
     const char *env = getenv( "PASTIX_NUM_THREADS" );
-    long        n   = env ? atol( env ) : sysconf( _SC_NPROCESSORS_ONLN );
+    long        n;
+    if ( env ) {
+        n = atol( env );
+    }
+    else {
+        cpu_set_t set;
+        n = sysconf( _SC_NPROCESSORS_ONLN );
+        if ( sched_getaffinity( 0, sizeof( set ), &set ) == 0 ) {
+            n = CPU_COUNT( &set );
+        }
+        if ( n > 16 ) {
+            n = 16;
+        }
+    }
     return ( n > 0 ) ? (int)n : 1;
 }
 

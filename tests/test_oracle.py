@@ -1,0 +1,145 @@
+"""The CPU oracle pinned against what the mathematics fixes: dense LAPACK-style
+Cholesky of P A P^T (tile by tile, zero outside the symbol), hand-computed
+static-pivot cases, exact SpMV on integer-valued matrices, solve and GMRES
+convergence, and the plrnt generator against sequential LCG stepping."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import pastix_amd as px
+import oracle_ref as orc
+from helpers import analyzed, permute_rhs, static_pivot_threshold
+
+
+def _dense(cp, rt, v, n):
+    A = np.zeros((n, n), dtype=v.dtype)
+    for j in range(n):
+        A[rt[cp[j]:cp[j + 1]], j] = v[cp[j]:cp[j + 1]]
+    return A
+
+
+def _setup(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    spm = px.Spm.laplacian7(nx, ny, nz)
+    pd, iparm, dparm = analyzed(spm, nx, ny, nz)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    return spm, pd, iparm, dparm, px.solver_export(pd), px.bcsc_export(pd)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (7, 5, 6), (10, 10, 10)])
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_oracle_cholesky_equals_dense(dims, nthreads):
+    spm, pd, iparm, dparm, s, bc = _setup(*dims)
+    n = s["nodenbr"]
+    F = orc.OracleFactor(s, bc, static_pivot_threshold(dparm, np.float64), nthreads=nthreads)
+    A = _dense(*bc, n)
+    Ld = np.linalg.cholesky(A)
+    Lo = orc.dense_from_factor(s, F.L)
+    assert np.abs(Lo - Ld).max() <= 1e-13
+    # the symbolic structure holds every nonzero of the factor
+    assert np.abs(Ld[Lo == 0]).max() <= 1e-13
+    assert np.abs(Lo @ Lo.T - A).max() <= 1e-12 * np.abs(A).max()
+    px.finalize(pd)
+
+
+def test_oracle_single_precision_factor():
+    spm, pd, iparm, dparm, s, bc = _setup(6)
+    n = s["nodenbr"]
+    F = orc.OracleFactor(s, bc, static_pivot_threshold(dparm, np.float32), np.float32)
+    Ld = np.linalg.cholesky(_dense(*bc, n))
+    Lo = orc.dense_from_factor(s, F.L).astype(np.float64)
+    assert np.abs(Lo - Ld).max() <= 50 * np.finfo(np.float32).eps * np.abs(Ld).max()
+    px.finalize(pd)
+
+
+def _one_cblk(n):
+    """Solver arrays of a single dense cblk of width n."""
+    cblk = np.array([0, n - 1, 0, n, 0, 0, n, n, 1, 0, n * n, 0], dtype=np.int64)
+    blok = np.array([0, n - 1, 0, 0, 0], dtype=np.int64)
+    return dict(cblknbr=1, bloknbr=1, coefnbr=n * n, nodenbr=n, cblk=cblk, blok=blok,
+                browtab=np.zeros(1, dtype=np.int64))
+
+
+def _csc(A):
+    M = sp.csc_matrix(A)
+    M.sort_indices()
+    return (M.indptr.astype(np.int64), M.indices.astype(np.int64), M.data.astype(np.float64))
+
+
+def test_static_pivot_known_answers():
+    # |a_00| < tau is replaced by tau (core_zpotf2sp), then L = sqrt
+    tau = 1e-8
+    A = np.array([[1e-20, 0., 0.], [0., 4., 2.], [0., 2., 5.]])
+    F = orc.OracleFactor(_one_cblk(3), _csc(A), tau)
+    L = F.L.reshape(3, 3).T
+    assert F.npiv == 1
+    assert L[0, 0] == pytest.approx(np.sqrt(tau))
+    assert L[1, 1] == pytest.approx(2.0) and L[2, 1] == pytest.approx(1.0) and L[2, 2] == pytest.approx(2.0)
+    # a pivot exactly at the threshold is kept
+    A2 = np.diag([tau, 9.0])
+    F2 = orc.OracleFactor(_one_cblk(2), _csc(A2), tau)
+    assert F2.npiv == 0 and F2.L[0] == pytest.approx(np.sqrt(tau))
+    # a pivot that becomes tiny during elimination is counted too
+    A3 = np.array([[1.0, 1.0], [1.0, 1.0 + 1e-18]])
+    F3 = orc.OracleFactor(_one_cblk(2), _csc(A3), tau)
+    assert F3.npiv == 1 and F3.L.reshape(2, 2).T[1, 1] == pytest.approx(np.sqrt(tau))
+
+
+def test_oracle_spmv_is_exact_on_the_integer_laplacian():
+    spm, pd, iparm, dparm, s, bc = _setup(8)
+    cp, rt, v = bc
+    x = np.round(px.plrnt(s["nodenbr"], seed=7) * 64)      # small integers: exact arithmetic
+    y = orc.spmv(cp, rt, v, x)
+    M = sp.csc_matrix((v, rt, cp), shape=(s["nodenbr"],) * 2)
+    assert np.array_equal(y, M.T @ x)
+    assert np.array_equal(y, M @ x)                         # symmetric storage is full
+    px.finalize(pd)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_oracle_solve_backward_error(nrhs):
+    spm, pd, iparm, dparm, s, bc = _setup(12)
+    F = orc.OracleFactor(s, bc, static_pivot_threshold(dparm, np.float64))
+    x0, b = spm.gen_rhs(nrhs, seed=0)
+    xp = F.solve(permute_rhs(pd, b))
+    perm = px.order_export(pd)[0]
+    x = xp[perm]
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 1e-14
+    assert np.abs(x - x0).max() <= 1e-12
+    px.finalize(pd)
+
+
+def test_oracle_gmres_with_single_factor_converges():
+    spm, pd, iparm, dparm, s, bc = _setup(10)
+    F32 = orc.OracleFactor(s, bc, static_pivot_threshold(dparm, np.float32), np.float32)
+    x0, b = spm.gen_rhs(1, seed=2)
+    bp = permute_rhs(pd, b)[:, 0]
+    xs = F32.solve(bp)
+    r0 = np.linalg.norm(bp - orc.spmv(*bc, xs)) / np.linalg.norm(bp)
+    assert r0 > 1e-12
+    x, it = F32.gmres(bp, xs)
+    assert np.linalg.norm(bp - orc.spmv(*bc, x)) / np.linalg.norm(bp) <= 1e-12
+    assert 1 <= it <= 10
+    # with an exact (double) preconditioner the first residual already passes
+    F64 = orc.OracleFactor(s, bc, static_pivot_threshold(dparm, np.float64))
+    x64, it64 = F64.gmres(bp, F64.solve(bp))
+    assert it64 == 0
+    px.finalize(pd)
+
+
+def test_plrnt_jump_matches_sequential_lcg():
+    a, c, m = 6364136223846793005, 1, 1 << 64
+    seed = 12345
+    ran, seq = seed, []
+    for _ in range(50):
+        seq.append(np.float32(0.5) - np.float32(ran) * np.float32(5.4210108624275222e-20))
+        ran = (a * ran + c) % m
+    assert np.array_equal(px.plrnt(50, seed=seed), np.array(seq, dtype=np.float64))
+    # the O(log n) jump lands on the same stream
+    assert np.array_equal(px.plrnt(20, seed=seed, shift=30), np.array(seq[30:50], dtype=np.float64))
+    v = px.plrnt(100000, seed=0)
+    assert v.min() >= -0.5 and v.max() <= 0.5 and abs(v.mean()) < 0.01
