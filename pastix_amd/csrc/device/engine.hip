@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 #include "kernels_solve.hpp"
+#include "kernels_mfma.hpp"
 
 namespace {
 
@@ -31,6 +32,7 @@ struct level_t {
     int64_t t0, t1;     // trsm tasks (32-row tiles)
     int64_t u0, u1;     // update tasks
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
+    int64_t i0, i1;     // block-column inversion tasks
     int32_t maxn;
 };
 
@@ -56,6 +58,7 @@ struct pxd_ctx_s {
     row_task *dtrsm = nullptr;
     upd_task *dupd  = nullptr;
     row_task *dsolv = nullptr;
+    inv_task *dinvt = nullptr;
     int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
 
     // matrix
@@ -75,8 +78,11 @@ struct pxd_ctx_s {
     size_t dvecs_bytes = 0;
     void  *dred = nullptr;         // reduction scratch
 
-    // rows per TRSM tile (32, or 16 when the LDS cannot hold 32 x colmax)
-    int          trsm_rt = 32;
+    // rows per TRSM tile: 64 for the matrix-core kernels; for the scalar
+    // kernels 32, or 16 when the LDS cannot hold 32 x colmax
+    int          trsm_rt = 64;
+    // 1: matrix-core kernels with the explicit diagonal inverse; 0: scalar kernels
+    int          kernels = 1;
 
     // options / stats
     int          timing = 0;
@@ -149,6 +155,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
 
     std::vector<int32_t>  list;
     std::vector<row_task> trsm, solv;
+    std::vector<inv_task> invt;
     std::vector<upd_task> upd;
     ctx->levels.assign( nlev, level_t{} );
 
@@ -161,6 +168,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
         lv.t0   = trsm.size();
         lv.u0   = upd.size();
         lv.f0   = solv.size();
+        lv.i0   = invt.size();
         lv.maxn = 0;
         ivs.clear();
         for ( int32_t k : bylev[L] ) {
@@ -173,6 +181,9 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
             }
             for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
                 solv.push_back( row_task{ k, r0 } );
+            }
+            for ( int32_t J = 0; J + PXD_NB < nk; J += PXD_NB ) {
+                invt.push_back( inv_task{ k, J } );
             }
             int32_t lb = ctx->hcb[k + 1].fblok;
             for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
@@ -213,6 +224,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
         lv.t1 = trsm.size();
         lv.u1 = upd.size();
         lv.f1 = solv.size();
+        lv.i1 = invt.size();
     }
     ctx->nupd  = upd.size();
     ctx->ntrsm = trsm.size();
@@ -220,7 +232,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
     ctx->stats.levels       = nlev;
     ctx->stats.update_tiles = upd.size();
     if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
-         upload( &ctx->dsolv, solv ) ) {
+         upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) ) {
         return 6;
     }
     return 0;
@@ -274,14 +286,31 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         ctx->stats.fill_seconds = 1e-3 * ev_ms( e0, ea );
     }
 
+    if ( ctx->kernels ) {
+        PXD_CHECK( hipMemsetAsync( DI, 0, ctx->dinvnbr * sizeof( T ), s ) );
+        launches++;
+    }
     for ( const level_t &lv : ctx->levels ) {
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
         unsigned nt = (unsigned)( lv.t1 - lv.t0 );
         unsigned nu = (unsigned)( lv.u1 - lv.u0 );
+        unsigned ni = (unsigned)( lv.i1 - lv.i0 );
         if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
-        hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
-                            ctx->dcb, L, DI, thresh, ctx->dscratch );
-        launches++;
+        if ( ctx->kernels ) {
+            hipLaunchKernelGGL( ( k_potrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
+                                ctx->dcb, L, DI, thresh, ctx->dscratch );
+            launches++;
+            if ( ni ) {
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, s, ctx->dinvt + lv.i0,
+                                    ctx->dcb, (const T *)L, DI );
+                launches++;
+            }
+        }
+        else {
+            hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
+                                ctx->dcb, L, DI, thresh, ctx->dscratch );
+            launches++;
+        }
         if ( ctx->timing ) {
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
@@ -289,7 +318,11 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         }
         if ( nt ) {
             if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
-            if ( ctx->trsm_rt == 32 ) {
+            if ( ctx->kernels ) {
+                hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+            }
+            else if ( ctx->trsm_rt == 32 ) {
                 hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ),
                                     s, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
             }
@@ -306,8 +339,14 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         }
         if ( nu ) {
             if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
-            hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s, ctx->dupd + lv.u0,
-                                ctx->dcb, ctx->dbk, L );
+            if ( ctx->kernels ) {
+                hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dupd + lv.u0, ctx->dcb, ctx->dbk, L );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s, ctx->dupd + lv.u0,
+                                    ctx->dcb, ctx->dbk, L );
+            }
             launches++;
             if ( ctx->timing ) {
                 PXD_CHECK( hipEventRecord( eb, s ) );
@@ -354,8 +393,14 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         unsigned       np = (unsigned)( lv.c1 - lv.c0 );
         unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
         size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
-        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
+        if ( ctx->kernels ) {
+            hipLaunchKernelGGL( ( k_solve_diag_inv<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dlist + lv.c0, ctx->dcb, DI, X, ldx, nrhs, 0 );
+        }
+        else {
+            hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
+        }
         if ( nf ) {
             hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                 ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
@@ -368,11 +413,23 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
         size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
         if ( nf ) {
-            hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+            if ( ctx->kernels ) {
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+            }
         }
-        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
+        if ( ctx->kernels ) {
+            hipLaunchKernelGGL( ( k_solve_diag_inv<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dlist + lv.c0, ctx->dcb, DI, X, ldx, nrhs, 1 );
+        }
+        else {
+            hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
+        }
     }
     PXD_CHECK( hipGetLastError() );
     return 0;
@@ -387,6 +444,7 @@ set_lds_attributes( size_t lds )
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_diag_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
 }
 
 }  // namespace
@@ -443,7 +501,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dsolv,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dsolv, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
     for ( void *p : ptrs ) {
@@ -475,6 +533,11 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     std::string k( key );
     if ( k == "timing" ) {
         ctx->timing = ( value != 0. );
+        return 0;
+    }
+    if ( k == "kernels" ) {
+        // takes effect at the next pxd_set_structure
+        ctx->kernels = ( value != 0. );
         return 0;
     }
     return 7;
@@ -550,7 +613,10 @@ pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
         if ( k < cblknbr ) {
             int32_t nk = c.lcolnum - c.fcolnum + 1;
             ctx->colmax = std::max( ctx->colmax, nk );
-            dinv += (int64_t)( ( nk + PXD_NB - 1 ) / PXD_NB ) * PXD_NB * PXD_NB;
+            // n x n explicit inverse (matrix-core path) or the inverted
+            // PXD_NB x PXD_NB diagonal sub-blocks (scalar path)
+            dinv += std::max<int64_t>( (int64_t)nk * nk,
+                                       (int64_t)( ( nk + PXD_NB - 1 ) / PXD_NB ) * PXD_NB * PXD_NB );
             for ( int32_t j = c.fcolnum; j <= c.lcolnum; j++ ) {
                 col2cblk[j] = (int32_t)k;
             }
@@ -572,8 +638,18 @@ pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
         }
     }
     // the trsm tile keeps RT rows x n columns in LDS (static LDS of the kernel ~9 KB)
-    ctx->trsm_rt = ( trsm_lds<double>( ctx->colmax, 32 ) + 12288 <= ctx->lds_max ) ? 32 : 16;
-    if ( trsm_lds<double>( ctx->colmax, 16 ) + 12288 > ctx->lds_max ) {
+    if ( ctx->kernels ) {
+        ctx->trsm_rt = 64;
+        if ( ctx->colmax > PXD_TRSM_NT * 16 ) {
+            fprintf( stderr, "pastix_amd device: cblk width %d above the %d columns of the TRSM tile\n",
+                     ctx->colmax, PXD_TRSM_NT * 16 );
+            return 3;
+        }
+    }
+    else {
+        ctx->trsm_rt = ( trsm_lds<double>( ctx->colmax, 32 ) + 12288 <= ctx->lds_max ) ? 32 : 16;
+    }
+    if ( !ctx->kernels && trsm_lds<double>( ctx->colmax, 16 ) + 12288 > ctx->lds_max ) {
         fprintf( stderr, "pastix_amd device: cblk width %d exceeds the LDS budget of the TRSM tile\n",
                  ctx->colmax );
         return 3;

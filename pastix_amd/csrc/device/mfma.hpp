@@ -1,0 +1,32 @@
+// mfma.hpp -- CDNA4 matrix-core helpers (v_mfma_*_16x16x4 for f32 and f64).
+//
+// Lane layout of the 16x16x4 instructions (checked on gfx950): lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; the accumulator holds 4 values of D with
+// column l%16 and row 4*(l/16)+v for f32, row (l/16)+4*v for f64.
+#pragma once
+#include "px_device.hpp"
+
+typedef float  pxf4 __attribute__( ( ext_vector_type( 4 ) ) );
+typedef double pxd4 __attribute__( ( ext_vector_type( 4 ) ) );
+
+template <typename T> struct mfma16;
+
+template <> struct mfma16<float> {
+    typedef pxf4 acc_t;
+    __device__ static inline acc_t zero() { return acc_t{ 0.f, 0.f, 0.f, 0.f }; }
+    __device__ static inline acc_t mma( float a, float b, acc_t c )
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32( a, b, c, 0, 0, 0 );
+    }
+    __device__ static inline int row( int lane, int v ) { return 4 * ( lane >> 4 ) + v; }
+};
+
+template <> struct mfma16<double> {
+    typedef pxd4 acc_t;
+    __device__ static inline acc_t zero() { return acc_t{ 0., 0., 0., 0. }; }
+    __device__ static inline acc_t mma( double a, double b, acc_t c )
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64( a, b, c, 0, 0, 0 );
+    }
+    __device__ static inline int row( int lane, int v ) { return ( lane >> 4 ) + 4 * v; }
+};

@@ -15,16 +15,18 @@ from helpers import analyzed, cblk_diff, make_problem, permute_rhs, static_pivot
 pytestmark = pytest.mark.gpu
 
 
-def _factor_case(nx, mixed=0):
+def _factor_case(nx, mixed=0, kernels=1):
     spm = make_problem("lap7", nx)
     pd, iparm, dparm = analyzed(spm, nx, mixed=mixed)
+    px.set_option(pd, "kernels", kernels)
     px.task_numfact(pd, spm)
     return spm, pd, iparm, dparm
 
 
+@pytest.mark.parametrize("kernels", [1, 0])
 @pytest.mark.parametrize("nx", [3, 5, 8, 12, 20])
-def test_potrf_fp64_tiles_match_oracle(nx):
-    spm, pd, iparm, dparm = _factor_case(nx)
+def test_potrf_fp64_tiles_match_oracle(nx, kernels):
+    spm, pd, iparm, dparm = _factor_case(nx, kernels=kernels)
     s = px.solver_export(pd)
     L = px.get_coeftab(pd, s["coefnbr"], np.float64)
     F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float64))
@@ -33,9 +35,10 @@ def test_potrf_fp64_tiles_match_oracle(nx):
     px.finalize(pd)
 
 
-@pytest.mark.parametrize("nx", [5, 12])
-def test_potrf_fp32_tiles_match_oracle(nx):
-    spm, pd, iparm, dparm = _factor_case(nx, mixed=1)
+@pytest.mark.parametrize("kernels", [1, 0])
+@pytest.mark.parametrize("nx", [5, 12, 24])
+def test_potrf_fp32_tiles_match_oracle(nx, kernels):
+    spm, pd, iparm, dparm = _factor_case(nx, mixed=1, kernels=kernels)
     s = px.solver_export(pd)
     L = px.get_coeftab(pd, s["coefnbr"], np.float32)
     F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float32), np.float32)
@@ -43,9 +46,10 @@ def test_potrf_fp32_tiles_match_oracle(nx):
     px.finalize(pd)
 
 
-@pytest.mark.parametrize("nx,nrhs", [(8, 1), (20, 1), (20, 3), (12, 9)])
-def test_solve_fp64_backward_error(nx, nrhs):
-    spm, pd, iparm, dparm = _factor_case(nx)
+@pytest.mark.parametrize("kernels", [1, 0])
+@pytest.mark.parametrize("nx,nrhs", [(8, 1), (20, 1), (20, 3), (12, 9), (30, 2)])
+def test_solve_fp64_backward_error(nx, nrhs, kernels):
+    spm, pd, iparm, dparm = _factor_case(nx, kernels=kernels)
     x0, b = spm.gen_rhs(nrhs, seed=0)
     x = np.array(b, order="F", copy=True)
     px.task_solve(pd, spm, x)
