@@ -30,7 +30,8 @@ enum { FACT_POTRF = 0, FACT_SYTRF = 1, FACT_GETRF = 2 };
 struct level_t {
     int64_t c0, c1;     // cblk list
     int64_t t0, t1;     // trsm tasks (32-row tiles)
-    int64_t u0, u1;     // update tasks
+    int64_t u0, uc, u1; // update tasks (64 x 64 tiles); [u0, uc) face the next level
+    int64_t b0, bc, b1; // update tasks (tall tiles); [b0, bc) face the next level
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
     int64_t i0, i1;     // block-column inversion tasks
     int32_t maxn;
@@ -41,6 +42,8 @@ struct level_t {
 struct pxd_ctx_s {
     int         device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;    // panels of the next level (lookahead)
+    hipEvent_t  ev_ready = nullptr, ev_panel = nullptr;
     size_t      lds_max = 65536;
 
     // structure
@@ -53,10 +56,14 @@ struct pxd_ctx_s {
     int32_t *dcol2cblk = nullptr;
 
     // schedule
+    std::vector<int32_t> level;     // dependency level of each cblk
+    int32_t              nlev = 0;
+    int                  tasks_dirty = 0;   // options changed the task lists
     std::vector<level_t> levels;
     int32_t  *dlist = nullptr;
     row_task *dtrsm = nullptr;
     upd_task *dupd  = nullptr;
+    upd_task *dupdb = nullptr;
     row_task *dsolv = nullptr;
     inv_task *dinvt = nullptr;
     int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
@@ -83,6 +90,10 @@ struct pxd_ctx_s {
     int          trsm_rt = 64;
     // 1: matrix-core kernels with the explicit diagonal inverse; 0: scalar kernels
     int          kernels = 1;
+    // blok updates with at least this many stacked rows use the tall tiles
+    int          upd_big_rows = 192;
+    // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
+    int          upd_bm = 256;
 
     // options / stats
     int          timing = 0;
@@ -129,23 +140,57 @@ nblocks( int64_t n, int bs )
     return (unsigned)( ( n + bs - 1 ) / bs );
 }
 
-// Build the level schedule of the factorization and the solve.
-int
-build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks,
-                const int64_t *browtab )
+// LDS bytes of the scalar TRSM tile: rt rows x n columns
+template <typename T>
+size_t
+trsm_lds( int32_t n, int rt )
 {
-    const int64_t        nc = ctx->cblknbr;
-    std::vector<int32_t> level( nc, 0 );
-    int32_t              nlev = 0;
+    return (size_t)n * ( rt + 1 ) * sizeof( T );
+}
 
+// Dependency level of every cblk: 1 + the highest level of its sources.
+void
+compute_levels( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks, const int64_t *browtab )
+{
+    const int64_t         nc    = ctx->cblknbr;
+    std::vector<int32_t> &level = ctx->level;
+    level.assign( nc, 0 );
+    ctx->nlev = 0;
     for ( int64_t k = 0; k < nc; k++ ) {
         int32_t l = 0;
         for ( int64_t e = cblks[k].brownum; e < cblks[k + 1].brownum; e++ ) {
             int64_t src = bloks[browtab[e]].lcblknm;
             l = std::max( l, level[src] + 1 );
         }
-        level[k] = l;
-        nlev     = std::max( nlev, l + 1 );
+        level[k]  = l;
+        ctx->nlev = std::max( ctx->nlev, l + 1 );
+    }
+}
+
+// Build the task lists of the factorization and the solve, level by level.
+int
+build_schedule( pxd_ctx_t *ctx )
+{
+    const int64_t               nc    = ctx->cblknbr;
+    const std::vector<int32_t> &level = ctx->level;
+    const int32_t               nlev  = ctx->nlev;
+
+    // the trsm tile keeps RT rows x n columns in LDS (static LDS of the kernel ~9 KB)
+    if ( ctx->kernels ) {
+        ctx->trsm_rt = 64;
+        if ( ctx->colmax > PXD_TRSM_NT * 16 ) {
+            fprintf( stderr, "pastix_amd device: cblk width %d above the %d columns of the TRSM tile\n",
+                     ctx->colmax, PXD_TRSM_NT * 16 );
+            return 3;
+        }
+    }
+    else {
+        ctx->trsm_rt = ( trsm_lds<double>( ctx->colmax, 32 ) + 12288 <= ctx->lds_max ) ? 32 : 16;
+    }
+    if ( !ctx->kernels && trsm_lds<double>( ctx->colmax, 16 ) + 12288 > ctx->lds_max ) {
+        fprintf( stderr, "pastix_amd device: cblk width %d exceeds the LDS budget of the TRSM tile\n",
+                 ctx->colmax );
+        return 3;
     }
 
     std::vector<std::vector<int32_t>> bylev( nlev );
@@ -156,10 +201,11 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
     std::vector<int32_t>  list;
     std::vector<row_task> trsm, solv;
     std::vector<inv_task> invt;
-    std::vector<upd_task> upd;
+    std::vector<upd_task> upd, updb;
     ctx->levels.assign( nlev, level_t{} );
+    const bool big_ok = ctx->kernels != 0;
 
-    struct iv { int32_t F, f, l; int64_t first, last; };
+    struct iv { int32_t F, f, l; std::vector<upd_task> *list; int64_t first, last; };
     std::vector<iv> ivs;
 
     for ( int32_t L = 0; L < nlev; L++ ) {
@@ -167,6 +213,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
         lv.c0   = list.size();
         lv.t0   = trsm.size();
         lv.u0   = upd.size();
+        lv.b0   = updb.size();
         lv.f0   = solv.size();
         lv.i0   = invt.size();
         lv.maxn = 0;
@@ -185,18 +232,43 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
             for ( int32_t J = 0; J + PXD_NB < nk; J += PXD_NB ) {
                 invt.push_back( inv_task{ k, J } );
             }
-            int32_t lb = ctx->hcb[k + 1].fblok;
-            for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
-                const dblok &bb = ctx->hbk[b];
-                int32_t      N  = bb.lrownum - bb.frownum + 1;
-                int32_t      M  = c.stride - bb.rowoff;
-                int64_t      first = upd.size();
-                for ( int32_t tn = 0; tn * 64 < N; tn++ ) {
-                    for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
-                        upd.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
+        }
+        // update tiles: first those into cblks of the next level (the
+        // critical part that the next level's panels wait for), then the rest
+        for ( int pass = 0; pass < 2; pass++ ) {
+            if ( pass == 1 ) {
+                lv.uc = upd.size();
+                lv.bc = updb.size();
+            }
+            for ( int32_t k : bylev[L] ) {
+                const dcblk &c  = ctx->hcb[k];
+                int32_t      lb = ctx->hcb[k + 1].fblok;
+                for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
+                    const dblok &bb = ctx->hbk[b];
+                    if ( ( level[bb.fcblknm] == L + 1 ) != ( pass == 0 ) ) {
+                        continue;
+                    }
+                    int32_t N = bb.lrownum - bb.frownum + 1;
+                    int32_t M = c.stride - bb.rowoff;
+                    if ( big_ok && M >= ctx->upd_big_rows ) {
+                        int64_t first = updb.size();
+                        for ( int32_t tn = 0; tn * PXD_UPD_BN < N; tn++ ) {
+                            for ( int32_t tm = 0; tm * ctx->upd_bm < M; tm++ ) {
+                                updb.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
+                            }
+                        }
+                        ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, &updb, first, (int64_t)updb.size() } );
+                    }
+                    else {
+                        int64_t first = upd.size();
+                        for ( int32_t tn = 0; tn * 64 < N; tn++ ) {
+                            for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
+                                upd.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
+                            }
+                        }
+                        ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, &upd, first, (int64_t)upd.size() } );
                     }
                 }
-                ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, first, (int64_t)upd.size() } );
             }
         }
         // atomic flags for updates whose facing columns overlap another source's
@@ -214,7 +286,7 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
                 maxl = std::max( maxl, ivs[p].l );
                 if ( conflict ) {
                     for ( int64_t u = ivs[p].first; u < ivs[p].last; u++ ) {
-                        upd[u].flags |= 1;
+                        ( *ivs[p].list )[u].flags |= 1;
                     }
                 }
             }
@@ -223,15 +295,17 @@ build_schedule( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
         lv.c1 = list.size();
         lv.t1 = trsm.size();
         lv.u1 = upd.size();
+        lv.b1 = updb.size();
         lv.f1 = solv.size();
         lv.i1 = invt.size();
     }
-    ctx->nupd  = upd.size();
+    ctx->nupd  = upd.size() + updb.size();
     ctx->ntrsm = trsm.size();
     ctx->nsolv = solv.size();
     ctx->stats.levels       = nlev;
-    ctx->stats.update_tiles = upd.size();
+    ctx->stats.update_tiles = upd.size() + updb.size();
     if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
+         upload( &ctx->dupdb, updb ) ||
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) ) {
         return 6;
     }
@@ -245,13 +319,6 @@ ev_ms( hipEvent_t a, hipEvent_t b )
     float ms = 0.f;
     (void)hipEventElapsedTime( &ms, a, b );
     return ms;
-}
-
-template <typename T>
-size_t
-trsm_lds( int32_t n, int rt )
-{
-    return (size_t)n * ( rt + 1 ) * sizeof( T );
 }
 
 // ---------------------------------------------------------------------------
@@ -290,69 +357,103 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         PXD_CHECK( hipMemsetAsync( DI, 0, ctx->dinvnbr * sizeof( T ), s ) );
         launches++;
     }
-    for ( const level_t &lv : ctx->levels ) {
+    // launchers of one level's phases on a stream
+    auto panel = [&]( const level_t &lv, hipStream_t st ) {
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
-        unsigned nt = (unsigned)( lv.t1 - lv.t0 );
-        unsigned nu = (unsigned)( lv.u1 - lv.u0 );
         unsigned ni = (unsigned)( lv.i1 - lv.i0 );
-        if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
         if ( ctx->kernels ) {
-            hipLaunchKernelGGL( ( k_potrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
+            hipLaunchKernelGGL( ( k_potrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
             launches++;
             if ( ni ) {
-                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, s, ctx->dinvt + lv.i0,
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
                                     ctx->dcb, (const T *)L, DI );
                 launches++;
             }
         }
         else {
-            hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, s, ctx->dlist + lv.c0,
+            hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
             launches++;
         }
-        if ( ctx->timing ) {
+    };
+    auto trsm = [&]( const level_t &lv, hipStream_t st ) {
+        unsigned nt = (unsigned)( lv.t1 - lv.t0 );
+        if ( !nt ) {
+            return;
+        }
+        if ( ctx->kernels ) {
+            hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
+                                ctx->dcb, ctx->dbk, L, (const T *)DI );
+        }
+        else if ( ctx->trsm_rt == 32 ) {
+            hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ), st,
+                                ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+        }
+        else {
+            hipLaunchKernelGGL( ( k_trsm<T, 16> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 16 ), st,
+                                ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+        }
+        launches++;
+    };
+    // update tiles [u0, u1) of the 64 x 64 list and [b0, b1) of the tall list
+    auto update = [&]( int64_t u0, int64_t u1, int64_t b0, int64_t b1, hipStream_t st ) {
+        unsigned nu = (unsigned)( u1 - u0 );
+        unsigned nb = (unsigned)( b1 - b0 );
+        if ( nb ) {
+            hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L );
+            launches++;
+        }
+        if ( nu && ctx->kernels ) {
+            hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
+                                ctx->dcb, ctx->dbk, L );
+            launches++;
+        }
+        else if ( nu ) {
+            hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0, ctx->dcb,
+                                ctx->dbk, L );
+            launches++;
+        }
+    };
+
+    if ( ctx->timing ) {
+        // one stream, every phase timed on its own
+        for ( const level_t &lv : ctx->levels ) {
+            PXD_CHECK( hipEventRecord( ea, s ) );
+            panel( lv, s );
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_panel += 1e-3 * ev_ms( ea, eb );
+            PXD_CHECK( hipEventRecord( ea, s ) );
+            trsm( lv, s );
+            PXD_CHECK( hipEventRecord( eb, s ) );
+            PXD_CHECK( hipEventSynchronize( eb ) );
+            t_trsm += 1e-3 * ev_ms( ea, eb );
+            PXD_CHECK( hipEventRecord( ea, s ) );
+            update( lv.u0, lv.u1, lv.b0, lv.b1, s );
+            PXD_CHECK( hipEventRecord( eb, s ) );
+            PXD_CHECK( hipEventSynchronize( eb ) );
+            t_upd += 1e-3 * ev_ms( ea, eb );
         }
-        if ( nt ) {
-            if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
-            if ( ctx->kernels ) {
-                hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
-            }
-            else if ( ctx->trsm_rt == 32 ) {
-                hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ),
-                                    s, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
-            }
-            else {
-                hipLaunchKernelGGL( ( k_trsm<T, 16> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 16 ),
-                                    s, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
-            }
-            launches++;
-            if ( ctx->timing ) {
-                PXD_CHECK( hipEventRecord( eb, s ) );
-                PXD_CHECK( hipEventSynchronize( eb ) );
-                t_trsm += 1e-3 * ev_ms( ea, eb );
-            }
-        }
-        if ( nu ) {
-            if ( ctx->timing ) PXD_CHECK( hipEventRecord( ea, s ) );
-            if ( ctx->kernels ) {
-                hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dupd + lv.u0, ctx->dcb, ctx->dbk, L );
-            }
-            else {
-                hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, s, ctx->dupd + lv.u0,
-                                    ctx->dcb, ctx->dbk, L );
-            }
-            launches++;
-            if ( ctx->timing ) {
-                PXD_CHECK( hipEventRecord( eb, s ) );
-                PXD_CHECK( hipEventSynchronize( eb ) );
-                t_upd += 1e-3 * ev_ms( ea, eb );
-            }
+    }
+    else {
+        // Lookahead: the panels, inverses and TRSM of level L+1 (stream s2)
+        // start as soon as the updates into level L+1 are done, while the
+        // remaining updates of level L go on in stream s.  Those only write
+        // cblks of later levels, so the two streams touch disjoint data.
+        hipStream_t s2 = ctx->stream2;
+        hipEvent_t  ready = ctx->ev_ready, paneldone = ctx->ev_panel;
+        PXD_CHECK( hipEventRecord( ready, s ) );
+        for ( const level_t &lv : ctx->levels ) {
+            PXD_CHECK( hipStreamWaitEvent( s2, ready, 0 ) );
+            panel( lv, s2 );
+            trsm( lv, s2 );
+            PXD_CHECK( hipEventRecord( paneldone, s2 ) );
+            PXD_CHECK( hipStreamWaitEvent( s, paneldone, 0 ) );
+            update( lv.u0, lv.uc, lv.b0, lv.bc, s );
+            PXD_CHECK( hipEventRecord( ready, s ) );
+            update( lv.uc, lv.u1, lv.bc, lv.b1, s );
         }
     }
     PXD_CHECK( hipGetLastError() );
@@ -477,6 +578,9 @@ pxd_create( pxd_ctx_t **pctx, int device )
     ctx->device = device;
     PXD_CHECK( hipSetDevice( device ) );
     PXD_CHECK( hipStreamCreateWithFlags( &ctx->stream, hipStreamNonBlocking ) );
+    PXD_CHECK( hipStreamCreateWithFlags( &ctx->stream2, hipStreamNonBlocking ) );
+    PXD_CHECK( hipEventCreateWithFlags( &ctx->ev_ready, hipEventDisableTiming ) );
+    PXD_CHECK( hipEventCreateWithFlags( &ctx->ev_panel, hipEventDisableTiming ) );
     hipDeviceProp_t prop;
     PXD_CHECK( hipGetDeviceProperties( &prop, device ) );
     ctx->lds_max = prop.sharedMemPerBlock;
@@ -501,7 +605,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dsolv, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
     for ( void *p : ptrs ) {
@@ -509,6 +613,10 @@ pxd_destroy( pxd_ctx_t *ctx )
             (void)hipFree( p );
         }
     }
+    (void)hipStreamSynchronize( ctx->stream2 );
+    (void)hipEventDestroy( ctx->ev_ready );
+    (void)hipEventDestroy( ctx->ev_panel );
+    (void)hipStreamDestroy( ctx->stream2 );
     (void)hipStreamDestroy( ctx->stream );
     delete ctx;
 }
@@ -536,8 +644,15 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
         return 0;
     }
     if ( k == "kernels" ) {
-        // takes effect at the next pxd_set_structure
-        ctx->kernels = ( value != 0. );
+        // takes effect at the next factorization
+        ctx->kernels     = ( value != 0. );
+        ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "upd_big_rows" ) {
+        // takes effect at the next factorization
+        ctx->upd_big_rows = (int)std::max( 1., std::min( value, 2e9 ) );
+        ctx->tasks_dirty  = 1;
         return 0;
     }
     return 7;
@@ -637,27 +752,12 @@ pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
             bb.rowoff = 0;
         }
     }
-    // the trsm tile keeps RT rows x n columns in LDS (static LDS of the kernel ~9 KB)
-    if ( ctx->kernels ) {
-        ctx->trsm_rt = 64;
-        if ( ctx->colmax > PXD_TRSM_NT * 16 ) {
-            fprintf( stderr, "pastix_amd device: cblk width %d above the %d columns of the TRSM tile\n",
-                     ctx->colmax, PXD_TRSM_NT * 16 );
-            return 3;
-        }
-    }
-    else {
-        ctx->trsm_rt = ( trsm_lds<double>( ctx->colmax, 32 ) + 12288 <= ctx->lds_max ) ? 32 : 16;
-    }
-    if ( !ctx->kernels && trsm_lds<double>( ctx->colmax, 16 ) + 12288 > ctx->lds_max ) {
-        fprintf( stderr, "pastix_amd device: cblk width %d exceeds the LDS budget of the TRSM tile\n",
-                 ctx->colmax );
-        return 3;
-    }
     if ( upload( &ctx->dcb, ctx->hcb ) || upload( &ctx->dbk, ctx->hbk ) || upload( &ctx->dcol2cblk, col2cblk ) ) {
         return 6;
     }
-    return build_schedule( ctx, cblks, bloks, browtab );
+    compute_levels( ctx, cblks, bloks, browtab );
+    ctx->tasks_dirty = 0;
+    return build_schedule( ctx );
 }
 
 int
@@ -749,6 +849,15 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
     ctx->fflt  = factor_flttype;
     ctx->facto = factotype;
     ctx->stats.coef_bytes = bytes;
+    const int bm = ( factor_flttype == FT_DOUBLE ) ? upd_rows<double>::value : upd_rows<float>::value;
+    if ( ctx->tasks_dirty || ctx->upd_bm != bm ) {
+        ctx->upd_bm      = bm;
+        ctx->tasks_dirty = 0;
+        int rc = build_schedule( ctx );
+        if ( rc ) {
+            return rc;
+        }
+    }
     if ( factor_flttype == FT_DOUBLE ) {
         return factorize_potrf<double>( ctx, threshold, nbpivots );
     }

@@ -16,6 +16,57 @@ struct inv_task {
     int32_t J;
 };
 
+// Largest i in [0, n) with a[i] <= key (a ascending, a[0] <= key).
+__device__ inline int32_t
+lds_search_le( const int32_t *a, int32_t n, int32_t key )
+{
+    int32_t lo = 0, hi = n - 1;
+    while ( lo < hi ) {
+        const int32_t mid = ( lo + hi + 1 ) >> 1;
+        if ( a[mid] <= key ) {
+            lo = mid;
+        }
+        else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+// Keys of the row searches of an update tile, staged in LDS with one parallel
+// load: stacked row offsets of the source bloks [b, lb) and first rows of the
+// facing cblk's bloks [ffb, flb).  Lists longer than PXD_SEARCH_CAP are
+// searched in global memory instead.
+#define PXD_SEARCH_CAP 512
+struct search_keys {
+    int32_t row[PXD_SEARCH_CAP];
+    int32_t frow[PXD_SEARCH_CAP];
+};
+
+__device__ inline void
+stage_search_keys( search_keys &sk, const dblok *__restrict__ bk, int32_t b, int32_t lb, int32_t ffb,
+                   int32_t flb, int tid )
+{
+    const int32_t ns = min( lb - b, PXD_SEARCH_CAP ), nf = min( flb - ffb, PXD_SEARCH_CAP );
+    for ( int32_t i = tid; i < ns; i += PXD_THREADS ) sk.row[i] = bk[b + i].rowoff;
+    for ( int32_t i = tid; i < nf; i += PXD_THREADS ) sk.frow[i] = bk[ffb + i].frownum;
+}
+
+// Source blok (in [b, lb)) holding stacked row s, and facing blok (in
+// [ffb, flb)) holding global row g, from the staged keys.
+__device__ inline int32_t
+search_row( const search_keys &sk, const dblok *__restrict__ bk, int32_t b, int32_t lb, int32_t s )
+{
+    return ( lb - b <= PXD_SEARCH_CAP ) ? b + lds_search_le( sk.row, lb - b, s ) : find_blok_by_row( bk, b, lb, s );
+}
+__device__ inline int32_t
+search_grow( const search_keys &sk, const dblok *__restrict__ bk, int32_t ffb, int32_t flb, int32_t g )
+{
+    return ( flb - ffb <= PXD_SEARCH_CAP ) ? ffb + lds_search_le( sk.frow, flb - ffb, g )
+                                           : find_blok_by_grow( bk, ffb, flb, g );
+}
+
+
 // ---------------------------------------------------------------------------
 // Panel: Cholesky of the diagonal tile with static pivoting, right looking by
 // PXD_NB columns; the unblocked step updates with unscaled columns so that it
@@ -44,10 +95,20 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     for ( int32_t kb = 0; kb < n; kb += PXD_NB ) {
         const int32_t bs = min( PXD_NB, n - kb );
 
-        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
-            int r = idx % PXD_NB, cc = idx / PXD_NB;
-            sA[cc][r] = ( r < bs && cc < bs && r >= cc ) ? A[( kb + r ) + (int64_t)( kb + cc ) * n] : (T)0;
-            sI[cc][r] = (T)0;
+        {
+            constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
+            T v[NU];
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                v[u] = ( r < bs && cc < bs && r >= cc ) ? A[( kb + r ) + (int64_t)( kb + cc ) * n] : (T)0;
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                sA[cc][r] = v[u];
+                sI[cc][r] = (T)0;
+            }
         }
         __syncthreads();
 
@@ -107,29 +168,36 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             LI[( kb + rr ) + (int64_t)( kb + cc ) * n] = sI[cc][rr];
         }
 
-        // rows below the block inside the tile: A21 <- A21 * L11^{-T}; each
-        // thread keeps its row's results in registers until all are computed
-        for ( int32_t r = kb + bs + tid; r < n; r += PXD_THREADS ) {
-            T out[PXD_NB];
+        // rows below the block inside the tile: A21 <- A21 * L11^{-T} on the
+        // matrix cores, one wave per 16-row block (two 16 x 16 output tiles,
+        // the inverse's transpose read from LDS); a block's loads all precede
+        // its stores and no other wave touches its rows
+        {
+            typedef mfma16<T>         M;
+            typedef typename M::acc_t acc_t;
+            const int lane = tid & 63, w = tid >> 6;
+            for ( int32_t rb = kb + bs + 16 * w; rb < n; rb += 64 ) {
+                T a[PXD_NB / 4];
+                const int32_t ra = rb + ( lane & 15 );
 #pragma unroll
-            for ( int c0 = 0; c0 < PXD_NB; c0 += 8 ) {
-                T acc[8];
+                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                    const int l = 4 * s4 + ( lane >> 4 );
+                    a[s4] = ( ra < n && l < bs ) ? A[ra + (int64_t)( kb + l ) * n] : (T)0;
+                }
+                acc_t acc0 = M::zero(), acc1 = M::zero();
 #pragma unroll
-                for ( int j = 0; j < 8; j++ ) acc[j] = (T)0;
-                const int lend = min( c0 + 8, (int)bs );
-                for ( int l = 0; l < lend; l++ ) {
-                    const T a = A[r + (int64_t)( kb + l ) * n];
-#pragma unroll
-                    for ( int j = 0; j < 8; j++ ) {
-                        if ( l <= c0 + j ) acc[j] += a * sI[l][c0 + j];
-                    }
+                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                    const int l = 4 * s4 + ( lane >> 4 );
+                    acc0 = M::mma( a[s4], sI[l][( lane & 15 )], acc0 );
+                    acc1 = M::mma( a[s4], sI[l][16 + ( lane & 15 )], acc1 );
                 }
 #pragma unroll
-                for ( int j = 0; j < 8; j++ ) out[c0 + j] = acc[j];
-            }
-#pragma unroll
-            for ( int cc = 0; cc < PXD_NB; cc++ ) {
-                if ( cc < bs ) A[r + (int64_t)( kb + cc ) * n] = out[cc];
+                for ( int v = 0; v < 4; v++ ) {
+                    const int32_t rr = rb + M::row( lane, v );
+                    const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
+                    if ( rr < n && c0 < bs ) A[rr + (int64_t)( kb + c0 ) * n] = acc0[v];
+                    if ( rr < n && c1 < bs ) A[rr + (int64_t)( kb + c1 ) * n] = acc1[v];
+                }
             }
         }
         __syncthreads();
@@ -141,11 +209,22 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         const int     tx = tid % 16, ty = tid / 16;
         for ( int32_t ti = 0; ti < nt; ti++ ) {
             for ( int32_t tj = 0; tj <= ti; tj++ ) {
-                for ( int idx = tid; idx < 64 * PXD_NB; idx += PXD_THREADS ) {
-                    int r = idx % 64, l = idx / 64;
-                    int32_t ri = ti * 64 + r, rj = tj * 64 + r;
-                    sPi[l][r] = ( ri < m && l < bs ) ? A[( m0 + ri ) + (int64_t)( kb + l ) * n] : (T)0;
-                    sPj[l][r] = ( rj < m && l < bs ) ? A[( m0 + rj ) + (int64_t)( kb + l ) * n] : (T)0;
+                {
+                    constexpr int NU = 64 * PXD_NB / PXD_THREADS;
+                    T vi[NU], vj[NU];
+#pragma unroll
+                    for ( int u = 0; u < NU; u++ ) {
+                        const int     idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
+                        const int32_t ri = ti * 64 + r, rj = tj * 64 + r;
+                        vi[u] = ( ri < m && l < bs ) ? A[( m0 + ri ) + (int64_t)( kb + l ) * n] : (T)0;
+                        vj[u] = ( rj < m && l < bs ) ? A[( m0 + rj ) + (int64_t)( kb + l ) * n] : (T)0;
+                    }
+#pragma unroll
+                    for ( int u = 0; u < NU; u++ ) {
+                        const int idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
+                        sPi[l][r] = vi[u];
+                        sPj[l][r] = vj[u];
+                    }
                 }
                 __syncthreads();
                 T acc[4][4];
@@ -165,13 +244,22 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 #pragma unroll
                         for ( int b = 0; b < 4; b++ ) acc[a][b] += ai[a] * aj[b];
                 }
+                T old[4][4];
+#pragma unroll
+                for ( int a = 0; a < 4; a++ ) {
+#pragma unroll
+                    for ( int b = 0; b < 4; b++ ) {
+                        int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
+                        old[a][b] = ( r < m && cc < m && r >= cc ) ? A[( m0 + r ) + (int64_t)( m0 + cc ) * n] : (T)0;
+                    }
+                }
 #pragma unroll
                 for ( int a = 0; a < 4; a++ ) {
 #pragma unroll
                     for ( int b = 0; b < 4; b++ ) {
                         int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
                         if ( r < m && cc < m && r >= cc ) {
-                            A[( m0 + r ) + (int64_t)( m0 + cc ) * n] -= acc[a][b];
+                            A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[a][b] - acc[a][b];
                         }
                     }
                 }
@@ -217,10 +305,21 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
         const int r = tid % PXD_NB, c0 = tid / PXD_NB;
         for ( int32_t K = J; K < I; K += PXD_NB ) {
             const int32_t bk = min( PXD_NB, n - K );
-            for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
-                int rr = idx % PXD_NB, cc = idx / PXD_NB;
-                sL[cc][rr] = ( rr < bi && cc < bk ) ? A[( I + rr ) + (int64_t)( K + cc ) * n] : (T)0;
-                sX[cc][rr] = ( rr < bk && cc < bj ) ? LI[( K + rr ) + (int64_t)( J + cc ) * n] : (T)0;
+            {
+                constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
+                T vl[NU], vx[NU];
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
+                    vl[u] = ( rr < bi && cc < bk ) ? A[( I + rr ) + (int64_t)( K + cc ) * n] : (T)0;
+                    vx[u] = ( rr < bk && cc < bj ) ? LI[( K + rr ) + (int64_t)( J + cc ) * n] : (T)0;
+                }
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
+                    sL[cc][rr] = vl[u];
+                    sX[cc][rr] = vx[u];
+                }
             }
             __syncthreads();
 #pragma unroll 8
@@ -234,9 +333,19 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 #pragma unroll
         for ( int q = 0; q < 4; q++ ) sT[c0 + 8 * q][r] = acc[q];
         // Linv[I,I] into sL
-        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
-            int rr = idx % PXD_NB, cc = idx / PXD_NB;
-            sL[cc][rr] = ( rr < bi && cc < bi ) ? LI[( I + rr ) + (int64_t)( I + cc ) * n] : (T)0;
+        {
+            constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
+            T vl[NU];
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
+                vl[u] = ( rr < bi && cc < bi ) ? LI[( I + rr ) + (int64_t)( I + cc ) * n] : (T)0;
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
+                sL[cc][rr] = vl[u];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -307,14 +416,28 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const int32_t lld   = sLd[lr];
 
     for ( int32_t l0 = 0; l0 < n; l0 += BK ) {
-        for ( int kk = tid >> 6; kk < BK; kk += 4 ) {
-            int32_t l = l0 + kk;
-            sB[kk][lr] = ( lbase >= 0 && l < n ) ? L[lbase + (int64_t)l * lld] : (T)0;
+        // all loads of the stage are issued before the first LDS store
+        T rb[BK / 4];
+#pragma unroll
+        for ( int u = 0; u < BK / 4; u++ ) {
+            const int32_t l = l0 + ( tid >> 6 ) + 4 * u;
+            rb[u] = ( lbase >= 0 && l < n ) ? L[lbase + (int64_t)l * lld] : (T)0;
         }
-        for ( int idx = tid; idx < BK * PXD_TRSM_NT * 16; idx += PXD_THREADS ) {
-            int cc = idx % ( PXD_TRSM_NT * 16 ), kk = idx / ( PXD_TRSM_NT * 16 );
-            int32_t l = l0 + kk;
-            sLi[kk][cc] = ( cc < n && l < n ) ? LI[cc + (int64_t)l * n] : (T)0;
+        constexpr int NLI = ( BK * PXD_TRSM_NT * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
+        T rl[NLI];
+#pragma unroll
+        for ( int u = 0; u < NLI; u++ ) {
+            const int idx = tid + u * PXD_THREADS;
+            const int cc = idx % ( PXD_TRSM_NT * 16 ), kk = idx / ( PXD_TRSM_NT * 16 );
+            const int32_t l = l0 + kk;
+            rl[u] = ( kk < BK && cc < n && l < n && 16 * ( cc >> 4 ) + 15 >= l ) ? LI[cc + (int64_t)l * n] : (T)0;
+        }
+#pragma unroll
+        for ( int u = 0; u < BK / 4; u++ ) sB[( tid >> 6 ) + 4 * u][lr] = rb[u];
+#pragma unroll
+        for ( int u = 0; u < NLI; u++ ) {
+            const int idx = tid + u * PXD_THREADS;
+            if ( idx < BK * PXD_TRSM_NT * 16 ) sLi[idx / ( PXD_TRSM_NT * 16 )][idx % ( PXD_TRSM_NT * 16 )] = rl[u];
         }
         __syncthreads();
 #pragma unroll
@@ -364,6 +487,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ int32_t sSrcLd[64];
     __shared__ int64_t sDst[64];
     __shared__ int32_t sDstLd[64];
+    __shared__ search_keys sk;
 
     const upd_task t    = tasks[blockIdx.x];
     const int      tid  = threadIdx.x;
@@ -384,24 +508,28 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
         return;   // inconsistent structure: the host checks prevent this
     }
 
+    stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
+    __syncthreads();
     if ( tid < 64 ) {
         int32_t r = t.tm * 64 + tid;
         if ( r < Mr ) {
             int32_t s  = bb.rowoff + r;
-            int32_t bi = find_blok_by_row( bk, t.blok, lb, s );
+            int32_t bi = search_row( sk, bk, t.blok, lb, s );
             dblok   ib = bk[bi];
             int32_t g  = ib.frownum + ( s - ib.rowoff );
             sSrc[tid]   = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
             sSrcLd[tid] = ib.lrownum - ib.frownum + 1;
-            int32_t fi  = find_blok_by_grow( bk, cf.fblok, flb, g );
+            int32_t fi  = search_grow( sk, bk, cf.fblok, flb, g );
             dblok   fb  = bk[fi];
             bool    ok  = ( g >= fb.frownum && g <= fb.lrownum );
             sDst[tid]   = ok ? cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum ) : -1;
             sDstLd[tid] = fb.lrownum - fb.frownum + 1;
         }
         else {
-            sSrc[tid] = -1;
-            sDst[tid] = -1;
+            sSrc[tid]   = -1;
+            sSrcLd[tid] = 0;
+            sDst[tid]   = -1;
+            sDstLd[tid] = 0;
         }
     }
     __syncthreads();
@@ -441,32 +569,252 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
         __syncthreads();
     }
 
+    // scatter into F: all loads of the read-modify-writes first, then the stores
     const bool atomic = ( t.flags & 1 );
+    T         *p[2][4];
+    bool       pv[2][2][4];
 #pragma unroll
     for ( int i = 0; i < 2; i++ ) {
 #pragma unroll
-        for ( int j = 0; j < 2; j++ ) {
-            const int cc = wc + 16 * j + ( lane & 15 );
+        for ( int v = 0; v < 4; v++ ) {
+            const int r = wr + 16 * i + M::row( lane, v );
+            p[i][v]     = L + ( sDst[r] >= 0 ? sDst[r] : 0 ) + (int64_t)colF0 * sDstLd[r];
+#pragma unroll
+            for ( int j = 0; j < 2; j++ ) pv[i][j][v] = sDst[r] >= 0 && wc + 16 * j + ( lane & 15 ) < nvalid;
+        }
+    }
+    T old[2][2][4];
+#pragma unroll
+    for ( int i = 0; i < 2; i++ )
+#pragma unroll
+        for ( int j = 0; j < 2; j++ )
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
-                const int r = wr + 16 * i + M::row( lane, v );
-                if ( sDst[r] >= 0 && cc < nvalid ) {
-                    T *dst = L + sDst[r] + (int64_t)( colF0 + cc ) * sDstLd[r];
-                    if ( atomic ) {
-                        px_atomic_add( dst, -acc[i][j][v] );
-                    }
-                    else {
-                        *dst -= acc[i][j][v];
-                    }
+                const int r  = wr + 16 * i + M::row( lane, v );
+                const int cc = wc + 16 * j + ( lane & 15 );
+                T *dst = p[i][v] + (int64_t)cc * sDstLd[r];
+                if ( atomic ) {
+                    if ( pv[i][j][v] ) px_atomic_add( dst, -acc[i][j][v] );
+                }
+                else {
+                    old[i][j][v] = pv[i][j][v] ? *dst : (T)0;
                 }
             }
+    if ( !atomic ) {
+#pragma unroll
+        for ( int i = 0; i < 2; i++ )
+#pragma unroll
+            for ( int j = 0; j < 2; j++ )
+#pragma unroll
+                for ( int v = 0; v < 4; v++ ) {
+                    const int r  = wr + 16 * i + M::row( lane, v );
+                    const int cc = wc + 16 * j + ( lane & 15 );
+                    if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - acc[i][j][v];
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Supernodal update for tall tiles: C[f(i)] -= A_i * A_b^T for upd_rows<T>
+// stacked rows x PXD_UPD_BN facing columns, wave w owning rows
+// [w*BM/4, (w+1)*BM/4).  The next k-stage is loaded into registers while the
+// current one is on the matrix cores.  LDS holds each row's k values
+// contiguously, and the k values of one MFMA step are permuted (lane group g
+// takes k = KV*g + s in step s) so that every fragment is one 16-byte read.
+// C[rows, cols] -= acc for one 16 x 64 row block of an update tile: row r of
+// the block goes to L + dst[r] + col * ld[r], columns colF0 + [0, nvalid).
+template <typename T>
+__device__ inline void
+update_scatter( T *__restrict__ L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int lane,
+                const int64_t *dst, const int32_t *ld, int32_t colF0, int32_t nvalid, bool atomic )
+{
+    typedef mfma16<T> M;
+    T      *p[4];
+    int32_t pl[4];
+    bool    pv[4];
+#pragma unroll
+    for ( int v = 0; v < 4; v++ ) {
+        const int r = r0 + M::row( lane, v );
+        pv[v]       = dst[r] >= 0;
+        pl[v]       = ld[r];
+        p[v]        = L + ( pv[v] ? dst[r] : 0 ) + (int64_t)colF0 * pl[v];
+    }
+    if ( atomic ) {
+#pragma unroll
+        for ( int j = 0; j < 4; j++ ) {
+            const int cc = 16 * j + ( lane & 15 );
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) {
+                if ( pv[v] && cc < nvalid ) px_atomic_add( p[v] + (int64_t)cc * pl[v], -acc[j][v] );
+            }
         }
+        return;
+    }
+    T old[4][4];
+#pragma unroll
+    for ( int j = 0; j < 4; j++ ) {
+        const int cc = 16 * j + ( lane & 15 );
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) old[j][v] = ( pv[v] && cc < nvalid ) ? p[v][(int64_t)cc * pl[v]] : (T)0;
+    }
+#pragma unroll
+    for ( int j = 0; j < 4; j++ ) {
+        const int cc = 16 * j + ( lane & 15 );
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) {
+            if ( pv[v] && cc < nvalid ) p[v][(int64_t)cc * pl[v]] = old[j][v] - acc[j][v];
+        }
+    }
+}
+
+#define PXD_UPD_BN 64
+template <typename T> struct upd_rows;
+template <> struct upd_rows<float>  { static constexpr int value = 256; };
+template <> struct upd_rows<double> { static constexpr int value = 128; };
+
+template <typename T>
+__global__ void __launch_bounds__( PXD_THREADS, 3 )
+k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
+              const dblok *__restrict__ bk, T *__restrict__ L )
+{
+    typedef mfma16<T>          M;
+    typedef typename M::acc_t  acc_t;
+    typedef typename lds_vec<T>::type vec_t;
+    constexpr int BM  = upd_rows<T>::value;
+    constexpr int KV  = lds_vec<T>::n;    // k values per fragment
+    constexpr int BK  = 4 * KV;           // k values per stage
+    constexpr int LDK = BK + KV;          // row stride of the staged tiles
+    constexpr int TPR = PXD_THREADS / BM; // loader threads per row
+    constexpr int KPT = BK / TPR;         // k values loaded per thread
+    constexpr int RB  = BM / 64;          // 16-row blocks per wave
+    __shared__ __align__( 16 ) T sA[BM * LDK];
+    __shared__ __align__( 16 ) T sB[PXD_UPD_BN * LDK];
+    __shared__ int64_t sDst[BM];
+    __shared__ int32_t sDstLd[BM];
+    __shared__ search_keys sk;
+
+    const upd_task t    = tasks[xcd_task( blockIdx.x, ntasks )];
+    const int      tid  = threadIdx.x;
+    const int      lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    const dcblk    c    = cb[t.cblk];
+    const int32_t  K    = c.lcolnum - c.fcolnum + 1;
+    const int32_t  lb   = cb[t.cblk + 1].fblok;
+    const dblok    bb   = bk[t.blok];
+    const int32_t  N    = bb.lrownum - bb.frownum + 1;
+    const int32_t  Mr   = c.stride - bb.rowoff;
+    const int32_t  F    = bb.fcblknm;
+    const dcblk    cf   = cb[F];
+    const int32_t  Fn   = cf.lcolnum - cf.fcolnum + 1;
+    const int32_t  flb  = cb[F + 1].fblok;
+    const int32_t  colF0 = bb.frownum - cf.fcolnum + t.tn * PXD_UPD_BN;
+    if ( bb.frownum < cf.fcolnum || bb.lrownum > cf.lcolnum ) {
+        return;   // inconsistent structure: the host checks prevent this
+    }
+
+    stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
+    __syncthreads();
+
+    // loader row ar (k values [ap*KPT, ap*KPT+KPT) of each stage); the
+    // threads with ap == 0 also map the row into F
+    const int ar = tid % BM, ap = tid / BM;
+    int64_t   src   = -1;
+    int32_t   srcld = 0;
+    {
+        const int32_t r = t.tm * BM + ar;
+        int64_t dst = -1;
+        int32_t dld = 0;
+        if ( r < Mr ) {
+            const int32_t s  = bb.rowoff + r;
+            const dblok   ib = bk[search_row( sk, bk, t.blok, lb, s )];
+            const int32_t g  = ib.frownum + ( s - ib.rowoff );
+            src   = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
+            srcld = ib.lrownum - ib.frownum + 1;
+            if ( ap == 0 ) {
+                const dblok fb = bk[search_grow( sk, bk, cf.fblok, flb, g )];
+                dst = ( g >= fb.frownum && g <= fb.lrownum ) ? cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum )
+                                                              : -1;
+                dld = fb.lrownum - fb.frownum + 1;
+            }
+        }
+        if ( ap == 0 ) {
+            sDst[ar]   = dst;
+            sDstLd[ar] = dld;
+        }
+    }
+    // B loader: facing column bc, k group bq
+    const int     bc     = tid & ( PXD_UPD_BN - 1 ), bq = tid / PXD_UPD_BN;
+    const int32_t nvalid = min( PXD_UPD_BN, N - t.tn * PXD_UPD_BN );
+    const bool    bok    = bc < nvalid;
+    const T      *bsrc   = L + c.coefoff + (int64_t)bb.rowoff * K + t.tn * PXD_UPD_BN + bc;
+
+    T ra[KPT], rb[KV];
+    auto load = [&]( int32_t k0 ) {
+#pragma unroll
+        for ( int q = 0; q < KPT; q++ ) {
+            const int32_t kg = k0 + ap * KPT + q;
+            ra[q] = ( src >= 0 && kg < K ) ? L[src + (int64_t)kg * srcld] : (T)0;
+        }
+#pragma unroll
+        for ( int q = 0; q < KV; q++ ) {
+            const int32_t kg = k0 + bq * KV + q;
+            rb[q] = ( bok && kg < K ) ? bsrc[(int64_t)kg * N] : (T)0;
+        }
+    };
+
+    acc_t acc[RB][4];
+#pragma unroll
+    for ( int i = 0; i < RB; i++ )
+#pragma unroll
+        for ( int j = 0; j < 4; j++ ) acc[i][j] = M::zero();
+
+    load( 0 );
+    for ( int32_t k0 = 0; k0 < K; k0 += BK ) {
+#pragma unroll
+        for ( int q = 0; q < KPT; q += KV ) {
+            vec_t v;
+#pragma unroll
+            for ( int e = 0; e < KV; e++ ) v[e] = ra[q + e];
+            *(vec_t *)&sA[ar * LDK + ap * KPT + q] = v;
+        }
+        {
+            vec_t v;
+#pragma unroll
+            for ( int e = 0; e < KV; e++ ) v[e] = rb[e];
+            *(vec_t *)&sB[bc * LDK + bq * KV] = v;
+        }
+        __syncthreads();
+        if ( k0 + BK < K ) {
+            load( k0 + BK );
+        }
+        vec_t av[RB], bv[4];
+#pragma unroll
+        for ( int i = 0; i < RB; i++ ) av[i] = *(const vec_t *)&sA[( 16 * ( RB * w + i ) + lr ) * LDK + KV * lg];
+#pragma unroll
+        for ( int j = 0; j < 4; j++ ) bv[j] = *(const vec_t *)&sB[( 16 * j + lr ) * LDK + KV * lg];
+#pragma unroll
+        for ( int s = 0; s < KV; s++ )
+#pragma unroll
+            for ( int i = 0; i < RB; i++ )
+#pragma unroll
+                for ( int j = 0; j < 4; j++ ) acc[i][j] = M::mma( av[i][s], bv[j][s], acc[i][j] );
+        __syncthreads();
+    }
+
+    // scatter into F: the 16 read-modify-writes of a row block are issued as
+    // 16 loads, then 16 stores, so their memory latencies overlap
+    const bool atomic = ( t.flags & 1 );
+#pragma unroll
+    for ( int i = 0; i < RB; i++ ) {
+        update_scatter<T>( L, acc[i], 16 * ( RB * w + i ), lane, sDst, sDstLd, colF0, nvalid, atomic );
     }
 }
 
 // ---------------------------------------------------------------------------
 // Diagonal solves with the explicit inverse: x_k <- Linv x_k (forward) or
-// Linv^T x_k (backward) for a list of cblks, nrhs <= PXD_MAXRHS.
+// Linv^T x_k (backward) for a list of cblks, nrhs <= PXD_MAXRHS.  The
+// product runs over 64 x 64 chunks of the operator staged in LDS (loads
+// coalesced along the columns of Linv in both orientations); chunks in the
+// zero triangle of the operator are skipped.
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_diag_inv( const int32_t *__restrict__ list, const dcblk *__restrict__ cb,
@@ -474,6 +822,8 @@ k_solve_diag_inv( const int32_t *__restrict__ list, const dcblk *__restrict__ cb
 {
     extern __shared__ unsigned char smem_raw[];
     T *sb = reinterpret_cast<T *>( smem_raw );   // [n][nrhs]
+    __shared__ T sM[64][64 + 1];                 // operator chunk, sM[row][col]; then partial sums
+    T           *sred = &sM[0][0];               // [4][64][PXD_MAXRHS] after the chunk loop
 
     const int32_t k   = list[blockIdx.x];
     const dcblk   c   = cb[k];
@@ -481,48 +831,65 @@ k_solve_diag_inv( const int32_t *__restrict__ list, const dcblk *__restrict__ cb
     const int     tid = threadIdx.x;
     const T      *LI  = Linv + c.dinvoff;
     T            *x   = X + c.fcolnum;
+    const int     r   = tid & 63, part = tid >> 6;
 
     for ( int idx = tid; idx < n * nrhs; idx += PXD_THREADS ) {
-        int r = idx % n, q = idx / n;
-        sb[r * nrhs + q] = x[r + q * ldx];
+        int rr = idx % n, q = idx / n;
+        sb[rr * nrhs + q] = x[rr + q * ldx];
     }
     __syncthreads();
-    if ( !trans ) {
-        // x[r] = sum_{l<=r} Linv[r][l] b[l]; consecutive threads read consecutive rows
-        for ( int32_t r = tid; r < n; r += PXD_THREADS ) {
-            T acc[PXD_MAXRHS];
+
+    for ( int32_t R0 = 0; R0 < n; R0 += 64 ) {
+        T acc[PXD_MAXRHS];
 #pragma unroll
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
-#pragma unroll 4
-            for ( int32_t l = 0; l <= r; l++ ) {
-                T a = LI[r + (int64_t)l * n];
+        for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
+        // operator M = Linv (lower) or Linv^T (upper): chunk columns C0 <= R0 or >= R0
+        const int32_t cfirst = trans ? R0 : 0, clast = trans ? n : R0 + 1;
+        for ( int32_t C0 = cfirst; C0 < clast; C0 += 64 ) {
+            constexpr int NU = 64 * 64 / PXD_THREADS;
+            T             v[NU];
 #pragma unroll
-                for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                    if ( q < nrhs ) acc[q] += a * sb[l * nrhs + q];
+            for ( int u = 0; u < NU; u++ ) {
+                // lane i = tid & 63 walks the contiguous dimension of Linv
+                const int     i = tid & 63, j = ( tid >> 6 ) + 4 * u;
+                const int32_t gi = ( trans ? C0 : R0 ) + i, gj = ( trans ? R0 : C0 ) + j;
+                v[u] = ( gi < n && gj < n && gi >= gj ) ? LI[gi + (int64_t)gj * n] : (T)0;
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int i = tid & 63, j = ( tid >> 6 ) + 4 * u;
+                if ( trans ) {
+                    sM[j][i] = v[u];   // M[R0 + j][C0 + i] = Linv[C0 + i][R0 + j]
+                }
+                else {
+                    sM[i][j] = v[u];   // M[R0 + i][C0 + j] = Linv[R0 + i][C0 + j]
                 }
             }
-            for ( int q = 0; q < nrhs; q++ ) x[r + q * ldx] = acc[q];
+            __syncthreads();
+#pragma unroll
+            for ( int cc = 16 * part; cc < 16 * part + 16; cc++ ) {
+                if ( C0 + cc < n ) {
+                    const T m = sM[r][cc];
+#pragma unroll
+                    for ( int q = 0; q < PXD_MAXRHS; q++ ) {
+                        if ( q < nrhs ) acc[q] += m * sb[( C0 + cc ) * nrhs + q];
+                    }
+                }
+            }
+            __syncthreads();
         }
-    }
-    else {
-        // x[cc] = sum_{l>=cc} Linv[l][cc] b[l]; one wave per column, lanes along the column
-        const int lane = tid & 63, w = tid >> 6;
-        for ( int32_t cc = w; cc < n; cc += PXD_THREADS / 64 ) {
-            T acc[PXD_MAXRHS];
 #pragma unroll
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
-            for ( int32_t l = cc + lane; l < n; l += 64 ) {
-                T a = LI[l + (int64_t)cc * n];
-#pragma unroll
-                for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                    if ( q < nrhs ) acc[q] += a * sb[l * nrhs + q];
-                }
-            }
+        for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[( part * 64 + r ) * PXD_MAXRHS + q] = acc[q];
+        __syncthreads();
+        if ( part == 0 && R0 + r < n ) {
             for ( int q = 0; q < nrhs; q++ ) {
-                T s = wave_sum( acc[q] );
-                if ( lane == 0 ) x[cc + q * ldx] = s;
+                T v = (T)0;
+#pragma unroll
+                for ( int p2 = 0; p2 < 4; p2++ ) v += sred[( p2 * 64 + r ) * PXD_MAXRHS + q];
+                x[R0 + r + q * ldx] = v;
             }
         }
+        __syncthreads();
     }
 }
 
@@ -566,9 +933,19 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
     __syncthreads();
     const int col = tid & 63, part = tid >> 6;
     for ( int32_t c0 = 0; c0 < n; c0 += 64 ) {
-        for ( int idx = tid; idx < 64 * 64; idx += PXD_THREADS ) {
-            int r = idx & 63, cc = idx >> 6;
-            sA[cc][r] = ( sBase[r] >= 0 && c0 + cc < n ) ? L[sBase[r] + (int64_t)( c0 + cc ) * sLd[r]] : (T)0;
+        {
+            constexpr int NU = 64 * 64 / PXD_THREADS;
+            const int     r  = tid & 63;
+            const int64_t rb = sBase[r];
+            const int32_t rl = sLd[r];
+            T             v[NU];
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int cc = ( tid >> 6 ) + 4 * u;
+                v[u] = ( rb >= 0 && c0 + cc < n ) ? L[rb + (int64_t)( c0 + cc ) * rl] : (T)0;
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) sA[( tid >> 6 ) + 4 * u][r] = v[u];
         }
         __syncthreads();
         T acc[PXD_MAXRHS];

@@ -124,12 +124,20 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
 #pragma unroll
     for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
     if ( valid ) {
-        int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
-        for ( int32_t cc = c0; cc < c1; cc++ ) {
-            T a = L[base + (int64_t)cc * ld];
+        // 16 columns per batch: the loads of a batch are all in flight at once
+        const int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
+        for ( int32_t cb0 = c0; cb0 < c1; cb0 += 16 ) {
+            T a[16];
 #pragma unroll
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                if ( q < nrhs ) acc[q] += a * sx[cc * nrhs + q];
+            for ( int u = 0; u < 16; u++ ) a[u] = ( cb0 + u < c1 ) ? L[base + (int64_t)( cb0 + u ) * ld] : (T)0;
+#pragma unroll
+            for ( int u = 0; u < 16; u++ ) {
+                if ( cb0 + u < c1 ) {
+#pragma unroll
+                    for ( int q = 0; q < PXD_MAXRHS; q++ ) {
+                        if ( q < nrhs ) acc[q] += a[u] * sx[( cb0 + u ) * nrhs + q];
+                    }
+                }
             }
         }
     }

@@ -30,3 +30,24 @@ template <> struct mfma16<double> {
     }
     __device__ static inline int row( int lane, int v ) { return ( lane >> 4 ) + 4 * v; }
 };
+
+// 16-byte LDS fragments: KV consecutive k values of one row
+template <typename T> struct lds_vec;
+template <> struct lds_vec<float> {
+    typedef float type __attribute__( ( ext_vector_type( 4 ) ) );
+    static constexpr int n = 4;
+};
+template <> struct lds_vec<double> {
+    typedef double type __attribute__( ( ext_vector_type( 2 ) ) );
+    static constexpr int n = 2;
+};
+
+// Task of workgroup b among nb: the workgroups that the dispatcher sends to
+// one XCD (b % 8) take a contiguous range of the task list, so that tiles
+// that share source rows share that XCD's L2.
+__device__ inline int32_t
+xcd_task( int32_t b, int32_t nb )
+{
+    const int32_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return x * q + min( x, r ) + i;
+}

@@ -666,8 +666,10 @@ pastix_subtask_sopalin( pastix_data_t *pd )
     fflt  = px_factor_flttype( pd );
 
     /* static pivoting threshold (sopalin/pastix_task_sopalin.c:461-485); the
-     * epsilon is that of the factor precision (the reference takes the bcsc
-     * precision; see DESIGN: mixed precision uses the single epsilon) */
+     * epsilon is that of the factor precision, where the reference takes the
+     * bcsc precision: a single precision factor of a double matrix uses the
+     * single epsilon, so tiny pivots are judged at the precision they are
+     * stored in */
     if ( dparm[DPARM_EPSILON_MAGN_CTRL] < 0. ) {
         threshold = -dparm[DPARM_EPSILON_MAGN_CTRL];
     }

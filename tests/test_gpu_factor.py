@@ -59,6 +59,26 @@ def test_solve_fp64_backward_error(nx, nrhs, kernels):
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("big_rows,timing", [(1, 0), (10 ** 9, 0), (192, 1)])
+def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing):
+    """Every blok update on the tall tiles (big_rows=1), every one on the
+    64 x 64 tiles (big_rows=1e9), and the single-stream timed schedule
+    instead of the two-stream lookahead: the factor matches the oracle."""
+    nx = 18
+    mixed = int(dtype == np.float32)
+    spm = make_problem("lap7", nx)
+    pd, iparm, dparm = analyzed(spm, nx, mixed=mixed)
+    px.set_option(pd, "upd_big_rows", big_rows)
+    px.set_option(pd, "timing", timing)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    L = px.get_coeftab(pd, s["coefnbr"], dtype)
+    F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype)
+    assert cblk_diff(s, L, F.L, dtype) <= 10.0
+    px.finalize(pd)
+
+
 def test_device_solve_matches_oracle_solve():
     nx = 12
     spm, pd, iparm, dparm = _factor_case(nx)
