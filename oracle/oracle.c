@@ -19,6 +19,9 @@
  *   solve              kernels/solve_ztrsmsp.c:74-625
  *   spmv               bcsc/bcsc_zspmv.c:57-76
  *   gmres              refinement/z_refine_gmres.c:48-371
+ *   LU                 kernels/core_zgetrfsp.c:64-184 (getf2 / 64-blocked getrf
+ *                      with static pivot), :216-272 (geadd, getrf, getmo),
+ *                      :354-397 (two TRSMs, updates of both sides)
  *
  * Structure arrays: cblk[6*(cblknbr+1)] = {fcolnum, lcolnum, fblok, stride,
  * coefoff, brownum}, blok[5*bloknbr] = {frownum, lrownum, lcblknm, fcblknm,
@@ -441,6 +444,247 @@ orc_factorize( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_
     free( s.locks );
     free( (void *)s.ctrbcnt );
     free( col2cblk );
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LU with static pivoting (kernels/core_zgetrfsp.c), sequential in cblk     */
+/* order (every source of cblk k precedes k).  The U side holds U^T in the   */
+/* layout of L (ucoeftab).                                                   */
+
+#define ORC_LU_TYPED( T, SFX, GEMM, FABS )                                                          \
+    /* U tiles: A[j][i] at (i, j) for i > j (the strict upper part, transposed) */                  \
+    static void fillu_##SFX( int64_t n, const int64_t *cblk, const int64_t *blok,                   \
+                             const int64_t *tcolptr, const int64_t *trowtab, const double *tval,    \
+                             const int64_t *col2cblk, T *U )                                        \
+    {                                                                                               \
+        for ( int64_t j = 0; j < n; j++ ) {                                                         \
+            int64_t k = col2cblk[j];                                                                \
+            int64_t b = CB( k, C_FBLOK ), lb = CB( k + 1, C_FBLOK );                                \
+            for ( int64_t e = tcolptr[j]; e < tcolptr[j + 1]; e++ ) {                               \
+                int64_t i = trowtab[e];                                                             \
+                if ( i <= j ) continue;                                                             \
+                while ( b < lb && ( BK( b, B_LROW ) < i || BK( b, B_FROW ) > i ) ) b++;             \
+                if ( b >= lb ) continue;                                                            \
+                int64_t rows = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                               \
+                U[CB( k, C_COEFOFF ) + BK( b, B_COEFIND ) + ( i - BK( b, B_FROW ) ) +               \
+                  ( j - CB( k, C_FCOL ) ) * rows] = (T)tval[e];                                     \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* unblocked LU of an m x n panel, static pivot +-tau (core_zgetf2sp) */                        \
+    static void getf2_##SFX( int64_t m, int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )   \
+    {                                                                                               \
+        int64_t mn = ( m < n ) ? m : n;                                                             \
+        for ( int64_t k = 0; k < mn; k++ ) {                                                        \
+            T *akk = A + k * ( lda + 1 );                                                           \
+            if ( FABS( *akk ) < (T)tau ) {                                                          \
+                *akk = ( *akk < (T)0 ) ? (T)-tau : (T)tau;                                          \
+                ( *npiv )++;                                                                        \
+            }                                                                                       \
+            T a = (T)1 / *akk;                                                                      \
+            for ( int64_t i = k + 1; i < m; i++ ) A[i + k * lda] *= a;                              \
+            for ( int64_t j = k + 1; j < n; j++ ) {                                                 \
+                T f = A[k + j * lda];                                                               \
+                for ( int64_t i = k + 1; i < m; i++ ) A[i + j * lda] -= A[i + k * lda] * f;         \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* blocked (64) LU of the diagonal tile (core_zgetrfsp) */                                      \
+    static void getrf_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
+    {                                                                                               \
+        for ( int64_t k = 0; k < n; k += 64 ) {                                                     \
+            int64_t bs = ( n - k < 64 ) ? n - k : 64, tm = n - k, ms = tm - bs;                     \
+            T      *akk = A + k * ( lda + 1 );                                                      \
+            getf2_##SFX( tm, bs, akk, lda, npiv, tau );                                             \
+            if ( ms > 0 ) {                                                                         \
+                T *ukj = akk + bs * lda, *lik = akk + bs, *aij = ukj + bs;                          \
+                /* U_k,k+1 = L_kk^{-1} A_k,k+1 (left, lower, unit) */                               \
+                for ( int64_t c = 0; c < ms; c++ )                                                  \
+                    for ( int64_t r = 1; r < bs; r++ ) {                                            \
+                        T acc = ukj[r + c * lda];                                                   \
+                        for ( int64_t l = 0; l < r; l++ ) acc -= akk[r + l * lda] * ukj[l + c * lda]; \
+                        ukj[r + c * lda] = acc;                                                     \
+                    }                                                                               \
+                GEMM( ORC_NOTRANS, ms, ms, bs, (T)-1, lik, lda, ukj, lda, (T)1, aij, lda );         \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* B <- B U^{-1}, U the upper part of the n x n tile D (unit: no division) */                   \
+    static void trsm_ru_##SFX( int64_t m, int64_t n, const T *D, int unit, T *B )                   \
+    {                                                                                               \
+        for ( int64_t j = 0; j < n; j++ ) {                                                         \
+            for ( int64_t l = 0; l < j; l++ ) {                                                     \
+                T u = D[l + j * n];                                                                 \
+                if ( u != (T)0 )                                                                    \
+                    for ( int64_t i = 0; i < m; i++ ) B[i + j * m] -= B[i + l * m] * u;             \
+            }                                                                                       \
+            if ( !unit ) {                                                                          \
+                T a = (T)1 / D[j + j * n];                                                          \
+                for ( int64_t i = 0; i < m; i++ ) B[i + j * m] *= a;                                \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* C[f(i)] -= A_i B_b^T for the bloks i >= b + shift (core_zgemmsp_2d2d) */                     \
+    static void update2_##SFX( const int64_t *cblk, const int64_t *blok, int64_t k, int64_t b,      \
+                               int shift, const T *A, const T *B, T *C )                            \
+    {                                                                                               \
+        int64_t  K  = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                        \
+        int64_t  F  = BK( b, B_FCBLK );                                                             \
+        int64_t  N  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                        \
+        int64_t  fb = CB( F, C_FBLOK );                                                             \
+        const T *Bm = B + CB( k, C_COEFOFF ) + BK( b, B_COEFIND );                                  \
+        for ( int64_t i = b + shift; i < CB( k + 1, C_FBLOK ); i++ ) {                              \
+            while ( !( BK( i, B_FROW ) >= BK( fb, B_FROW ) && BK( i, B_LROW ) <= BK( fb, B_LROW ) ) ) \
+                fb++;                                                                               \
+            int64_t M   = BK( i, B_LROW ) - BK( i, B_FROW ) + 1;                                    \
+            int64_t ldc = BK( fb, B_LROW ) - BK( fb, B_FROW ) + 1;                                  \
+            T *Cp = C + CB( F, C_COEFOFF ) + BK( fb, B_COEFIND ) + ( BK( i, B_FROW ) - BK( fb, B_FROW ) ) + \
+                    ( BK( b, B_FROW ) - CB( F, C_FCOL ) ) * ldc;                                    \
+            GEMM( ORC_TRANS, M, N, K, (T)-1, A + CB( k, C_COEFOFF ) + BK( i, B_COEFIND ), M, Bm, N, \
+                  (T)1, Cp, ldc );                                                                  \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    static void lu_cblk_##SFX( const int64_t *cblk, const int64_t *blok, int64_t k, T *L, T *U,     \
+                               int64_t *npiv, double tau )                                          \
+    {                                                                                               \
+        int64_t n  = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                         \
+        T      *Ld = L + CB( k, C_COEFOFF ), *Ud = U + CB( k, C_COEFOFF );                          \
+        /* geadd: L_kk += U_kk^T, then GETRF, then getmo: U_kk <- L_kk^T */                         \
+        for ( int64_t j = 0; j < n; j++ )                                                           \
+            for ( int64_t i = 0; i < n; i++ ) Ld[i + j * n] += Ud[j + i * n];                       \
+        getrf_##SFX( n, Ld, n, npiv, tau );                                                         \
+        for ( int64_t j = 0; j < n; j++ )                                                           \
+            for ( int64_t i = 0; i < n; i++ ) Ud[i + j * n] = Ld[j + i * n];                        \
+        /* L_off <- L_off U_kk^{-1};  U^T_off <- U^T_off L_kk^{-T} (the upper part */               \
+        /* of the U tile is L_kk^T, unit)                                          */               \
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {                   \
+            int64_t m = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                      \
+            trsm_ru_##SFX( m, n, Ld, 0, L + CB( k, C_COEFOFF ) + BK( b, B_COEFIND ) );              \
+            trsm_ru_##SFX( m, n, Ud, 1, U + CB( k, C_COEFOFF ) + BK( b, B_COEFIND ) );              \
+        }                                                                                           \
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {                   \
+            update2_##SFX( cblk, blok, k, b, 0, L, U, L );                                          \
+            if ( b + 1 < CB( k + 1, C_FBLOK ) ) update2_##SFX( cblk, blok, k, b, 1, U, L, U );      \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* L y = b (unit L), then U x = y with U read from the U^T tiles */                             \
+    static void lusolve_##SFX( int64_t nc, const int64_t *cblk, const int64_t *blok,                \
+                               const int64_t *browtab, const T *L, const T *U, int64_t nrhs, T *X,  \
+                               int64_t ldx )                                                        \
+    {                                                                                               \
+        for ( int64_t k = 0; k < nc; k++ ) {                                                        \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = 0; j < n; j++ ) {                                                 \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = 0; l < j; l++ ) s -= A[j + l * n] * xq[l];                    \
+                    xq[j] = s;                                                                      \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {               \
+                int64_t m = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                  \
+                GEMM( ORC_NOTRANS, m, nrhs, n, (T)-1, A + BK( b, B_COEFIND ), m, x, ldx, (T)1,      \
+                      X + BK( b, B_FROW ), ldx );                                                   \
+            }                                                                                       \
+        }                                                                                           \
+        for ( int64_t k = nc - 1; k >= 0; k-- ) {                                                   \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = U + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = n - 1; j >= 0; j-- ) {                                            \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = j + 1; l < n; l++ ) s -= A[l + j * n] * xq[l];                \
+                    xq[j] = s / A[j + j * n];                                                       \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t e = CB( k + 1, C_BROWNUM ) - 1; e >= CB( k, C_BROWNUM ); e-- ) {          \
+                int64_t  b  = browtab[e];                                                           \
+                int64_t  s  = BK( b, B_LCBLK );                                                     \
+                int64_t  m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                \
+                int64_t  ns = CB( s, C_LCOL ) - CB( s, C_FCOL ) + 1;                                \
+                const T *Ab = U + CB( s, C_COEFOFF ) + BK( b, B_COEFIND );                          \
+                for ( int64_t q = 0; q < nrhs; q++ ) {                                              \
+                    const T *xk = X + BK( b, B_FROW ) + q * ldx;                                    \
+                    T       *xs = X + CB( s, C_FCOL ) + q * ldx;                                    \
+                    for ( int64_t c = 0; c < ns; c++ ) {                                            \
+                        T acc = 0;                                                                  \
+                        for ( int64_t r = 0; r < m; r++ ) acc += Ab[r + c * m] * xk[r];             \
+                        xs[c] -= acc;                                                               \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+ORC_LU_TYPED( double, d, dgemm_w, fabs )
+ORC_LU_TYPED( float, s, sgemm_w, fabsf )
+
+/*
+ * Fill + LU factorization with static pivoting.  L and U (coefnbr entries
+ * each, factor type flt) are overwritten; tcolptr/trowtab/tval is the CSC of
+ * A^T (for a symmetric matrix, the CSC itself).
+ */
+int
+orc_factorize_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+                  int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const double *val,
+                  const int64_t *tcolptr, const int64_t *trowtab, const double *tval, double tau,
+                  void *L, void *U, int64_t *npiv )
+{
+    int64_t *col2cblk = malloc( ( n > 0 ? n : 1 ) * sizeof( int64_t ) );
+    int64_t  piv = 0;
+    orc_blas_init();
+    for ( int64_t k = 0; k < nc; k++ ) {
+        for ( int64_t j = CB( k, C_FCOL ); j <= CB( k, C_LCOL ); j++ ) col2cblk[j] = k;
+    }
+    if ( flt == 3 ) {
+        memset( L, 0, coefnbr * sizeof( double ) );
+        memset( U, 0, coefnbr * sizeof( double ) );
+        fill_d( n, cblk, blok, colptr, rowtab, val, col2cblk, (double *)L );
+        fillu_d( n, cblk, blok, tcolptr, trowtab, tval, col2cblk, (double *)U );
+        for ( int64_t k = 0; k < nc; k++ ) lu_cblk_d( cblk, blok, k, (double *)L, (double *)U, &piv, tau );
+    }
+    else {
+        memset( L, 0, coefnbr * sizeof( float ) );
+        memset( U, 0, coefnbr * sizeof( float ) );
+        fill_s( n, cblk, blok, colptr, rowtab, val, col2cblk, (float *)L );
+        fillu_s( n, cblk, blok, tcolptr, trowtab, tval, col2cblk, (float *)U );
+        for ( int64_t k = 0; k < nc; k++ ) lu_cblk_s( cblk, blok, k, (float *)L, (float *)U, &piv, tau );
+    }
+    if ( npiv ) *npiv = piv;
+    free( col2cblk );
+    return 0;
+}
+
+/* x <- A^{-1} x with an LU factor (double x, factor in flt). */
+int
+orc_solve_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+              const int64_t *browtab, const void *L, const void *U, int64_t nrhs, double *x, int64_t ldx )
+{
+    orc_blas_init();
+    if ( flt == 3 ) {
+        lusolve_d( nc, cblk, blok, browtab, (const double *)L, (const double *)U, nrhs, x, ldx );
+    }
+    else {
+        float *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
+        lusolve_s( nc, cblk, blok, browtab, (const float *)L, (const float *)U, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
     return 0;
 }
 

@@ -205,7 +205,8 @@ build_schedule( pxd_ctx_t *ctx )
     ctx->levels.assign( nlev, level_t{} );
     const bool big_ok = ctx->kernels != 0;
 
-    struct iv { int32_t F, f, l; std::vector<upd_task> *list; int64_t first, last; };
+    struct iv { int32_t side, F, f, l; std::vector<upd_task> *list; int64_t first, last; };
+    const bool lu = ( ctx->facto == FACT_GETRF );
     std::vector<iv> ivs;
 
     for ( int32_t L = 0; L < nlev; L++ ) {
@@ -248,36 +249,46 @@ build_schedule( pxd_ctx_t *ctx )
                     if ( ( level[bb.fcblknm] == L + 1 ) != ( pass == 0 ) ) {
                         continue;
                     }
-                    int32_t N = bb.lrownum - bb.frownum + 1;
-                    int32_t M = c.stride - bb.rowoff;
-                    if ( big_ok && M >= ctx->upd_big_rows ) {
-                        int64_t first = updb.size();
-                        for ( int32_t tn = 0; tn * PXD_UPD_BN < N; tn++ ) {
-                            for ( int32_t tm = 0; tm * ctx->upd_bm < M; tm++ ) {
-                                updb.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
-                            }
+                    const int32_t N = bb.lrownum - bb.frownum + 1;
+                    // L side: rows of bloks b.. ; U side (LU): rows of bloks b+1..
+                    for ( int side = 0; side < ( lu ? 2 : 1 ); side++ ) {
+                        const int32_t M = c.stride - bb.rowoff - ( side ? N : 0 );
+                        if ( M <= 0 ) {
+                            continue;
                         }
-                        ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, &updb, first, (int64_t)updb.size() } );
-                    }
-                    else {
-                        int64_t first = upd.size();
-                        for ( int32_t tn = 0; tn * 64 < N; tn++ ) {
-                            for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
-                                upd.push_back( upd_task{ k, b, tm, tn, 0, 0 } );
+                        const int32_t fl = side ? 2 : 0;
+                        if ( big_ok && M >= ctx->upd_big_rows ) {
+                            int64_t first = updb.size();
+                            for ( int32_t tn = 0; tn * PXD_UPD_BN < N; tn++ ) {
+                                for ( int32_t tm = 0; tm * ctx->upd_bm < M; tm++ ) {
+                                    updb.push_back( upd_task{ k, b, tm, tn, fl, 0 } );
+                                }
                             }
+                            ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &updb, first,
+                                               (int64_t)updb.size() } );
                         }
-                        ivs.push_back( iv{ bb.fcblknm, bb.frownum, bb.lrownum, &upd, first, (int64_t)upd.size() } );
+                        else {
+                            int64_t first = upd.size();
+                            for ( int32_t tn = 0; tn * 64 < N; tn++ ) {
+                                for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
+                                    upd.push_back( upd_task{ k, b, tm, tn, fl, 0 } );
+                                }
+                            }
+                            ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &upd, first,
+                                               (int64_t)upd.size() } );
+                        }
                     }
                 }
             }
         }
         // atomic flags for updates whose facing columns overlap another source's
         std::sort( ivs.begin(), ivs.end(), []( const iv &a, const iv &b ) {
+            if ( a.side != b.side ) return a.side < b.side;
             return ( a.F != b.F ) ? ( a.F < b.F ) : ( a.f < b.f );
         } );
         for ( size_t i = 0; i < ivs.size(); ) {
             size_t j = i;
-            while ( j < ivs.size() && ivs[j].F == ivs[i].F ) {
+            while ( j < ivs.size() && ivs[j].F == ivs[i].F && ivs[j].side == ivs[i].side ) {
                 j++;
             }
             int32_t maxl = -1;
@@ -324,10 +335,13 @@ ev_ms( hipEvent_t a, hipEvent_t b )
 // ---------------------------------------------------------------------------
 template <typename T>
 int
-factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
+factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
 {
+    const bool lu = ( ctx->facto == FACT_GETRF );
     T *L  = (T *)ctx->dL;
-    T *DI = (T *)ctx->ddinv;
+    T *DI = (T *)ctx->ddinv;                        // Cholesky: L_kk^{-1}; LU: (unit L_kk)^{-1}
+    T *U  = lu ? (T *)ctx->dU : L;                  // LU: U^T tiles
+    T *DU = lu ? (T *)ctx->dudinv : DI;             // LU: (U_kk^T)^{-1}
     hipStream_t s = ctx->stream;
     hipEvent_t  e0, e1, ea, eb;
     PXD_CHECK( hipEventCreate( &e0 ) );
@@ -340,13 +354,25 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     PXD_CHECK( hipMemsetAsync( ctx->dscratch, 0, 4 * sizeof( int ), s ) );
     PXD_CHECK( hipEventRecord( e0, s ) );
 
-    // fill: zero + scatter of the lower triangle of P A P^T
+    // fill: zero + scatter of the lower triangle of P A P^T (LU: and of the
+    // strict upper triangle, transposed, into the U tiles)
     PXD_CHECK( hipMemsetAsync( L, 0, ctx->coefnbr * sizeof( T ), s ) );
     double ovf = ( sizeof( T ) == 4 ) ? (double)FLT_MAX : DBL_MAX;
     hipLaunchKernelGGL( ( k_fill<T, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
                         ctx->n, ctx->dcolptr, ctx->drowtab, (const double *)ctx->dval, ctx->dcol2cblk,
                         ctx->dcb, ctx->dbk, L, 0, ovf, ctx->dscratch + 1 );
     launches += 2;
+    if ( lu ) {
+        PXD_CHECK( hipMemsetAsync( U, 0, ctx->coefnbr * sizeof( T ), s ) );
+        // the U side scatters A^T (a symmetric matrix is its own transpose)
+        const bool     tr = ( ctx->dtval != nullptr );
+        const int64_t *tc = tr ? ctx->dtcolptr : ctx->dcolptr;
+        const int64_t *tt = tr ? ctx->dtrowtab : ctx->drowtab;
+        const double  *tv = (const double *)( tr ? ctx->dtval : ctx->dval );
+        hipLaunchKernelGGL( ( k_fill<T, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
+                            ctx->n, tc, tt, tv, ctx->dcol2cblk, ctx->dcb, ctx->dbk, U, 1, ovf, ctx->dscratch + 1 );
+        launches += 2;
+    }
     if ( ctx->timing ) {
         PXD_CHECK( hipEventRecord( ea, s ) );
         PXD_CHECK( hipEventSynchronize( ea ) );
@@ -356,12 +382,28 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     if ( ctx->kernels ) {
         PXD_CHECK( hipMemsetAsync( DI, 0, ctx->dinvnbr * sizeof( T ), s ) );
         launches++;
+        if ( lu ) {
+            PXD_CHECK( hipMemsetAsync( DU, 0, ctx->dinvnbr * sizeof( T ), s ) );
+            launches++;
+        }
     }
     // launchers of one level's phases on a stream
     auto panel = [&]( const level_t &lv, hipStream_t st ) {
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
         unsigned ni = (unsigned)( lv.i1 - lv.i0 );
-        if ( ctx->kernels ) {
+        if ( lu ) {
+            hipLaunchKernelGGL( ( k_getrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
+                                ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch );
+            launches++;
+            if ( ni ) {
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
+                                    ctx->dcb, (const T *)L, DI );
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
+                                    ctx->dcb, (const T *)U, DU );
+                launches += 2;
+            }
+        }
+        else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_potrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
             launches++;
@@ -382,7 +424,15 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         if ( !nt ) {
             return;
         }
-        if ( ctx->kernels ) {
+        if ( lu ) {
+            // L_off <- L_off U_kk^{-1};  U^T_off <- U^T_off L_kk^{-T}
+            hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
+                                ctx->dcb, ctx->dbk, L, (const T *)DU );
+            hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
+                                ctx->dcb, ctx->dbk, U, (const T *)DI );
+            launches++;
+        }
+        else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
                                 ctx->dcb, ctx->dbk, L, (const T *)DI );
         }
@@ -402,12 +452,12 @@ factorize_potrf( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         unsigned nb = (unsigned)( b1 - b0 );
         if ( nb ) {
             hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L );
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U );
             launches++;
         }
         if ( nu && ctx->kernels ) {
             hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
-                                ctx->dcb, ctx->dbk, L );
+                                ctx->dcb, ctx->dbk, L, U );
             launches++;
         }
         else if ( nu ) {
@@ -485,6 +535,11 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
 {
     const T    *L  = (const T *)ctx->dL;
     const T    *DI = (const T *)ctx->ddinv;
+    // backward sweep: Cholesky L^T; LU: U, read from the U^T tiles with
+    // the inverses of the U_kk^T
+    const bool  lu = ( ctx->facto == FACT_GETRF );
+    const T    *LB = lu ? (const T *)ctx->dU : L;
+    const T    *DB = lu ? (const T *)ctx->dudinv : DI;
     hipStream_t s  = ctx->stream;
     const int   nl = (int)ctx->levels.size();
 
@@ -516,7 +571,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         if ( nf ) {
             if ( ctx->kernels ) {
                 hipLaunchKernelGGL( ( k_solve_bwd_gather2<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, X, ldx, nrhs );
             }
             else {
                 hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
@@ -525,7 +580,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         }
         if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_solve_diag_inv<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dlist + lv.c0, ctx->dcb, DI, X, ldx, nrhs, 1 );
+                                ctx->dlist + lv.c0, ctx->dcb, DB, X, ldx, nrhs, 1 );
         }
         else {
             hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
@@ -830,28 +885,46 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
     if ( ctx->dval == nullptr || ctx->dcb == nullptr ) {
         return 7;
     }
-    if ( factotype != FACT_POTRF ) {
+    if ( factotype != FACT_POTRF && factotype != FACT_GETRF ) {
         return 3;
     }
     if ( factor_flttype != FT_DOUBLE && factor_flttype != FT_FLOAT ) {
         return 3;
     }
-    size_t es    = elt_size( factor_flttype );
-    size_t bytes = std::max<int64_t>( ctx->coefnbr, 1 ) * es;
+    if ( factotype == FACT_GETRF && ( ( ctx->dtval == nullptr && ctx->mtx == 111 ) || !ctx->kernels ) ) {
+        // LU of a general matrix needs its transpose for the U side, and the
+        // matrix-core kernels
+        return 7;
+    }
+    const bool lu    = ( factotype == FACT_GETRF );
+    size_t     es    = elt_size( factor_flttype );
+    size_t     bytes = std::max<int64_t>( ctx->coefnbr, 1 ) * es;
+    size_t     dinvb = std::max<int64_t>( ctx->dinvnbr, 1 ) * es;
     if ( ctx->dL == nullptr || ctx->coef_bytes != bytes || ctx->fflt != factor_flttype ) {
-        if ( ctx->dL ) PXD_CHECK( hipFree( ctx->dL ) );
-        if ( ctx->ddinv ) PXD_CHECK( hipFree( ctx->ddinv ) );
-        ctx->dL = ctx->ddinv = nullptr;
+        void *old[] = { ctx->dL, ctx->ddinv, ctx->dU, ctx->dudinv };
+        for ( void *q : old ) {
+            if ( q ) PXD_CHECK( hipFree( q ) );
+        }
+        ctx->dL = ctx->ddinv = ctx->dU = ctx->dudinv = nullptr;
         PXD_CHECK( hipMalloc( &ctx->dL, bytes ) );
-        PXD_CHECK( hipMalloc( &ctx->ddinv, std::max<int64_t>( ctx->dinvnbr, 1 ) * es ) );
+        PXD_CHECK( hipMalloc( &ctx->ddinv, dinvb ) );
         ctx->coef_bytes = bytes;
     }
+    if ( lu && ctx->dU == nullptr ) {
+        PXD_CHECK( hipMalloc( &ctx->dU, bytes ) );
+        PXD_CHECK( hipMalloc( &ctx->dudinv, dinvb ) );
+    }
+    if ( !lu && ctx->dU != nullptr ) {
+        PXD_CHECK( hipFree( ctx->dU ) );
+        PXD_CHECK( hipFree( ctx->dudinv ) );
+        ctx->dU = ctx->dudinv = nullptr;
+    }
     ctx->fflt  = factor_flttype;
-    ctx->facto = factotype;
-    ctx->stats.coef_bytes = bytes;
+    ctx->stats.coef_bytes = bytes * ( lu ? 2 : 1 );
     const int bm = ( factor_flttype == FT_DOUBLE ) ? upd_rows<double>::value : upd_rows<float>::value;
-    if ( ctx->tasks_dirty || ctx->upd_bm != bm ) {
+    if ( ctx->tasks_dirty || ctx->upd_bm != bm || ctx->facto != factotype ) {
         ctx->upd_bm      = bm;
+        ctx->facto       = factotype;
         ctx->tasks_dirty = 0;
         int rc = build_schedule( ctx );
         if ( rc ) {
@@ -859,9 +932,9 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
         }
     }
     if ( factor_flttype == FT_DOUBLE ) {
-        return factorize_potrf<double>( ctx, threshold, nbpivots );
+        return factorize<double>( ctx, threshold, nbpivots );
     }
-    return factorize_potrf<float>( ctx, threshold, nbpivots );
+    return factorize<float>( ctx, threshold, nbpivots );
 }
 
 int

@@ -274,6 +274,268 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 }
 
 // ---------------------------------------------------------------------------
+// X = B * S^T for a rows x 32 block B of the diagonal tile, in place, on the
+// matrix cores: wave w takes 16-row blocks w, w+4, ...; row i of B is
+// element (i, k) at base + i * rs + k * ks; S is a 32 x 32 matrix in LDS,
+// S[row][col] = sS[col][row].  Only the first bs columns of B are read or
+// written (S is zero outside its bs x bs corner).
+template <typename T>
+__device__ inline void
+small_trsm_mfma( T *base, int64_t rs, int64_t ks, int32_t rows, int32_t bs, const T ( *sS )[PXD_NB + 1],
+                 int tid )
+{
+    typedef mfma16<T>         M;
+    typedef typename M::acc_t acc_t;
+    const int lane = tid & 63, w = tid >> 6;
+    for ( int32_t rb = 16 * w; rb < rows; rb += 64 ) {
+        T             a[PXD_NB / 4];
+        const int32_t ra = rb + ( lane & 15 );
+#pragma unroll
+        for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+            const int l = 4 * s4 + ( lane >> 4 );
+            a[s4] = ( ra < rows && l < bs ) ? base[ra * rs + l * ks] : (T)0;
+        }
+        acc_t acc0 = M::zero(), acc1 = M::zero();
+#pragma unroll
+        for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+            const int l = 4 * s4 + ( lane >> 4 );
+            acc0 = M::mma( a[s4], sS[l][( lane & 15 )], acc0 );
+            acc1 = M::mma( a[s4], sS[l][16 + ( lane & 15 )], acc1 );
+        }
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) {
+            const int32_t rr = rb + M::row( lane, v );
+            const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
+            if ( rr < rows && c0 < bs ) base[rr * rs + c0 * ks] = acc0[v];
+            if ( rr < rows && c1 < bs ) base[rr * rs + c1 * ks] = acc1[v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LU panel with static pivoting (core_zgetrfsp.c:64-272): the diagonal tile
+// of L holds the lower part of A_kk and the U tile its strict upper part,
+// transposed; they are summed (the reference's geadd), factorized without
+// pivoting by PXD_NB-column blocks (|a_kk| < tau -> sign(a_kk) tau), and the
+// packed LU is copied transposed into the U tile (the reference's getmo), so
+// that the lower part of the U tile is U^T.  The inverses of the diagonal
+// PXD_NB x PXD_NB blocks of L_kk (unit lower) and of U_kk^T go to the
+// diagonal blocks of DL and DU.
+template <typename T>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L, T *__restrict__ U,
+          T *__restrict__ DL, T *__restrict__ DU, double thresh, int *__restrict__ npiv )
+{
+    __shared__ T sA[PXD_NB][PXD_NB + 1];    // sA[col][row]
+    __shared__ T sIL[PXD_NB][PXD_NB + 1];   // L11^{-1}, [col][row]
+    __shared__ T sIU[PXD_NB][PXD_NB + 1];   // (U11^T)^{-1}, [col][row]
+    __shared__ T sD[PXD_NB];
+    __shared__ T sPi[PXD_NB][64 + 1];
+    __shared__ T sPj[PXD_NB][64 + 1];
+
+    const int32_t k   = list[blockIdx.x];
+    const dcblk   c   = cb[k];
+    const int32_t n   = c.lcolnum - c.fcolnum + 1;
+    const int     tid = threadIdx.x;
+    T *A  = L + c.coefoff;       // n x n, ld n
+    T *AU = U + c.coefoff;       // n x n, ld n
+    T *IL = DL + c.dinvoff;
+    T *IU = DU + c.dinvoff;
+    int     piv = 0;
+    const T tau = (T)thresh;
+    const int64_t nn = (int64_t)n * n;
+
+    // A_kk = L part + (U part)^T, 8 elements per thread in flight
+    for ( int64_t i0 = 0; i0 < nn; i0 += 8 * PXD_THREADS ) {
+        T va[8], vu[8];
+#pragma unroll
+        for ( int u = 0; u < 8; u++ ) {
+            const int64_t idx = i0 + tid + u * PXD_THREADS;
+            const int32_t r = (int32_t)( idx % n ), cc = (int32_t)( idx / n );
+            const bool    up = idx < nn && r < cc;
+            va[u] = up ? A[idx] : (T)0;
+            vu[u] = up ? AU[cc + (int64_t)r * n] : (T)0;
+        }
+#pragma unroll
+        for ( int u = 0; u < 8; u++ ) {
+            const int64_t idx = i0 + tid + u * PXD_THREADS;
+            const int32_t r = (int32_t)( idx % n ), cc = (int32_t)( idx / n );
+            if ( idx < nn && r < cc ) A[idx] = va[u] + vu[u];
+        }
+    }
+    __syncthreads();
+
+    for ( int32_t kb = 0; kb < n; kb += PXD_NB ) {
+        const int32_t bs = min( PXD_NB, n - kb );
+        {
+            constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
+            T v[NU];
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                v[u] = ( r < bs && cc < bs ) ? A[( kb + r ) + (int64_t)( kb + cc ) * n] : (T)0;
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                sA[cc][r]  = v[u];
+                sIL[cc][r] = (T)0;
+                sIU[cc][r] = (T)0;
+            }
+        }
+        __syncthreads();
+
+        // unblocked right-looking LU, the multipliers left unscaled until the end
+        for ( int cc = 0; cc < bs; cc++ ) {
+            T d = sA[cc][cc];
+            if ( px_abs( d ) < tau ) {
+                d = ( d < (T)0 ) ? -tau : tau;
+                if ( tid == 0 ) {
+                    piv++;
+                }
+            }
+            const int m = bs - cc - 1;
+            const T   r = (T)1 / d;
+            for ( int idx = tid; idx < m * m; idx += PXD_THREADS ) {
+                const int rr = cc + 1 + idx % m, c2 = cc + 1 + idx / m;
+                sA[c2][rr] -= sA[cc][rr] * r * sA[c2][cc];
+            }
+            if ( tid == 0 ) {
+                sD[cc] = d;
+            }
+            __syncthreads();
+        }
+        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
+            const int rr = idx % bs, cc = idx / bs;
+            if ( rr > cc ) {
+                sA[cc][rr] = sA[cc][rr] / sD[cc];
+            }
+            else if ( rr == cc ) {
+                sA[cc][cc] = sD[cc];
+            }
+        }
+        __syncthreads();
+
+        // inverses: threads [0, bs) build the columns of L11^{-1} (unit
+        // lower), threads [64, 64 + bs) those of (U11^T)^{-1}
+        if ( tid < bs ) {
+            const int t = tid;
+            sIL[t][t] = (T)1;
+            for ( int i = t + 1; i < bs; i++ ) {
+                T s = (T)0;
+                for ( int l = t; l < i; l++ ) s += sA[l][i] * sIL[t][l];
+                sIL[t][i] = -s;
+            }
+        }
+        else if ( tid >= 64 && tid < 64 + bs ) {
+            const int t = tid - 64;
+            // U11^T[i][l] = U11[l][i] = sA[i][l]
+            sIU[t][t] = (T)1 / sA[t][t];
+            for ( int i = t + 1; i < bs; i++ ) {
+                T s = (T)0;
+                for ( int l = t; l < i; l++ ) s += sA[i][l] * sIU[t][l];
+                sIU[t][i] = -s / sA[i][i];
+            }
+        }
+        __syncthreads();
+
+        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
+            const int rr = idx % bs, cc = idx / bs;
+            A[( kb + rr ) + (int64_t)( kb + cc ) * n]  = sA[cc][rr];
+            IL[( kb + rr ) + (int64_t)( kb + cc ) * n] = sIL[cc][rr];
+            IU[( kb + rr ) + (int64_t)( kb + cc ) * n] = sIU[cc][rr];
+        }
+
+        // in the tile: L21 <- A21 U11^{-1} = A21 (sIU)^T and
+        // U12^T <- A12^T L11^{-T} = A12^T (sIL)^T
+        const int32_t m0 = kb + bs;
+        const int32_t m  = n - m0;
+        small_trsm_mfma<T>( A + m0 + (int64_t)kb * n, 1, n, m, bs, sIU, tid );
+        small_trsm_mfma<T>( A + kb + (int64_t)m0 * n, n, 1, m, bs, sIL, tid );
+        __syncthreads();
+
+        // trailing update A22 -= L21 U12 (full square, 64 x 64 macro tiles)
+        const int32_t nt = ( m + 63 ) / 64;
+        const int     tx = tid % 16, ty = tid / 16;
+        for ( int32_t ti = 0; ti < nt; ti++ ) {
+            for ( int32_t tj = 0; tj < nt; tj++ ) {
+                {
+                    constexpr int NU = 64 * PXD_NB / PXD_THREADS;
+                    T vi[NU], vj[NU];
+#pragma unroll
+                    for ( int u = 0; u < NU; u++ ) {
+                        const int     idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
+                        const int32_t ri = ti * 64 + r, rj = tj * 64 + r;
+                        vi[u] = ( ri < m && l < bs ) ? A[( m0 + ri ) + (int64_t)( kb + l ) * n] : (T)0;
+                        vj[u] = ( rj < m && l < bs ) ? A[( kb + l ) + (int64_t)( m0 + rj ) * n] : (T)0;
+                    }
+#pragma unroll
+                    for ( int u = 0; u < NU; u++ ) {
+                        const int idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
+                        sPi[l][r] = vi[u];
+                        sPj[l][r] = vj[u];
+                    }
+                }
+                __syncthreads();
+                T acc[4][4];
+#pragma unroll
+                for ( int a = 0; a < 4; a++ )
+#pragma unroll
+                    for ( int b = 0; b < 4; b++ ) acc[a][b] = (T)0;
+                for ( int l = 0; l < bs; l++ ) {
+                    T ai[4], aj[4];
+#pragma unroll
+                    for ( int a = 0; a < 4; a++ ) {
+                        ai[a] = sPi[l][tx + 16 * a];
+                        aj[a] = sPj[l][ty + 16 * a];
+                    }
+#pragma unroll
+                    for ( int a = 0; a < 4; a++ )
+#pragma unroll
+                        for ( int b = 0; b < 4; b++ ) acc[a][b] += ai[a] * aj[b];
+                }
+                T old[4][4];
+#pragma unroll
+                for ( int a = 0; a < 4; a++ )
+#pragma unroll
+                    for ( int b = 0; b < 4; b++ ) {
+                        const int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
+                        old[a][b] = ( r < m && cc < m ) ? A[( m0 + r ) + (int64_t)( m0 + cc ) * n] : (T)0;
+                    }
+#pragma unroll
+                for ( int a = 0; a < 4; a++ )
+#pragma unroll
+                    for ( int b = 0; b < 4; b++ ) {
+                        const int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
+                        if ( r < m && cc < m ) A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[a][b] - acc[a][b];
+                    }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+
+    // U tile <- (packed LU)^T, so that its lower part is U^T
+    for ( int64_t i0 = 0; i0 < nn; i0 += 8 * PXD_THREADS ) {
+        T va[8];
+#pragma unroll
+        for ( int u = 0; u < 8; u++ ) {
+            const int64_t idx = i0 + tid + u * PXD_THREADS;
+            va[u] = ( idx < nn ) ? A[idx] : (T)0;
+        }
+#pragma unroll
+        for ( int u = 0; u < 8; u++ ) {
+            const int64_t idx = i0 + tid + u * PXD_THREADS;
+            const int32_t r = (int32_t)( idx % n ), cc = (int32_t)( idx / n );
+            if ( idx < nn ) AU[cc + (int64_t)r * n] = va[u];
+        }
+    }
+    if ( tid == 0 && piv > 0 ) {
+        atomicAdd( npiv, piv );
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse of the lower-triangular L_kk, one workgroup per (cblk, block column
 // J): Linv[I,J] = -Linv[I,I] * sum_{K=J}^{I-1} L[I,K] Linv[K,J] for I > J, the
 // diagonal blocks Linv[I,I] written by the panel.  Block column J of Linv is
@@ -476,7 +738,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-               const dblok *__restrict__ bk, T *__restrict__ L )
+               const dblok *__restrict__ bk, T *L, T *U )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -497,7 +759,13 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const int32_t  lb   = cb[t.cblk + 1].fblok;
     const dblok    bb   = bk[t.blok];
     const int32_t  N    = bb.lrownum - bb.frownum + 1;
-    const int32_t  Mr   = c.stride - bb.rowoff;
+    // U-side tile (LU): A = U^T rows after blok b, B = L rows of blok b, C = U
+    const bool     uside = ( t.flags & 2 ) != 0;
+    const T       *LA   = uside ? U : L;
+    const T       *LB   = uside ? L : U;
+    T             *LC   = uside ? U : L;
+    const int32_t  r0   = bb.rowoff + ( uside ? N : 0 );
+    const int32_t  Mr   = c.stride - r0;
     const int32_t  F    = bb.fcblknm;
     const dcblk    cf   = cb[F];
     const int32_t  Fn   = cf.lcolnum - cf.fcolnum + 1;
@@ -513,7 +781,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     if ( tid < 64 ) {
         int32_t r = t.tm * 64 + tid;
         if ( r < Mr ) {
-            int32_t s  = bb.rowoff + r;
+            int32_t s  = r0 + r;
             int32_t bi = search_row( sk, bk, t.blok, lb, s );
             dblok   ib = bk[bi];
             int32_t g  = ib.frownum + ( s - ib.rowoff );
@@ -547,11 +815,17 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const int wr = 32 * ( w >> 1 ), wc = 32 * ( w & 1 );
 
     for ( int32_t k0 = 0; k0 < K; k0 += BK ) {
+        T va[BK / 4], vb[BK / 4];
 #pragma unroll
-        for ( int kk = tid >> 6; kk < BK; kk += 4 ) {
-            int32_t kg = k0 + kk;
-            sA[kk][lr] = ( kg < K && src >= 0 ) ? L[src + (int64_t)kg * srcld] : (T)0;
-            sB[kk][lr] = ( kg < K && lr < nvalid ) ? L[bbase + lr + (int64_t)kg * N] : (T)0;
+        for ( int u = 0; u < BK / 4; u++ ) {
+            const int32_t kg = k0 + ( tid >> 6 ) + 4 * u;
+            va[u] = ( kg < K && src >= 0 ) ? LA[src + (int64_t)kg * srcld] : (T)0;
+            vb[u] = ( kg < K && lr < nvalid ) ? LB[bbase + lr + (int64_t)kg * N] : (T)0;
+        }
+#pragma unroll
+        for ( int u = 0; u < BK / 4; u++ ) {
+            sA[( tid >> 6 ) + 4 * u][lr] = va[u];
+            sB[( tid >> 6 ) + 4 * u][lr] = vb[u];
         }
         __syncthreads();
 #pragma unroll
@@ -578,7 +852,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 #pragma unroll
         for ( int v = 0; v < 4; v++ ) {
             const int r = wr + 16 * i + M::row( lane, v );
-            p[i][v]     = L + ( sDst[r] >= 0 ? sDst[r] : 0 ) + (int64_t)colF0 * sDstLd[r];
+            p[i][v]     = LC + ( sDst[r] >= 0 ? sDst[r] : 0 ) + (int64_t)colF0 * sDstLd[r];
 #pragma unroll
             for ( int j = 0; j < 2; j++ ) pv[i][j][v] = sDst[r] >= 0 && wc + 16 * j + ( lane & 15 ) < nvalid;
         }
@@ -625,7 +899,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 // the block goes to L + dst[r] + col * ld[r], columns colF0 + [0, nvalid).
 template <typename T>
 __device__ inline void
-update_scatter( T *__restrict__ L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int lane,
+update_scatter( T *L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int lane,
                 const int64_t *dst, const int32_t *ld, int32_t colF0, int32_t nvalid, bool atomic )
 {
     typedef mfma16<T> M;
@@ -675,7 +949,7 @@ template <> struct upd_rows<double> { static constexpr int value = 128; };
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS, 3 )
 k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
-              const dblok *__restrict__ bk, T *__restrict__ L )
+              const dblok *__restrict__ bk, T *L, T *U )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -701,7 +975,13 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const int32_t  lb   = cb[t.cblk + 1].fblok;
     const dblok    bb   = bk[t.blok];
     const int32_t  N    = bb.lrownum - bb.frownum + 1;
-    const int32_t  Mr   = c.stride - bb.rowoff;
+    // U-side tile (LU): A = U^T rows after blok b, B = L rows of blok b, C = U
+    const bool     uside = ( t.flags & 2 ) != 0;
+    const T       *LA   = uside ? U : L;
+    const T       *LB   = uside ? L : U;
+    T             *LC   = uside ? U : L;
+    const int32_t  r0   = bb.rowoff + ( uside ? N : 0 );
+    const int32_t  Mr   = c.stride - r0;
     const int32_t  F    = bb.fcblknm;
     const dcblk    cf   = cb[F];
     const int32_t  Fn   = cf.lcolnum - cf.fcolnum + 1;
@@ -724,7 +1004,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
         int64_t dst = -1;
         int32_t dld = 0;
         if ( r < Mr ) {
-            const int32_t s  = bb.rowoff + r;
+            const int32_t s  = r0 + r;
             const dblok   ib = bk[search_row( sk, bk, t.blok, lb, s )];
             const int32_t g  = ib.frownum + ( s - ib.rowoff );
             src   = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
@@ -745,14 +1025,14 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const int     bc     = tid & ( PXD_UPD_BN - 1 ), bq = tid / PXD_UPD_BN;
     const int32_t nvalid = min( PXD_UPD_BN, N - t.tn * PXD_UPD_BN );
     const bool    bok    = bc < nvalid;
-    const T      *bsrc   = L + c.coefoff + (int64_t)bb.rowoff * K + t.tn * PXD_UPD_BN + bc;
+    const T      *bsrc   = LB + c.coefoff + (int64_t)bb.rowoff * K + t.tn * PXD_UPD_BN + bc;
 
     T ra[KPT], rb[KV];
     auto load = [&]( int32_t k0 ) {
 #pragma unroll
         for ( int q = 0; q < KPT; q++ ) {
             const int32_t kg = k0 + ap * KPT + q;
-            ra[q] = ( src >= 0 && kg < K ) ? L[src + (int64_t)kg * srcld] : (T)0;
+            ra[q] = ( src >= 0 && kg < K ) ? LA[src + (int64_t)kg * srcld] : (T)0;
         }
 #pragma unroll
         for ( int q = 0; q < KV; q++ ) {
@@ -805,7 +1085,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const bool atomic = ( t.flags & 1 );
 #pragma unroll
     for ( int i = 0; i < RB; i++ ) {
-        update_scatter<T>( L, acc[i], 16 * ( RB * w + i ), lane, sDst, sDstLd, colF0, nvalid, atomic );
+        update_scatter<T>( LC, acc[i], 16 * ( RB * w + i ), lane, sDst, sDstLd, colF0, nvalid, atomic );
     }
 }
 

@@ -49,9 +49,10 @@ def static_pivot_threshold(dparm, dtype):
     return eps * dparm[px.DPARM_A_NORM]
 
 
-def cblk_diff(solver, La, Lb, dtype):
+def cblk_diff(solver, La, Lb, dtype, full_diag=False):
     """Largest per-cblk ||La - Lb||_max / (||Lb||_F * eps) over the stored lower
-    part (cpucblk_zdiff criterion, kernels/cpucblk_zdiff.c:63-130)."""
+    part (cpucblk_zdiff criterion, kernels/cpucblk_zdiff.c:63-130); with
+    full_diag the whole diagonal tile is compared (LU factors)."""
     cb = solver["cblk"].reshape(-1, 6)
     bk = solver["blok"].reshape(-1, 5)
     eps = np.finfo(dtype).eps / 2.0
@@ -62,8 +63,10 @@ def cblk_diff(solver, La, Lb, dtype):
         a = La[off0:off1].astype(np.float64)
         b = Lb[off0:off1].astype(np.float64)
         # the diagonal tile: compare its lower triangle only
-        d_a = np.tril(a[:n * n].reshape(n, n).T)
-        d_b = np.tril(b[:n * n].reshape(n, n).T)
+        d_a = a[:n * n].reshape(n, n).T
+        d_b = b[:n * n].reshape(n, n).T
+        if not full_diag:
+            d_a, d_b = np.tril(d_a), np.tril(d_b)
         diff = max(np.abs(d_a - d_b).max(), np.abs(a[n * n:] - b[n * n:]).max() if off1 - off0 > n * n else 0.0)
         nrm = np.sqrt((d_b ** 2).sum() + (b[n * n:] ** 2).sum())
         if nrm > 0:

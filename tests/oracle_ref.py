@@ -33,6 +33,11 @@ def lib():
         _lib.orc_gmres.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p,
                                    _i64, _i64, ctypes.c_double]
         _lib.orc_blas.restype = ctypes.c_char_p
+        _lib.orc_factorize_lu.restype = ctypes.c_int
+        _lib.orc_factorize_lu.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _i64, _p, _p, _p, _p, _p, _p,
+                                          ctypes.c_double, _p, _p, _p]
+        _lib.orc_solve_lu.restype = ctypes.c_int
+        _lib.orc_solve_lu.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p, _i64]
     return _lib
 
 
@@ -76,6 +81,42 @@ class OracleFactor:
                              _ptr(self.s["browtab"]), _ptr(self.L), _ptr(self.cp), _ptr(self.rt),
                              _ptr(self.val), _ptr(b), _ptr(x), im, itermax, eps)
         return x, int(it)
+
+
+def transpose_csc(cp, rt, val):
+    """CSC of A^T from the CSC of A (rows sorted in every column)."""
+    import scipy.sparse as sp
+    n = len(cp) - 1
+    T = sp.csc_matrix((val, rt, cp), shape=(n, n)).T.tocsc()
+    T.sort_indices()
+    return T.indptr.astype(np.int64), T.indices.astype(np.int64), np.ascontiguousarray(T.data)
+
+
+class OracleLU:
+    """LU factor with static pivoting computed by the oracle: L tiles (unit
+    lower + the U part of the diagonal tiles) and U^T tiles."""
+
+    def __init__(self, solver, bcsc, thresh, dtype=np.float64):
+        self.s = solver
+        self.cp, self.rt, self.val = bcsc
+        self.tcp, self.trt, self.tval = transpose_csc(*bcsc)
+        self.dtype = dtype
+        self.flt = 3 if dtype == np.float64 else 2
+        self.n = solver["nodenbr"]
+        self.L = np.zeros(solver["coefnbr"], dtype=dtype)
+        self.U = np.zeros(solver["coefnbr"], dtype=dtype)
+        npiv = np.zeros(1, dtype=np.int64)
+        lib().orc_factorize_lu(self.flt, self.n, solver["cblknbr"], _ptr(solver["cblk"]), _ptr(solver["blok"]),
+                               solver["coefnbr"], _ptr(self.cp), _ptr(self.rt), _ptr(self.val), _ptr(self.tcp),
+                               _ptr(self.trt), _ptr(self.tval), thresh, _ptr(self.L), _ptr(self.U), _ptr(npiv))
+        self.npiv = int(npiv[0])
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        lib().orc_solve_lu(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
+                           _ptr(self.s["browtab"]), _ptr(self.L), _ptr(self.U), nrhs, _ptr(x), self.n)
+        return x
 
 
 def spmv(cp, rt, val, x, alpha=1.0, beta=0.0, y=None):

@@ -143,3 +143,58 @@ def test_plrnt_jump_matches_sequential_lcg():
     assert np.array_equal(px.plrnt(20, seed=seed, shift=30), np.array(seq[30:50], dtype=np.float64))
     v = px.plrnt(100000, seed=0)
     assert v.min() >= -0.5 and v.max() <= 0.5 and abs(v.mean()) < 0.01
+
+
+def _dense_lu_nopiv(A):
+    """Doolittle LU without pivoting, packed (unit L strictly below, U on and above)."""
+    A = np.array(A, dtype=np.float64, copy=True)
+    n = A.shape[0]
+    for k in range(n):
+        A[k + 1:, k] /= A[k, k]
+        A[k + 1:, k + 1:] -= np.outer(A[k + 1:, k], A[k, k + 1:])
+    return A
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (6, 5, 4), (7, 7, 7)])
+def test_oracle_lu_equals_dense(dims):
+    spm = px.Spm.convdiff27(*dims, 1.0, 0.5, 3)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLU)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd)
+    n = s["nodenbr"]
+    A = _dense(*bc, n)
+    assert np.abs(A - A.T).max() > 0.01          # the values are not symmetric
+    F = orc.OracleLU(s, bc, static_pivot_threshold(dparm, np.float64))
+    assert F.npiv == 0
+    D = _dense_lu_nopiv(A)
+    Lo = orc.dense_from_factor(s, F.L)
+    Uo = orc.dense_from_factor(s, F.U)
+    packed = np.tril(Lo, -1) + np.triu(Uo.T)
+    assert np.abs(packed - D).max() <= 1e-12 * np.abs(D).max()
+    # the symbol holds every nonzero of the factors
+    assert np.abs(D[(np.tril(Lo, -1) == 0) & (np.tril(np.ones_like(D), -1) > 0)]).max() <= 1e-13
+    Ld = np.tril(D, -1) + np.eye(n)
+    assert np.abs(Ld @ np.triu(D) - A).max() <= 1e-12 * np.abs(A).max()
+    # solve
+    x0, b = spm.gen_rhs(2, seed=1)
+    xp = F.solve(permute_rhs(pd, b))
+    x = xp[px.order_export(pd)[0]]
+    assert np.abs(x - x0).max() <= 1e-11
+    px.finalize(pd)
+
+
+def test_lu_static_pivot_known_answers():
+    # |a_00| < tau is replaced by sign(a_00) tau (core_zgetf2sp), no square root
+    tau = 1e-8
+    A = np.array([[-1e-20, 1.0, 0.0], [2.0, 3.0, 1.0], [0.0, 1.0, 4.0]])
+    F = orc.OracleLU(_one_cblk(3), _csc(A), tau)
+    assert F.npiv == 1
+    P = F.L.reshape(3, 3).T                    # packed LU in the L diagonal tile
+    assert P[0, 0] == -tau
+    assert P[1, 0] == pytest.approx(2.0 / -tau)
+    assert P[1, 1] == pytest.approx(3.0 - (2.0 / -tau) * 1.0)
+    # the U tile holds the packed factor transposed
+    assert np.array_equal(F.U.reshape(3, 3), P)
+    # a pivot that is exactly tau is kept, a positive tiny one becomes +tau
+    F2 = orc.OracleLU(_one_cblk(2), _csc(np.array([[1e-30, 0.0], [0.0, 1.0]])), tau)
+    assert F2.npiv == 1 and F2.L[0] == tau
