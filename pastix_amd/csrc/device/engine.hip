@@ -259,8 +259,10 @@ build_schedule( pxd_ctx_t *ctx )
                         const int32_t fl = side ? 2 : 0;
                         if ( big_ok && M >= ctx->upd_big_rows ) {
                             int64_t first = updb.size();
-                            for ( int32_t tn = 0; tn * PXD_UPD_BN < N; tn++ ) {
-                                for ( int32_t tm = 0; tm * ctx->upd_bm < M; tm++ ) {
+                            // column tiles fastest: the tiles that share a row
+                            // panel run together and share its L2 lines
+                            for ( int32_t tm = 0; tm * ctx->upd_bm < M; tm++ ) {
+                                for ( int32_t tn = 0; tn * PXD_UPD_BN < N; tn++ ) {
                                     updb.push_back( upd_task{ k, b, tm, tn, fl, 0 } );
                                 }
                             }
@@ -793,6 +795,15 @@ pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
         }
     }
     ctx->dinvnbr = dinv;
+    // the update kernels address a cblk's coefficients with 32-bit byte
+    // offsets (buffer loads): every cblk must stay below 2 GiB in double
+    for ( int64_t k = 0; k < cblknbr; k++ ) {
+        const dcblk &c = ctx->hcb[k];
+        if ( (double)c.stride * ( c.lcolnum - c.fcolnum + 1 ) * 8. >= 2147483648. ) {
+            fprintf( stderr, "pastix_amd device: cblk %lld holds more than 2 GiB of coefficients\n", (long long)k );
+            return 3;
+        }
+    }
     for ( int64_t b = 0; b <= bloknbr; b++ ) {
         dblok &bb  = ctx->hbk[b];
         bb.frownum = (int32_t)bloks[b].frownum;

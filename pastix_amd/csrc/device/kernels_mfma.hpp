@@ -8,6 +8,7 @@
 // (solve_ztrsmsp.c:74-108) become Linv b and Linv^T b.
 #pragma once
 #include "kernels_fact.hpp"
+#include "kernels_solve.hpp"
 #include "mfma.hpp"
 
 // tasks of the block-column inversion: cblk and first column of the block
@@ -961,15 +962,18 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     constexpr int TPR = PXD_THREADS / BM; // loader threads per row
     constexpr int KPT = BK / TPR;         // k values loaded per thread
     constexpr int RB  = BM / 64;          // 16-row blocks per wave
-    __shared__ __align__( 16 ) T sA[BM * LDK];
+    constexpr uint32_t ES = sizeof( T );
+    constexpr int WR  = 16 * RB;          // rows per wave
+    constexpr int SE  = ( BM * LDK > 4 * WR * 17 ) ? BM * LDK : 4 * WR * 17;
+    __shared__ __align__( 16 ) T sA[SE];  // staged A tile; after the k loop, the epilogue's transposes
     __shared__ __align__( 16 ) T sB[PXD_UPD_BN * LDK];
-    __shared__ int64_t sDst[BM];
-    __shared__ int32_t sDstLd[BM];
+    __shared__ uint32_t sDst[BM];         // byte offset of the row in F (PXD_OOB: none)
+    __shared__ uint32_t sDstLd[BM];       // byte stride between F's columns for the row
     __shared__ search_keys sk;
 
     const upd_task t    = tasks[xcd_task( blockIdx.x, ntasks )];
     const int      tid  = threadIdx.x;
-    const int      lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    const int      lane = tid & 63, w = tid >> 6;
     const dcblk    c    = cb[t.cblk];
     const int32_t  K    = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb   = cb[t.cblk + 1].fblok;
@@ -990,30 +994,33 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     if ( bb.frownum < cf.fcolnum || bb.lrownum > cf.lcolnum ) {
         return;   // inconsistent structure: the host checks prevent this
     }
+    const __amdgpu_buffer_rsrc_t rA = make_rsrc( LA + c.coefoff, (uint32_t)( (int64_t)c.stride * K * ES ) );
+    const __amdgpu_buffer_rsrc_t rB = make_rsrc( LB + c.coefoff, (uint32_t)( (int64_t)c.stride * K * ES ) );
+    const __amdgpu_buffer_rsrc_t rC = make_rsrc( LC + cf.coefoff, (uint32_t)( (int64_t)cf.stride * Fn * ES ) );
 
     stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
     __syncthreads();
 
-    // loader row ar (k values [ap*KPT, ap*KPT+KPT) of each stage); the
+    // loader row ar (k values [ap*KPT, ap*KPT+KPT) of each stage): byte
+    // offset of its first element and byte stride between its columns; the
     // threads with ap == 0 also map the row into F
     const int ar = tid % BM, ap = tid / BM;
-    int64_t   src   = -1;
-    int32_t   srcld = 0;
+    uint32_t  abase = PXD_OOB, astep = 0;
     {
-        const int32_t r = t.tm * BM + ar;
-        int64_t dst = -1;
-        int32_t dld = 0;
+        const int32_t r   = t.tm * BM + ar;
+        uint32_t      dst = PXD_OOB, dld = 0;
         if ( r < Mr ) {
             const int32_t s  = r0 + r;
             const dblok   ib = bk[search_row( sk, bk, t.blok, lb, s )];
             const int32_t g  = ib.frownum + ( s - ib.rowoff );
-            src   = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
-            srcld = ib.lrownum - ib.frownum + 1;
+            abase = (uint32_t)( ib.rowoff * K + ( s - ib.rowoff ) ) * ES;
+            astep = (uint32_t)( ib.lrownum - ib.frownum + 1 ) * ES;
             if ( ap == 0 ) {
                 const dblok fb = bk[search_grow( sk, bk, cf.fblok, flb, g )];
-                dst = ( g >= fb.frownum && g <= fb.lrownum ) ? cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum )
-                                                              : -1;
-                dld = fb.lrownum - fb.frownum + 1;
+                if ( g >= fb.frownum && g <= fb.lrownum ) {
+                    dst = (uint32_t)( fb.rowoff * Fn + ( g - fb.frownum ) ) * ES;
+                    dld = (uint32_t)( fb.lrownum - fb.frownum + 1 ) * ES;
+                }
             }
         }
         if ( ap == 0 ) {
@@ -1022,22 +1029,22 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
         }
     }
     // B loader: facing column bc, k group bq
-    const int     bc     = tid & ( PXD_UPD_BN - 1 ), bq = tid / PXD_UPD_BN;
-    const int32_t nvalid = min( PXD_UPD_BN, N - t.tn * PXD_UPD_BN );
-    const bool    bok    = bc < nvalid;
-    const T      *bsrc   = LB + c.coefoff + (int64_t)bb.rowoff * K + t.tn * PXD_UPD_BN + bc;
+    const int      bc     = tid & ( PXD_UPD_BN - 1 ), bq = tid / PXD_UPD_BN;
+    const int32_t  nvalid = min( PXD_UPD_BN, N - t.tn * PXD_UPD_BN );
+    const uint32_t bbase  = ( bc < nvalid ) ? (uint32_t)( bb.rowoff * K + t.tn * PXD_UPD_BN + bc ) * ES : PXD_OOB;
+    const uint32_t bstep  = ( bc < nvalid ) ? (uint32_t)N * ES : 0;
 
     T ra[KPT], rb[KV];
     auto load = [&]( int32_t k0 ) {
 #pragma unroll
         for ( int q = 0; q < KPT; q++ ) {
             const int32_t kg = k0 + ap * KPT + q;
-            ra[q] = ( src >= 0 && kg < K ) ? LA[src + (int64_t)kg * srcld] : (T)0;
+            ra[q] = bufio<T>::load( rA, ( kg < K ) ? abase + (uint32_t)kg * astep : PXD_OOB );
         }
 #pragma unroll
         for ( int q = 0; q < KV; q++ ) {
             const int32_t kg = k0 + bq * KV + q;
-            rb[q] = ( bok && kg < K ) ? bsrc[(int64_t)kg * N] : (T)0;
+            rb[q] = bufio<T>::load( rB, ( kg < K ) ? bbase + (uint32_t)kg * bstep : PXD_OOB );
         }
     };
 
@@ -1047,6 +1054,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
 #pragma unroll
         for ( int j = 0; j < 4; j++ ) acc[i][j] = M::zero();
 
+    const int lr = lane & 15, lg = lane >> 4;
     load( 0 );
     for ( int32_t k0 = 0; k0 < K; k0 += BK ) {
 #pragma unroll
@@ -1080,12 +1088,48 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
         __syncthreads();
     }
 
-    // scatter into F: the 16 read-modify-writes of a row block are issued as
-    // 16 loads, then 16 stores, so their memory latencies overlap
-    const bool atomic = ( t.flags & 1 );
+    // scatter into F, one 16-column slab at a time: each wave transposes its
+    // WR x 16 block through LDS so that a lane owns one row and every memory
+    // instruction covers WR consecutive rows of one column (coalesced, as F is
+    // column major).  Plain tiles issue all loads of a slab, then the stores
+    // (out-of-range rows and columns read 0 and drop their stores); tiles
+    // flagged atomic use atomic adds.
+    const bool    atomic = ( t.flags & 1 );
+    T            *sE     = sA + w * ( WR * 17 );
+    constexpr int LPR    = 64 / WR;        // lanes per row
+    constexpr int CPL    = 16 / LPR;       // columns per lane
+    const int     erow   = lane % WR, epart = lane / WR;
+    const uint32_t rowoff = sDst[WR * w + erow];
+    const uint32_t rowld  = sDstLd[WR * w + erow];
 #pragma unroll
-    for ( int i = 0; i < RB; i++ ) {
-        update_scatter<T>( LC, acc[i], 16 * ( RB * w + i ), lane, sDst, sDstLd, colF0, nvalid, atomic );
+    for ( int j = 0; j < 4; j++ ) {
+#pragma unroll
+        for ( int i = 0; i < RB; i++ )
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) sE[( 16 * i + M::row( lane, v ) ) * 17 + lr] = acc[i][j][v];
+        __syncthreads();
+        uint32_t off[CPL];
+        T        val[CPL];
+#pragma unroll
+        for ( int c = 0; c < CPL; c++ ) {
+            const int col = epart * CPL + c, cc = 16 * j + col;
+            off[c] = ( rowoff != PXD_OOB && cc < nvalid ) ? rowoff + (uint32_t)( colF0 + cc ) * rowld : PXD_OOB;
+            val[c] = sE[erow * 17 + col];
+        }
+        if ( atomic ) {
+#pragma unroll
+            for ( int c = 0; c < CPL; c++ ) {
+                if ( off[c] != PXD_OOB ) px_atomic_add( LC + cf.coefoff + off[c] / ES, -val[c] );
+            }
+        }
+        else {
+            T old[CPL];
+#pragma unroll
+            for ( int c = 0; c < CPL; c++ ) old[c] = bufio<T>::load( rC, off[c] );
+#pragma unroll
+            for ( int c = 0; c < CPL; c++ ) bufio<T>::store( old[c] - val[c], rC, off[c] );
+        }
+        __syncthreads();
     }
 }
 

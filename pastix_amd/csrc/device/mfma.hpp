@@ -51,3 +51,35 @@ xcd_task( int32_t b, int32_t nb )
     const int32_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
     return x * q + min( x, r ) + i;
 }
+
+// Buffer loads and stores relative to one cblk's coefficients: a byte offset
+// at or beyond the range of the resource (PXD_OOB) reads 0 and drops the
+// store, which replaces per-element bounds branches.
+#define PXD_OOB 0x80000000u
+__device__ inline __amdgpu_buffer_rsrc_t
+make_rsrc( const void *p, uint32_t bytes )
+{
+    return __builtin_amdgcn_make_buffer_rsrc( const_cast<void *>( p ), 0, (int)bytes, 0x00020000 );
+}
+template <typename T> struct bufio;
+template <> struct bufio<float> {
+    __device__ static inline float load( __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        return __builtin_bit_cast( float, __builtin_amdgcn_raw_buffer_load_b32( r, off, 0, 0 ) );
+    }
+    __device__ static inline void store( float v, __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned, v ), r, off, 0, 0 );
+    }
+};
+template <> struct bufio<double> {
+    typedef unsigned u2 __attribute__( ( ext_vector_type( 2 ) ) );
+    __device__ static inline double load( __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        return __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 ) );
+    }
+    __device__ static inline void store( double v, __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( u2, v ), r, off, 0, 0 );
+    }
+};
