@@ -40,6 +40,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-stats", action="store_true", help="per-kernel-class event timing pass")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the timed region, one device copy of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "update_traffic_100cubed.json"),
+                    help="measured HBM bytes of the update kernels (from the counter runs), if present")
     return ap.parse_args()
 
 
@@ -220,6 +225,25 @@ def main():
             % (label, r["fact"], r["solve"], r["refine"], r["iters"], r["tts"], r["berr"]))
         run.close()
 
+    if args.copy_probe > 0:
+        # one plain device copy of known size, outside every timed region
+        import ctypes as ct
+        nbytes = args.copy_probe << 20
+        dev = ct.c_void_p()
+        iparm, dparm = px.initParam()
+        pdp = px.init(iparm, dparm)
+        px._check("device", px.lib.pastix_amd_device_handle(pdp, ct.byref(dev)))
+        a, b = ct.c_void_p(), ct.c_void_p()
+        px.lib.pxd_malloc(dev, nbytes, ct.byref(a))
+        px.lib.pxd_malloc(dev, nbytes, ct.byref(b))
+        px.lib.pxd_memset0(dev, a, nbytes)
+        px.lib.pxd_vec_copy(dev, nbytes // 8, 1, a, b)
+        px.lib.pxd_sync(dev)
+        px.lib.pxd_free(dev, a)
+        px.lib.pxd_free(dev, b)
+        px.finalize(pdp)
+        log("copy probe: %d MiB device copy done" % args.copy_probe)
+
     cpu = None
     if not args.no_cpu:
         log("step: CPU baseline sample")
@@ -231,6 +255,10 @@ def main():
     upd = head.get("update_flops", 0.0)
     upd_t = head.get("kernel_seconds", {}).get("update_seconds", 0.0)
     achieved = upd / upd_t / 1e12 if upd_t > 0 else 0.0
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("hbm_bytes_per_factorization")
     out = {
         "metric": "fact+solve s & GFLOP/s at bwd err<=1e-12: FP32+IR vs FP64, 100^3 Laplacian",
         "value": head["tts"],
@@ -251,8 +279,9 @@ def main():
         "gflops_fact": fl / head["fact"] / 1e9,
         "gflops_fact_binary": fl / head["fact"] / 1024 ** 3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS["f32"],
-                     "unit": "TFLOP/s", "frac": achieved / MFMA_PEAK_TFLOPS["f32"], "traffic": None,
-                     "kernel": "k_update<float>", "algorithmic_flops": upd},
+                     "unit": "TFLOP/s", "frac": achieved / MFMA_PEAK_TFLOPS["f32"], "traffic": traffic,
+                     "kernel": "k_update_big<float> + k_update_mfma<float> (supernodal update, all launches "
+                               "of one factorization)", "algorithmic_flops": upd},
         "cpu_baseline": None,
     }
     if "fp64" in results:

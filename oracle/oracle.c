@@ -19,6 +19,9 @@
  *   solve              kernels/solve_ztrsmsp.c:74-625
  *   spmv               bcsc/bcsc_zspmv.c:57-76
  *   gmres              refinement/z_refine_gmres.c:48-371
+ *   LDL^T              kernels/core_zsytrfsp.c:40-204 (sytf2 / 64-blocked
+ *                      sytrf), :445-537 (TRSM with D L^T, scalo workspace,
+ *                      gemmsp updates); kernels/core_zscalo.c:72-138
  *   LU                 kernels/core_zgetrfsp.c:64-184 (getf2 / 64-blocked getrf
  *                      with static pivot), :216-272 (geadd, getrf, getmo),
  *                      :354-397 (two TRSMs, updates of both sides)
@@ -681,6 +684,188 @@ orc_solve_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t
         for ( int64_t q = 0; q < nrhs; q++ )
             for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
         lusolve_s( nc, cblk, blok, browtab, (const float *)L, (const float *)U, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LDL^T without pivoting, static pivot +-tau (kernels/core_zsytrfsp.c),    */
+/* sequential in cblk order.  The diagonal tile holds L (strictly below), D  */
+/* and D L^T (above); the off-diagonal tiles hold L.                         */
+
+#define ORC_LDLT_TYPED( T, SFX, GEMM, FABS )                                                        \
+    /* unblocked (core_zsytf2sp): copy the column to the row, scale, update */                      \
+    static void sytf2_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
+    {                                                                                               \
+        for ( int64_t k = 0; k < n; k++ ) {                                                         \
+            T *akk = A + k * ( lda + 1 );                                                           \
+            if ( FABS( *akk ) < (T)tau ) {                                                          \
+                *akk = ( *akk < (T)0 ) ? (T)-tau : (T)tau;                                          \
+                ( *npiv )++;                                                                        \
+            }                                                                                       \
+            T a = (T)1 / *akk;                                                                      \
+            for ( int64_t i = k + 1; i < n; i++ ) A[k + i * lda] = A[i + k * lda];                  \
+            for ( int64_t i = k + 1; i < n; i++ ) A[i + k * lda] *= a;                              \
+            for ( int64_t j = k + 1; j < n; j++ )                                                   \
+                for ( int64_t i = j; i < n; i++ ) A[i + j * lda] -= *akk * A[i + k * lda] * A[j + k * lda]; \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* blocked (64) LDL^T of the diagonal tile (core_zsytrfsp) */                                   \
+    static void sytrf_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
+    {                                                                                               \
+        for ( int64_t k = 0; k < n; k += 64 ) {                                                     \
+            int64_t bs = ( n - k < 64 ) ? n - k : 64, ms = n - k - bs;                              \
+            T      *akk = A + k * ( lda + 1 ), *amk = akk + bs, *akm = akk + bs * lda;              \
+            sytf2_##SFX( bs, akk, lda, npiv, tau );                                                 \
+            if ( ms > 0 ) {                                                                         \
+                /* A(k+1:n,k) <- A(k+1:n,k) L(k,k)^{-T} (right, lower, trans, unit) */              \
+                for ( int64_t j = 0; j < bs; j++ )                                                  \
+                    for ( int64_t l = 0; l < j; l++ ) {                                             \
+                        T f = akk[j + l * lda];                                                     \
+                        for ( int64_t i = 0; i < ms; i++ ) amk[i + j * lda] -= amk[i + l * lda] * f; \
+                    }                                                                               \
+                /* D L^T into the upper part, then L = . / D */                                     \
+                for ( int64_t j = 0; j < bs; j++ ) {                                                \
+                    T a = (T)1 / akk[j + j * lda];                                                  \
+                    for ( int64_t i = 0; i < ms; i++ ) {                                            \
+                        akm[j + i * lda] = amk[i + j * lda];                                        \
+                        amk[i + j * lda] *= a;                                                      \
+                    }                                                                               \
+                }                                                                                   \
+                GEMM( ORC_NOTRANS, ms, ms, bs, (T)-1, amk, lda, akm, lda, (T)1, akm + bs, lda );    \
+            }                                                                                       \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    static void ldlt_cblk_##SFX( const int64_t *cblk, const int64_t *blok, int64_t k, T *L, T *W,   \
+                                 int64_t *npiv, double tau )                                        \
+    {                                                                                               \
+        int64_t n  = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                         \
+        T      *Ld = L + CB( k, C_COEFOFF );                                                        \
+        sytrf_##SFX( n, Ld, n, npiv, tau );                                                         \
+        /* L_off <- L_off (D L^T)^{-1} (upper part of the tile), W_off = L_off D (scalo) */         \
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {                   \
+            int64_t m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                     \
+            T      *Lb = L + CB( k, C_COEFOFF ) + BK( b, B_COEFIND );                               \
+            T      *Wb = W + CB( k, C_COEFOFF ) + BK( b, B_COEFIND );                               \
+            trsm_ru_##SFX( m, n, Ld, 0, Lb );                                                       \
+            for ( int64_t j = 0; j < n; j++ )                                                       \
+                for ( int64_t i = 0; i < m; i++ ) Wb[i + j * m] = Lb[i + j * m] * Ld[j + j * n];    \
+        }                                                                                           \
+        for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {                   \
+            update2_##SFX( cblk, blok, k, b, 0, L, W, L );                                          \
+        }                                                                                           \
+    }                                                                                               \
+                                                                                                    \
+    /* L y = b (unit), y <- D^{-1} y, L^T x = y (unit) */                                           \
+    static void ldltsolve_##SFX( int64_t nc, const int64_t *cblk, const int64_t *blok,              \
+                                 const int64_t *browtab, const T *L, int64_t nrhs, T *X, int64_t ldx ) \
+    {                                                                                               \
+        for ( int64_t k = 0; k < nc; k++ ) {                                                        \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = 0; j < n; j++ ) {                                                 \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = 0; l < j; l++ ) s -= A[j + l * n] * xq[l];                    \
+                    xq[j] = s;                                                                      \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {               \
+                int64_t m = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                  \
+                GEMM( ORC_NOTRANS, m, nrhs, n, (T)-1, A + BK( b, B_COEFIND ), m, x, ldx, (T)1,      \
+                      X + BK( b, B_FROW ), ldx );                                                   \
+            }                                                                                       \
+        }                                                                                           \
+        for ( int64_t k = 0; k < nc; k++ ) {                                                        \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            for ( int64_t q = 0; q < nrhs; q++ )                                                    \
+                for ( int64_t j = 0; j < n; j++ ) X[CB( k, C_FCOL ) + j + q * ldx] /= A[j + j * n]; \
+        }                                                                                           \
+        for ( int64_t k = nc - 1; k >= 0; k-- ) {                                                   \
+            int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                     \
+            const T *A = L + CB( k, C_COEFOFF );                                                    \
+            T       *x = X + CB( k, C_FCOL );                                                       \
+            for ( int64_t q = 0; q < nrhs; q++ ) {                                                  \
+                T *xq = x + q * ldx;                                                                \
+                for ( int64_t j = n - 1; j >= 0; j-- ) {                                            \
+                    T s = xq[j];                                                                    \
+                    for ( int64_t l = j + 1; l < n; l++ ) s -= A[l + j * n] * xq[l];                \
+                    xq[j] = s;                                                                      \
+                }                                                                                   \
+            }                                                                                       \
+            for ( int64_t e = CB( k + 1, C_BROWNUM ) - 1; e >= CB( k, C_BROWNUM ); e-- ) {          \
+                int64_t  b  = browtab[e];                                                           \
+                int64_t  s  = BK( b, B_LCBLK );                                                     \
+                int64_t  m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                                \
+                int64_t  ns = CB( s, C_LCOL ) - CB( s, C_FCOL ) + 1;                                \
+                const T *Ab = L + CB( s, C_COEFOFF ) + BK( b, B_COEFIND );                          \
+                for ( int64_t q = 0; q < nrhs; q++ ) {                                              \
+                    const T *xk = X + BK( b, B_FROW ) + q * ldx;                                    \
+                    T       *xs = X + CB( s, C_FCOL ) + q * ldx;                                    \
+                    for ( int64_t c = 0; c < ns; c++ ) {                                            \
+                        T acc = 0;                                                                  \
+                        for ( int64_t r = 0; r < m; r++ ) acc += Ab[r + c * m] * xk[r];             \
+                        xs[c] -= acc;                                                               \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+ORC_LDLT_TYPED( double, d, dgemm_w, fabs )
+ORC_LDLT_TYPED( float, s, sgemm_w, fabsf )
+
+/* Fill + LDL^T factorization with static pivoting (flt 3 double, 2 float). */
+int
+orc_factorize_ldlt( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+                    int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const double *val,
+                    double tau, void *L, int64_t *npiv )
+{
+    int64_t *col2cblk = malloc( ( n > 0 ? n : 1 ) * sizeof( int64_t ) );
+    int64_t  piv = 0;
+    size_t   es = ( flt == 3 ) ? sizeof( double ) : sizeof( float );
+    void    *W = calloc( coefnbr > 0 ? coefnbr : 1, es );
+    orc_blas_init();
+    for ( int64_t k = 0; k < nc; k++ ) {
+        for ( int64_t j = CB( k, C_FCOL ); j <= CB( k, C_LCOL ); j++ ) col2cblk[j] = k;
+    }
+    memset( L, 0, coefnbr * es );
+    if ( flt == 3 ) {
+        fill_d( n, cblk, blok, colptr, rowtab, val, col2cblk, (double *)L );
+        for ( int64_t k = 0; k < nc; k++ ) ldlt_cblk_d( cblk, blok, k, (double *)L, (double *)W, &piv, tau );
+    }
+    else {
+        fill_s( n, cblk, blok, colptr, rowtab, val, col2cblk, (float *)L );
+        for ( int64_t k = 0; k < nc; k++ ) ldlt_cblk_s( cblk, blok, k, (float *)L, (float *)W, &piv, tau );
+    }
+    if ( npiv ) *npiv = piv;
+    free( W );
+    free( col2cblk );
+    return 0;
+}
+
+/* x <- A^{-1} x with an LDL^T factor (double x, factor in flt). */
+int
+orc_solve_ldlt( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+                const int64_t *browtab, const void *L, int64_t nrhs, double *x, int64_t ldx )
+{
+    orc_blas_init();
+    if ( flt == 3 ) {
+        ldltsolve_d( nc, cblk, blok, browtab, (const double *)L, nrhs, x, ldx );
+    }
+    else {
+        float *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
+        ldltsolve_s( nc, cblk, blok, browtab, (const float *)L, nrhs, xs, n );
         for ( int64_t q = 0; q < nrhs; q++ )
             for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
         free( xs );

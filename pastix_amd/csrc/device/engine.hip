@@ -33,6 +33,7 @@ struct level_t {
     int64_t u0, uc, u1; // update tasks (64 x 64 tiles); [u0, uc) face the next level
     int64_t b0, bc, b1; // update tasks (tall tiles); [b0, bc) face the next level
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
+    int64_t d0, d1;     // solve diagonal tasks (64-row chunks of the diagonal blocks)
     int64_t i0, i1;     // block-column inversion tasks
     int32_t maxn;
 };
@@ -65,6 +66,7 @@ struct pxd_ctx_s {
     upd_task *dupd  = nullptr;
     upd_task *dupdb = nullptr;
     row_task *dsolv = nullptr;
+    row_task *ddiag = nullptr;
     inv_task *dinvt = nullptr;
     int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
 
@@ -83,6 +85,8 @@ struct pxd_ctx_s {
     int   *dscratch = nullptr;     // counters (pivots, overflow)
     void  *dvecs = nullptr;        // solve scratch in the factor precision
     size_t dvecs_bytes = 0;
+    void  *dsolvy = nullptr;       // the sweeps' second vector (n x PXD_MAXRHS, factor precision)
+    size_t dsolvy_bytes = 0;
     void  *dred = nullptr;         // reduction scratch
 
     // rows per TRSM tile: 64 for the matrix-core kernels; for the scalar
@@ -199,7 +203,7 @@ build_schedule( pxd_ctx_t *ctx )
     }
 
     std::vector<int32_t>  list;
-    std::vector<row_task> trsm, solv;
+    std::vector<row_task> trsm, solv, diag;
     std::vector<inv_task> invt;
     std::vector<upd_task> upd, updb;
     ctx->levels.assign( nlev, level_t{} );
@@ -216,6 +220,7 @@ build_schedule( pxd_ctx_t *ctx )
         lv.u0   = upd.size();
         lv.b0   = updb.size();
         lv.f0   = solv.size();
+        lv.d0   = diag.size();
         lv.i0   = invt.size();
         lv.maxn = 0;
         ivs.clear();
@@ -229,6 +234,9 @@ build_schedule( pxd_ctx_t *ctx )
             }
             for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
                 solv.push_back( row_task{ k, r0 } );
+            }
+            for ( int32_t r0 = 0; r0 < nk; r0 += 64 ) {
+                diag.push_back( row_task{ k, r0 } );
             }
             for ( int32_t J = 0; J + PXD_NB < nk; J += PXD_NB ) {
                 invt.push_back( inv_task{ k, J } );
@@ -310,6 +318,7 @@ build_schedule( pxd_ctx_t *ctx )
         lv.u1 = upd.size();
         lv.b1 = updb.size();
         lv.f1 = solv.size();
+        lv.d1 = diag.size();
         lv.i1 = invt.size();
     }
     ctx->nupd  = upd.size() + updb.size();
@@ -319,7 +328,7 @@ build_schedule( pxd_ctx_t *ctx )
     ctx->stats.update_tiles = upd.size() + updb.size();
     if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
          upload( &ctx->dupdb, updb ) ||
-         upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) ) {
+         upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
         return 6;
     }
     return 0;
@@ -339,7 +348,8 @@ template <typename T>
 int
 factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
 {
-    const bool lu = ( ctx->facto == FACT_GETRF );
+    const bool lu   = ( ctx->facto == FACT_GETRF );
+    const int  ldlt = ( ctx->facto == FACT_SYTRF );
     T *L  = (T *)ctx->dL;
     T *DI = (T *)ctx->ddinv;                        // Cholesky: L_kk^{-1}; LU: (unit L_kk)^{-1}
     T *U  = lu ? (T *)ctx->dU : L;                  // LU: U^T tiles
@@ -393,25 +403,34 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     auto panel = [&]( const level_t &lv, hipStream_t st ) {
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
         unsigned ni = (unsigned)( lv.i1 - lv.i0 );
+        // rows below a block column of the widest cblk of the level
+        size_t trtri_lds = (size_t)std::max( 1, lv.maxn - PXD_NB ) * ( PXD_NB + 1 ) * sizeof( T );
         if ( lu ) {
             hipLaunchKernelGGL( ( k_getrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch );
             launches++;
             if ( ni ) {
-                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
-                                    ctx->dcb, (const T *)L, DI );
-                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
-                                    ctx->dcb, (const T *)U, DU );
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI );
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU );
                 launches += 2;
             }
         }
         else if ( ctx->kernels ) {
-            hipLaunchKernelGGL( ( k_potrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
-                                ctx->dcb, L, DI, thresh, ctx->dscratch );
+            const size_t sh = (size_t)std::max( 1, lv.maxn ) * ( PXD_NB + 1 ) * sizeof( T );
+            if ( ldlt ) {
+                hipLaunchKernelGGL( ( k_potrf2<T, true> ), dim3( np ), dim3( PXD_THREADS ), sh, st, ctx->dlist + lv.c0,
+                                    ctx->dcb, L, DI, thresh, ctx->dscratch );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
+                                    ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch );
+            }
             launches++;
             if ( ni ) {
-                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), 0, st, ctx->dinvt + lv.i0,
-                                    ctx->dcb, (const T *)L, DI );
+                hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI );
                 launches++;
             }
         }
@@ -429,14 +448,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         if ( lu ) {
             // L_off <- L_off U_kk^{-1};  U^T_off <- U^T_off L_kk^{-T}
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DU );
+                                ctx->dcb, ctx->dbk, L, (const T *)DU, 0 );
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, U, (const T *)DI );
+                                ctx->dcb, ctx->dbk, U, (const T *)DI, 0 );
             launches++;
         }
         else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DI );
+                                ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt );
         }
         else if ( ctx->trsm_rt == 32 ) {
             hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ), st,
@@ -454,12 +473,12 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         unsigned nb = (unsigned)( b1 - b0 );
         if ( nb ) {
             hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U );
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt );
             launches++;
         }
         if ( nu && ctx->kernels ) {
             hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
-                                ctx->dcb, ctx->dbk, L, U );
+                                ctx->dcb, ctx->dbk, L, U, ldlt );
             launches++;
         }
         else if ( nu ) {
@@ -545,49 +564,76 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
     hipStream_t s  = ctx->stream;
     const int   nl = (int)ctx->levels.size();
 
-    // forward: L y = b
+    if ( ctx->kernels ) {
+        // matrix-core path, diagonal solves chunked and out of place:
+        // forward  -- the diagonal reads X and writes Y; the off-diagonal
+        //             update reads x_k from Y and subtracts into X;
+        // backward -- the gather reads the solved rows from X and subtracts
+        //             into Y; the diagonal reads Y and writes X.
+        size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( T );
+        if ( ctx->dsolvy_bytes < need ) {
+            if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
+            PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
+            ctx->dsolvy_bytes = need;
+        }
+        T            *Y   = (T *)ctx->dsolvy;
+        const int64_t ldy = ctx->n;
+        for ( int li = 0; li < nl; li++ ) {
+            const level_t &lv = ctx->levels[li];
+            unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
+            unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
+            hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
+                                ctx->dcb, DI, (const T *)X, ldx, Y, ldy, nrhs, 0 );
+            if ( nf ) {
+                hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
+            }
+        }
+        if ( ctx->facto == FACT_SYTRF ) {
+            hipLaunchKernelGGL( ( k_diag_scale<T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
+                                ctx->dcol2cblk, ctx->dcb, L, Y, ldy, nrhs );
+        }
+        for ( int li = nl - 1; li >= 0; li-- ) {
+            const level_t &lv = ctx->levels[li];
+            unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
+            unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
+            if ( nf ) {
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const T *)X, ldx, Y, ldy, nrhs );
+            }
+            hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
+                                ctx->dcb, DB, (const T *)Y, ldy, X, ldx, nrhs, 1 );
+        }
+        PXD_CHECK( hipGetLastError() );
+        return 0;
+    }
+
+    // scalar kernels, in place
     for ( int li = 0; li < nl; li++ ) {
         const level_t &lv = ctx->levels[li];
         unsigned       np = (unsigned)( lv.c1 - lv.c0 );
         unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
         size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
-        if ( ctx->kernels ) {
-            hipLaunchKernelGGL( ( k_solve_diag_inv<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dlist + lv.c0, ctx->dcb, DI, X, ldx, nrhs, 0 );
-        }
-        else {
-            hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
-        }
+        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
         if ( nf ) {
             hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
+                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)X, ldx );
         }
     }
-    // backward: L^T x = y
     for ( int li = nl - 1; li >= 0; li-- ) {
         const level_t &lv = ctx->levels[li];
         unsigned       np = (unsigned)( lv.c1 - lv.c0 );
         unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
         size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
         if ( nf ) {
-            if ( ctx->kernels ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, X, ldx, nrhs );
-            }
-            else {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
-            }
+            hipLaunchKernelGGL( ( k_solve_bwd_gather<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs );
         }
-        if ( ctx->kernels ) {
-            hipLaunchKernelGGL( ( k_solve_diag_inv<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dlist + lv.c0, ctx->dcb, DB, X, ldx, nrhs, 1 );
-        }
-        else {
-            hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
-        }
+        hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
+                            ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
     }
     PXD_CHECK( hipGetLastError() );
     return 0;
@@ -602,7 +648,10 @@ set_lds_attributes( size_t lds )
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_diag_inv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
 }
 
 }  // namespace
@@ -662,7 +711,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
     for ( void *p : ptrs ) {
@@ -896,8 +945,11 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
     if ( ctx->dval == nullptr || ctx->dcb == nullptr ) {
         return 7;
     }
-    if ( factotype != FACT_POTRF && factotype != FACT_GETRF ) {
+    if ( factotype != FACT_POTRF && factotype != FACT_GETRF && factotype != FACT_SYTRF ) {
         return 3;
+    }
+    if ( factotype == FACT_SYTRF && !ctx->kernels ) {
+        return 7;   // LDL^T runs on the matrix-core kernels only
     }
     if ( factor_flttype != FT_DOUBLE && factor_flttype != FT_FLOAT ) {
         return 3;

@@ -49,6 +49,7 @@ struct row_task {
 };
 
 #define PXD_NB 32          // sub-block size of panel / inverses
+#define PXD_TRSM_NT 22     // 16-column tiles of the widest cblk (352 columns)
 #define PXD_THREADS 256
 
 // Blok of cblk k (bloks [fb, lb)) that holds stacked row s.

@@ -73,7 +73,7 @@ search_grow( const search_keys &sk, const dblok *__restrict__ bk, int32_t ffb, i
 // PXD_NB columns; the unblocked step updates with unscaled columns so that it
 // needs one barrier per column.  The inverses of the PXD_NB x PXD_NB diagonal
 // sub-blocks go to the diagonal blocks of Linv.
-template <typename T>
+template <typename T, bool LDLT>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L,
           T *__restrict__ Linv, double thresh, int *__restrict__ npiv )
@@ -81,8 +81,10 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     __shared__ T sA[PXD_NB][PXD_NB + 1];   // sA[col][row]
     __shared__ T sI[PXD_NB][PXD_NB + 1];   // inverse, sI[col][row]
     __shared__ T sD[PXD_NB];
-    __shared__ T sPi[PXD_NB][64 + 1];
-    __shared__ T sPj[PXD_NB][64 + 1];
+    __shared__ T sDi[PXD_NB];               // LDL^T: 1 / D of the block, 0 beyond it
+    // L21 of the current block, sP[row][k]: dynamic LDS for the widest cblk of the launch
+    extern __shared__ unsigned char smem_raw[];
+    T ( *sP )[PXD_NB + 1] = reinterpret_cast<T ( * )[PXD_NB + 1]>( smem_raw );
 
     const int32_t k   = list[blockIdx.x];
     const dcblk   c   = cb[k];
@@ -116,7 +118,8 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         for ( int cc = 0; cc < bs; cc++ ) {
             T d = sA[cc][cc];
             if ( px_abs( d ) < tau ) {
-                d = tau;
+                // Cholesky: tau (core_zpotf2sp); LDL^T: sign(d) tau (core_zsytf2sp)
+                d = ( LDLT && d < (T)0 ) ? -tau : tau;
                 if ( tid == 0 ) {
                     piv++;
                 }
@@ -134,36 +137,48 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             }
             __syncthreads();
         }
-        // scale the columns by 1/sqrt(pivot)
+        // Cholesky: scale the columns by 1/sqrt(pivot).  LDL^T: L = column / d,
+        // the diagonal holds D and the upper part D L^T (the unscaled columns)
         for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
             int rr = idx % bs, cc = idx / bs;
             if ( rr > cc ) {
-                sA[cc][rr] = sA[cc][rr] / px_sqrt( sD[cc] );
+                const T a = sA[cc][rr];
+                if ( LDLT ) {
+                    sA[rr][cc] = a;
+                    sA[cc][rr] = a / sD[cc];
+                }
+                else {
+                    sA[cc][rr] = a / px_sqrt( sD[cc] );
+                }
             }
         }
         __syncthreads();
         if ( tid < bs ) {
-            sA[tid][tid] = px_sqrt( sD[tid] );
+            sA[tid][tid] = LDLT ? sD[tid] : px_sqrt( sD[tid] );
+        }
+        if ( tid < PXD_NB ) {
+            sDi[tid] = ( tid < bs ) ? (T)1 / sD[tid] : (T)0;
         }
         __syncthreads();
 
-        // inverse of the lower-triangular block: thread t builds column t
+        // inverse of the lower-triangular block (LDL^T: of the unit L):
+        // thread t builds column t
         if ( tid < bs ) {
             const int t = tid;
-            sI[t][t] = (T)1 / sA[t][t];
+            sI[t][t] = LDLT ? (T)1 : (T)1 / sA[t][t];
             for ( int i = t + 1; i < bs; i++ ) {
                 T s = (T)0;
                 for ( int l = t; l < i; l++ ) {
                     s += sA[l][i] * sI[t][l];
                 }
-                sI[t][i] = -s / sA[i][i];
+                sI[t][i] = LDLT ? -s : -s / sA[i][i];
             }
         }
         __syncthreads();
 
         for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
             int rr = idx % bs, cc = idx / bs;
-            if ( rr >= cc ) {
+            if ( LDLT || rr >= cc ) {
                 A[( kb + rr ) + (int64_t)( kb + cc ) * n] = sA[cc][rr];
             }
             LI[( kb + rr ) + (int64_t)( kb + cc ) * n] = sI[cc][rr];
@@ -171,100 +186,116 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 
         // rows below the block inside the tile: A21 <- A21 * L11^{-T} on the
         // matrix cores, one wave per 16-row block (two 16 x 16 output tiles,
-        // the inverse's transpose read from LDS); a block's loads all precede
-        // its stores and no other wave touches its rows
-        {
-            typedef mfma16<T>         M;
-            typedef typename M::acc_t acc_t;
-            const int lane = tid & 63, w = tid >> 6;
-            for ( int32_t rb = kb + bs + 16 * w; rb < n; rb += 64 ) {
-                T a[PXD_NB / 4];
-                const int32_t ra = rb + ( lane & 15 );
+        // the inverse's transpose read from LDS); the result goes to A and to
+        // sP for the trailing update
+        typedef mfma16<T>         M;
+        typedef typename M::acc_t acc_t;
+        const int     lane = tid & 63, w = tid >> 6;
+        const int32_t m0   = kb + bs;
+        const int32_t m    = n - m0;
+        for ( int32_t rb = 16 * w; rb < m; rb += 64 ) {
+            T             a[PXD_NB / 4];
+            const int32_t ra = m0 + rb + ( lane & 15 );
 #pragma unroll
-                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                    const int l = 4 * s4 + ( lane >> 4 );
-                    a[s4] = ( ra < n && l < bs ) ? A[ra + (int64_t)( kb + l ) * n] : (T)0;
-                }
-                acc_t acc0 = M::zero(), acc1 = M::zero();
+            for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                const int l = 4 * s4 + ( lane >> 4 );
+                a[s4] = ( ra < n && l < bs ) ? A[ra + (int64_t)( kb + l ) * n] : (T)0;
+            }
+            acc_t acc0 = M::zero(), acc1 = M::zero();
 #pragma unroll
-                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                    const int l = 4 * s4 + ( lane >> 4 );
-                    acc0 = M::mma( a[s4], sI[l][( lane & 15 )], acc0 );
-                    acc1 = M::mma( a[s4], sI[l][16 + ( lane & 15 )], acc1 );
-                }
+            for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                const int l = 4 * s4 + ( lane >> 4 );
+                acc0 = M::mma( a[s4], sI[l][( lane & 15 )], acc0 );
+                acc1 = M::mma( a[s4], sI[l][16 + ( lane & 15 )], acc1 );
+            }
 #pragma unroll
-                for ( int v = 0; v < 4; v++ ) {
-                    const int32_t rr = rb + M::row( lane, v );
-                    const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
-                    if ( rr < n && c0 < bs ) A[rr + (int64_t)( kb + c0 ) * n] = acc0[v];
-                    if ( rr < n && c1 < bs ) A[rr + (int64_t)( kb + c1 ) * n] = acc1[v];
+            for ( int v = 0; v < 4; v++ ) {
+                const int32_t rr = rb + M::row( lane, v );
+                const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
+                if ( rr < m ) {
+                    // LDL^T: the product is L21 D; L21 = product / D
+                    sP[rr][c0] = acc0[v];   // zero beyond bs: sI is zero there
+                    sP[rr][c1] = acc1[v];
+                    if ( c0 < bs ) A[( m0 + rr ) + (int64_t)( kb + c0 ) * n] = LDLT ? acc0[v] * sDi[c0] : acc0[v];
+                    if ( c1 < bs ) A[( m0 + rr ) + (int64_t)( kb + c1 ) * n] = LDLT ? acc1[v] * sDi[c1] : acc1[v];
                 }
             }
         }
         __syncthreads();
 
-        // trailing update A22 -= A21 * A21^T (lower part, 64 x 64 macro tiles)
-        const int32_t m0 = kb + bs;
-        const int32_t m  = n - m0;
-        const int32_t nt = ( m + 63 ) / 64;
-        const int     tx = tid % 16, ty = tid / 16;
-        for ( int32_t ti = 0; ti < nt; ti++ ) {
-            for ( int32_t tj = 0; tj <= ti; tj++ ) {
-                {
-                    constexpr int NU = 64 * PXD_NB / PXD_THREADS;
-                    T vi[NU], vj[NU];
+        // trailing update A22 -= L21 L21^T (lower part) on the matrix cores
+        // from sP: 32 x 32 tiles, wave w takes every fourth tile, two at a
+        // time, with the tiles' loads of A22 issued before their products
+        {
+            const int32_t nt = ( m + 31 ) / 32;
+            const int32_t ntile = nt * ( nt + 1 ) / 2;
+            for ( int32_t t0 = 2 * w; t0 < ntile; t0 += 8 ) {
+                int32_t ti[2], tj[2];
+                bool    tv[2];
 #pragma unroll
-                    for ( int u = 0; u < NU; u++ ) {
-                        const int     idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
-                        const int32_t ri = ti * 64 + r, rj = tj * 64 + r;
-                        vi[u] = ( ri < m && l < bs ) ? A[( m0 + ri ) + (int64_t)( kb + l ) * n] : (T)0;
-                        vj[u] = ( rj < m && l < bs ) ? A[( m0 + rj ) + (int64_t)( kb + l ) * n] : (T)0;
-                    }
-#pragma unroll
-                    for ( int u = 0; u < NU; u++ ) {
-                        const int idx = tid + u * PXD_THREADS, r = idx % 64, l = idx / 64;
-                        sPi[l][r] = vi[u];
-                        sPj[l][r] = vj[u];
-                    }
+                for ( int g = 0; g < 2; g++ ) {
+                    // tile t0 + g in row-wise order of the lower triangle
+                    int32_t tt = t0 + g, r = 0;
+                    while ( ( r + 1 ) * ( r + 2 ) / 2 <= tt ) r++;
+                    ti[g] = r;
+                    tj[g] = tt - r * ( r + 1 ) / 2;
+                    tv[g] = tt < ntile;
                 }
-                __syncthreads();
-                T acc[4][4];
+                T old[2][2][2][4];
 #pragma unroll
-                for ( int a = 0; a < 4; a++ )
+                for ( int g = 0; g < 2; g++ )
 #pragma unroll
-                    for ( int b = 0; b < 4; b++ ) acc[a][b] = (T)0;
-                for ( int l = 0; l < bs; l++ ) {
-                    T ai[4], aj[4];
+                    for ( int a = 0; a < 2; a++ )
 #pragma unroll
-                    for ( int a = 0; a < 4; a++ ) {
-                        ai[a] = sPi[l][tx + 16 * a];
-                        aj[a] = sPj[l][ty + 16 * a];
-                    }
+                        for ( int b = 0; b < 2; b++ )
 #pragma unroll
-                    for ( int a = 0; a < 4; a++ )
+                            for ( int v = 0; v < 4; v++ ) {
+                                const int32_t r  = 32 * ti[g] + 16 * a + M::row( lane, v );
+                                const int32_t cc = 32 * tj[g] + 16 * b + ( lane & 15 );
+                                const bool    ok = tv[g] && r < m && cc < m && r >= cc;
+                                old[g][a][b][v] = ok ? A[( m0 + r ) + (int64_t)( m0 + cc ) * n] : (T)0;
+                            }
+                acc_t acc[2][2][2];
 #pragma unroll
-                        for ( int b = 0; b < 4; b++ ) acc[a][b] += ai[a] * aj[b];
-                }
-                T old[4][4];
+                for ( int g = 0; g < 2; g++ )
 #pragma unroll
-                for ( int a = 0; a < 4; a++ ) {
+                    for ( int a = 0; a < 2; a++ )
 #pragma unroll
-                    for ( int b = 0; b < 4; b++ ) {
-                        int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
-                        old[a][b] = ( r < m && cc < m && r >= cc ) ? A[( m0 + r ) + (int64_t)( m0 + cc ) * n] : (T)0;
-                    }
-                }
+                        for ( int b = 0; b < 2; b++ ) acc[g][a][b] = M::zero();
 #pragma unroll
-                for ( int a = 0; a < 4; a++ ) {
+                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                    const int l = 4 * s4 + ( lane >> 4 );
 #pragma unroll
-                    for ( int b = 0; b < 4; b++ ) {
-                        int32_t r = ti * 64 + tx + 16 * a, cc = tj * 64 + ty + 16 * b;
-                        if ( r < m && cc < m && r >= cc ) {
-                            A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[a][b] - acc[a][b];
+                    for ( int g = 0; g < 2; g++ ) {
+                        T af[2], bf[2];
+#pragma unroll
+                        for ( int a = 0; a < 2; a++ ) {
+                            const int32_t ri = 32 * ti[g] + 16 * a + ( lane & 15 );
+                            const int32_t rj = 32 * tj[g] + 16 * a + ( lane & 15 );
+                            // LDL^T: A22 -= (L21 D) D^{-1} (L21 D)^T
+                            af[a] = ( ri < m ) ? ( LDLT ? sP[ri][l] * sDi[l] : sP[ri][l] ) : (T)0;
+                            bf[a] = ( rj < m ) ? sP[rj][l] : (T)0;
                         }
+#pragma unroll
+                        for ( int a = 0; a < 2; a++ )
+#pragma unroll
+                            for ( int b = 0; b < 2; b++ ) acc[g][a][b] = M::mma( af[a], bf[b], acc[g][a][b] );
                     }
                 }
-                __syncthreads();
+#pragma unroll
+                for ( int g = 0; g < 2; g++ )
+#pragma unroll
+                    for ( int a = 0; a < 2; a++ )
+#pragma unroll
+                        for ( int b = 0; b < 2; b++ )
+#pragma unroll
+                            for ( int v = 0; v < 4; v++ ) {
+                                const int32_t r  = 32 * ti[g] + 16 * a + M::row( lane, v );
+                                const int32_t cc = 32 * tj[g] + 16 * b + ( lane & 15 );
+                                if ( tv[g] && r < m && cc < m && r >= cc ) {
+                                    A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[g][a][b][v] - acc[g][a][b][v];
+                                }
+                            }
             }
         }
         __syncthreads();
@@ -537,92 +568,118 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 }
 
 // ---------------------------------------------------------------------------
-// Inverse of the lower-triangular L_kk, one workgroup per (cblk, block column
-// J): Linv[I,J] = -Linv[I,I] * sum_{K=J}^{I-1} L[I,K] Linv[K,J] for I > J, the
-// diagonal blocks Linv[I,I] written by the panel.  Block column J of Linv is
-// produced top-down and read back from global memory by the same workgroup.
+// Inverse of the lower-triangular L_kk by block columns, one workgroup per
+// (cblk, block column J).  The diagonal blocks D_K = L[K,K]^{-1} were written
+// to Linv by the panel.  Column sweep: R = -L[J+1:, J] D_J, then for K = J+1,
+// J+2, ...: X[K,J] = D_K R[K], R[K+1:] -= L[K+1:, K] X[K,J].  R stays in LDS;
+// both products run on the matrix cores.
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ L,
          T *__restrict__ Linv )
 {
-    __shared__ T sL[PXD_NB][PXD_NB + 1];   // L[I,K]        sL[col][row]
-    __shared__ T sX[PXD_NB][PXD_NB + 1];   // Linv[K,J]     sX[col][row]
-    __shared__ T sT[PXD_NB][PXD_NB + 1];   // accumulated T sT[col][row]
+    typedef mfma16<T>         M;
+    typedef typename M::acc_t acc_t;
+    __shared__ T sX[PXD_NB][PXD_NB + 1];   // the current X[K,J] (or D_J), sX[row][col]
+    __shared__ T sD[PXD_NB][PXD_NB + 1];   // D_K, sD[row][col]
+    extern __shared__ unsigned char smem_raw[];
+    T ( *sR )[PXD_NB + 1] = reinterpret_cast<T ( * )[PXD_NB + 1]>( smem_raw );   // rows below block J
 
-    const inv_task t   = tasks[blockIdx.x];
-    const dcblk    c   = cb[t.cblk];
-    const int32_t  n   = c.lcolnum - c.fcolnum + 1;
-    const int      tid = threadIdx.x;
-    const T       *A   = L + c.coefoff;
-    T             *LI  = Linv + c.dinvoff;
-    const int32_t  J   = t.J;
-    const int32_t  bj  = min( PXD_NB, n - J );
+    const inv_task t    = tasks[blockIdx.x];
+    const dcblk    c    = cb[t.cblk];
+    const int32_t  n    = c.lcolnum - c.fcolnum + 1;
+    const int      tid  = threadIdx.x;
+    const int      lane = tid & 63, w = tid >> 6;
+    const T       *A    = L + c.coefoff;
+    T             *LI   = Linv + c.dinvoff;
+    const int32_t  J    = t.J;                 // first column of the block column
+    const int32_t  r0   = J + PXD_NB;          // first row of R
+    const int32_t  nr   = n - r0;
+    const int32_t  bj   = min( PXD_NB, n - J );
 
-    for ( int32_t I = J + PXD_NB; I < n; I += PXD_NB ) {
-        const int32_t bi = min( PXD_NB, n - I );
-        T             acc[4];
+    // load a 32 x 32 block of Linv (rows R0, cols C0) into dst[row][col]
+    auto load_block = [&]( T ( *dst )[PXD_NB + 1], int32_t R0, int32_t C0, int32_t nrow, int32_t ncol ) {
+        constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
+        T v[NU];
 #pragma unroll
-        for ( int q = 0; q < 4; q++ ) acc[q] = (T)0;
-        // thread owns (row r, cols cq..): r = tid % 32, cols tid/32 + 8q
-        const int r = tid % PXD_NB, c0 = tid / PXD_NB;
-        for ( int32_t K = J; K < I; K += PXD_NB ) {
-            const int32_t bk = min( PXD_NB, n - K );
-            {
-                constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
-                T vl[NU], vx[NU];
-#pragma unroll
-                for ( int u = 0; u < NU; u++ ) {
-                    const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
-                    vl[u] = ( rr < bi && cc < bk ) ? A[( I + rr ) + (int64_t)( K + cc ) * n] : (T)0;
-                    vx[u] = ( rr < bk && cc < bj ) ? LI[( K + rr ) + (int64_t)( J + cc ) * n] : (T)0;
-                }
-#pragma unroll
-                for ( int u = 0; u < NU; u++ ) {
-                    const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
-                    sL[cc][rr] = vl[u];
-                    sX[cc][rr] = vx[u];
-                }
-            }
-            __syncthreads();
-#pragma unroll 8
-            for ( int l = 0; l < PXD_NB; l++ ) {
-                T a = sL[l][r];
-#pragma unroll
-                for ( int q = 0; q < 4; q++ ) acc[q] += a * sX[c0 + 8 * q][l];
-            }
-            __syncthreads();
+        for ( int u = 0; u < NU; u++ ) {
+            const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+            v[u] = ( r < nrow && cc < ncol ) ? LI[( R0 + r ) + (int64_t)( C0 + cc ) * n] : (T)0;
         }
 #pragma unroll
-        for ( int q = 0; q < 4; q++ ) sT[c0 + 8 * q][r] = acc[q];
-        // Linv[I,I] into sL
+        for ( int u = 0; u < NU; u++ ) {
+            const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+            dst[r][cc] = v[u];
+        }
+    };
+    // R[rows >= rfirst] -= L[r0 + rows, K .. K+31] * sX  (16-row blocks per wave)
+    auto sweep = [&]( int32_t rfirst, int32_t K, int32_t bk ) {
+        for ( int32_t rb = rfirst + 16 * w; rb < nr; rb += 64 ) {
+            T             a[PXD_NB / 4];
+            const int32_t ra = rb + ( lane & 15 );
+#pragma unroll
+            for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                const int l = 4 * s4 + ( lane >> 4 );
+                a[s4] = ( ra < nr && l < bk ) ? -A[( r0 + ra ) + (int64_t)( K + l ) * n] : (T)0;
+            }
+            acc_t acc0, acc1;
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) {
+                const int32_t rr = rb + M::row( lane, v );
+                acc0[v] = ( rr < nr ) ? sR[rr][lane & 15] : (T)0;
+                acc1[v] = ( rr < nr ) ? sR[rr][16 + ( lane & 15 )] : (T)0;
+            }
+#pragma unroll
+            for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                const int l = 4 * s4 + ( lane >> 4 );
+                acc0 = M::mma( a[s4], sX[l][lane & 15], acc0 );
+                acc1 = M::mma( a[s4], sX[l][16 + ( lane & 15 )], acc1 );
+            }
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) {
+                const int32_t rr = rb + M::row( lane, v );
+                if ( rr < nr ) {
+                    sR[rr][lane & 15]        = acc0[v];
+                    sR[rr][16 + ( lane & 15 )] = acc1[v];
+                }
+            }
+        }
+    };
+
+    for ( int idx = tid; idx < nr * ( PXD_NB + 1 ); idx += PXD_THREADS ) {
+        ( &sR[0][0] )[idx] = (T)0;
+    }
+    load_block( sX, J, J, bj, bj );   // D_J
+    __syncthreads();
+    sweep( 0, J, bj );                // R = -L[r0:, J] D_J
+    for ( int32_t K = r0; K < n; K += PXD_NB ) {
+        const int32_t bk = min( PXD_NB, n - K );
+        load_block( sD, K, K, bk, bk );
+        __syncthreads();
+        // X[K,J] = D_K R[K - r0 .. +31]: wave w computes the 16 x 16 block
+        // (w >> 1, w & 1) of the 32 x 32 product
         {
-            constexpr int NU = PXD_NB * PXD_NB / PXD_THREADS;
-            T vl[NU];
+            const int32_t bi = w >> 1, bc = w & 1;
+            acc_t         acc = M::zero();
 #pragma unroll
-            for ( int u = 0; u < NU; u++ ) {
-                const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
-                vl[u] = ( rr < bi && cc < bi ) ? LI[( I + rr ) + (int64_t)( I + cc ) * n] : (T)0;
+            for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                const int l = 4 * s4 + ( lane >> 4 );
+                const T   a = sD[16 * bi + ( lane & 15 )][l];
+                const T   b = ( K - r0 + l < nr ) ? sR[K - r0 + l][16 * bc + ( lane & 15 )] : (T)0;
+                acc = M::mma( a, b, acc );
             }
+            __syncthreads();   // every wave has read sX (previous step) and sR[K block]
 #pragma unroll
-            for ( int u = 0; u < NU; u++ ) {
-                const int idx = tid + u * PXD_THREADS, rr = idx % PXD_NB, cc = idx / PXD_NB;
-                sL[cc][rr] = vl[u];
+            for ( int v = 0; v < 4; v++ ) {
+                const int r = 16 * bi + M::row( lane, v ), cc = 16 * bc + ( lane & 15 );
+                sX[r][cc] = acc[v];
+                if ( r < bk && cc < bj ) LI[( K + r ) + (int64_t)( J + cc ) * n] = acc[v];
             }
         }
         __syncthreads();
-#pragma unroll
-        for ( int q = 0; q < 4; q++ ) {
-            const int cc = c0 + 8 * q;
-            T         s  = (T)0;
-            for ( int l = 0; l <= r; l++ ) {
-                s += sL[l][r] * sT[cc][l];
-            }
-            if ( r < bi && cc < bj ) {
-                LI[( I + r ) + (int64_t)( J + cc ) * n] = -s;
-            }
+        if ( K + PXD_NB < n ) {
+            sweep( K + PXD_NB - r0, K, bk );
         }
-        __syncthreads();
     }
 }
 
@@ -632,11 +689,10 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 // owns complete rows, so every read of B precedes every write of X).
 // Wave w computes rows 16w..16w+15 against every 16-column tile; column tiles
 // that only meet the zero upper part of Linv are skipped.
-#define PXD_TRSM_NT 22   /* up to 352 columns */
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-             const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv )
+             const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv, int ldlt )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -645,6 +701,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ T       sLi[BK][PXD_TRSM_NT * 16 + 1];
     __shared__ int64_t sBase[64];
     __shared__ int32_t sLd[64];
+    __shared__ T       sDi[PXD_TRSM_NT * 16];   // LDL^T: 1 / D of the cblk
 
     const row_task t    = tasks[blockIdx.x];
     const dcblk    c    = cb[t.cblk];
@@ -655,6 +712,9 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const T       *LI   = Linv + c.dinvoff;
     const int      nt   = ( n + 15 ) >> 4;
 
+    if ( ldlt ) {
+        for ( int32_t i = tid; i < n; i += PXD_THREADS ) sDi[i] = (T)1 / L[c.coefoff + (int64_t)i * ( n + 1 )];
+    }
     if ( tid < 64 ) {
         int32_t s = t.r0 + tid;
         if ( s < c.stride ) {
@@ -678,23 +738,25 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const int64_t lbase = sBase[lr];
     const int32_t lld   = sLd[lr];
 
-    for ( int32_t l0 = 0; l0 < n; l0 += BK ) {
-        // all loads of the stage are issued before the first LDS store
-        T rb[BK / 4];
+    // the next stage's loads are issued before the current stage's products
+    constexpr int NLI = ( BK * PXD_TRSM_NT * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
+    T             rb[BK / 4], rl[NLI];
+    auto          load = [&]( int32_t l0 ) {
 #pragma unroll
         for ( int u = 0; u < BK / 4; u++ ) {
             const int32_t l = l0 + ( tid >> 6 ) + 4 * u;
             rb[u] = ( lbase >= 0 && l < n ) ? L[lbase + (int64_t)l * lld] : (T)0;
         }
-        constexpr int NLI = ( BK * PXD_TRSM_NT * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
-        T rl[NLI];
 #pragma unroll
         for ( int u = 0; u < NLI; u++ ) {
-            const int idx = tid + u * PXD_THREADS;
-            const int cc = idx % ( PXD_TRSM_NT * 16 ), kk = idx / ( PXD_TRSM_NT * 16 );
+            const int     idx = tid + u * PXD_THREADS;
+            const int     cc = idx % ( PXD_TRSM_NT * 16 ), kk = idx / ( PXD_TRSM_NT * 16 );
             const int32_t l = l0 + kk;
             rl[u] = ( kk < BK && cc < n && l < n && 16 * ( cc >> 4 ) + 15 >= l ) ? LI[cc + (int64_t)l * n] : (T)0;
         }
+    };
+    load( 0 );
+    for ( int32_t l0 = 0; l0 < n; l0 += BK ) {
 #pragma unroll
         for ( int u = 0; u < BK / 4; u++ ) sB[( tid >> 6 ) + 4 * u][lr] = rb[u];
 #pragma unroll
@@ -703,6 +765,9 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
             if ( idx < BK * PXD_TRSM_NT * 16 ) sLi[idx / ( PXD_TRSM_NT * 16 )][idx % ( PXD_TRSM_NT * 16 )] = rl[u];
         }
         __syncthreads();
+        if ( l0 + BK < n ) {
+            load( l0 + BK );
+        }
 #pragma unroll
         for ( int k4 = 0; k4 < BK; k4 += 4 ) {
             const T a = sB[k4 + ( lane >> 4 )][16 * w + ( lane & 15 )];
@@ -724,7 +789,8 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
             for ( int v = 0; v < 4; v++ ) {
                 const int r = 16 * w + M::row( lane, v );
                 if ( sBase[r] >= 0 && cc < n ) {
-                    L[sBase[r] + (int64_t)cc * sLd[r]] = acc[j][v];
+                    // LDL^T: L_off = B L_kk^{-T} D^{-1}
+                    L[sBase[r] + (int64_t)cc * sLd[r]] = ldlt ? acc[j][v] * sDi[cc] : acc[j][v];
                 }
             }
         }
@@ -739,7 +805,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-               const dblok *__restrict__ bk, T *L, T *U )
+               const dblok *__restrict__ bk, T *L, T *U, int ldlt )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -751,6 +817,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ int64_t sDst[64];
     __shared__ int32_t sDstLd[64];
     __shared__ search_keys sk;
+    __shared__ T           sDk[PXD_TRSM_NT * 16];   // LDL^T: D of the source cblk
 
     const upd_task t    = tasks[blockIdx.x];
     const int      tid  = threadIdx.x;
@@ -778,6 +845,10 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     }
 
     stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
+    if ( ldlt ) {
+        // LDL^T: C -= A_i (A_b D)^T (the reference's DL^T workspace, formed on the fly)
+        for ( int32_t i = tid; i < K; i += PXD_THREADS ) sDk[i] = L[c.coefoff + (int64_t)i * ( K + 1 )];
+    }
     __syncthreads();
     if ( tid < 64 ) {
         int32_t r = t.tm * 64 + tid;
@@ -825,8 +896,9 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
         }
 #pragma unroll
         for ( int u = 0; u < BK / 4; u++ ) {
+            const int32_t kg = k0 + ( tid >> 6 ) + 4 * u;
             sA[( tid >> 6 ) + 4 * u][lr] = va[u];
-            sB[( tid >> 6 ) + 4 * u][lr] = vb[u];
+            sB[( tid >> 6 ) + 4 * u][lr] = ( ldlt && kg < K ) ? vb[u] * sDk[kg] : vb[u];
         }
         __syncthreads();
 #pragma unroll
@@ -950,7 +1022,7 @@ template <> struct upd_rows<double> { static constexpr int value = 128; };
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS, 3 )
 k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
-              const dblok *__restrict__ bk, T *L, T *U )
+              const dblok *__restrict__ bk, T *L, T *U, int ldlt )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -970,6 +1042,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     __shared__ uint32_t sDst[BM];         // byte offset of the row in F (PXD_OOB: none)
     __shared__ uint32_t sDstLd[BM];       // byte stride between F's columns for the row
     __shared__ search_keys sk;
+    __shared__ T           sDk[PXD_TRSM_NT * 16];   // LDL^T: D of the source cblk
 
     const upd_task t    = tasks[xcd_task( blockIdx.x, ntasks )];
     const int      tid  = threadIdx.x;
@@ -999,6 +1072,10 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const __amdgpu_buffer_rsrc_t rC = make_rsrc( LC + cf.coefoff, (uint32_t)( (int64_t)cf.stride * Fn * ES ) );
 
     stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
+    if ( ldlt ) {
+        // LDL^T: C -= A_i (A_b D)^T (the reference's DL^T workspace, formed on the fly)
+        for ( int32_t i = tid; i < K; i += PXD_THREADS ) sDk[i] = L[c.coefoff + (int64_t)i * ( K + 1 )];
+    }
     __syncthreads();
 
     // loader row ar (k values [ap*KPT, ap*KPT+KPT) of each stage): byte
@@ -1067,7 +1144,10 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
         {
             vec_t v;
 #pragma unroll
-            for ( int e = 0; e < KV; e++ ) v[e] = rb[e];
+            for ( int e = 0; e < KV; e++ ) {
+                const int32_t kg = k0 + bq * KV + e;
+                v[e] = ( ldlt && kg < K ) ? rb[e] * sDk[kg] : rb[e];
+            }
             *(vec_t *)&sB[bc * LDK + bq * KV] = v;
         }
         __syncthreads();
@@ -1134,86 +1214,80 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal solves with the explicit inverse: x_k <- Linv x_k (forward) or
-// Linv^T x_k (backward) for a list of cblks, nrhs <= PXD_MAXRHS.  The
-// product runs over 64 x 64 chunks of the operator staged in LDS (loads
-// coalesced along the columns of Linv in both orientations); chunks in the
-// zero triangle of the operator are skipped.
+// Diagonal solve of one 64-row chunk of a cblk, out of place: Y[R0..R0+63]
+// <- (Linv or Linv^T)[R0.., :] Xin_k, so that the chunks of a cblk run in
+// parallel workgroups (tasks {cblk, R0}).
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
-k_solve_diag_inv( const int32_t *__restrict__ list, const dcblk *__restrict__ cb,
-                  const T *__restrict__ Linv, T *__restrict__ X, int64_t ldx, int nrhs, int trans )
+k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ Linv,
+                    const T *__restrict__ Xin, int64_t ldx, T *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
 {
     extern __shared__ unsigned char smem_raw[];
     T *sb = reinterpret_cast<T *>( smem_raw );   // [n][nrhs]
     __shared__ T sM[64][64 + 1];                 // operator chunk, sM[row][col]; then partial sums
-    T           *sred = &sM[0][0];               // [4][64][PXD_MAXRHS] after the chunk loop
+    T           *sred = &sM[0][0];
 
-    const int32_t k   = list[blockIdx.x];
-    const dcblk   c   = cb[k];
-    const int32_t n   = c.lcolnum - c.fcolnum + 1;
-    const int     tid = threadIdx.x;
-    const T      *LI  = Linv + c.dinvoff;
-    T            *x   = X + c.fcolnum;
-    const int     r   = tid & 63, part = tid >> 6;
+    const row_task tk  = tasks[blockIdx.x];
+    const dcblk    c   = cb[tk.cblk];
+    const int32_t  n   = c.lcolnum - c.fcolnum + 1;
+    const int32_t  R0  = tk.r0;
+    const int      tid = threadIdx.x;
+    const T       *LI  = Linv + c.dinvoff;
+    const T       *x   = Xin + c.fcolnum;
+    const int      r   = tid & 63, part = tid >> 6;
 
     for ( int idx = tid; idx < n * nrhs; idx += PXD_THREADS ) {
-        int rr = idx % n, q = idx / n;
+        const int rr = idx % n, q = idx / n;
         sb[rr * nrhs + q] = x[rr + q * ldx];
     }
     __syncthreads();
 
-    for ( int32_t R0 = 0; R0 < n; R0 += 64 ) {
-        T acc[PXD_MAXRHS];
+    T acc[PXD_MAXRHS];
 #pragma unroll
-        for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
-        // operator M = Linv (lower) or Linv^T (upper): chunk columns C0 <= R0 or >= R0
-        const int32_t cfirst = trans ? R0 : 0, clast = trans ? n : R0 + 1;
-        for ( int32_t C0 = cfirst; C0 < clast; C0 += 64 ) {
-            constexpr int NU = 64 * 64 / PXD_THREADS;
-            T             v[NU];
+    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
+    const int32_t cfirst = trans ? R0 : 0, clast = trans ? n : R0 + 1;
+    for ( int32_t C0 = cfirst; C0 < clast; C0 += 64 ) {
+        constexpr int NU = 64 * 64 / PXD_THREADS;
+        T             v[NU];
 #pragma unroll
-            for ( int u = 0; u < NU; u++ ) {
-                // lane i = tid & 63 walks the contiguous dimension of Linv
-                const int     i = tid & 63, j = ( tid >> 6 ) + 4 * u;
-                const int32_t gi = ( trans ? C0 : R0 ) + i, gj = ( trans ? R0 : C0 ) + j;
-                v[u] = ( gi < n && gj < n && gi >= gj ) ? LI[gi + (int64_t)gj * n] : (T)0;
-            }
-#pragma unroll
-            for ( int u = 0; u < NU; u++ ) {
-                const int i = tid & 63, j = ( tid >> 6 ) + 4 * u;
-                if ( trans ) {
-                    sM[j][i] = v[u];   // M[R0 + j][C0 + i] = Linv[C0 + i][R0 + j]
-                }
-                else {
-                    sM[i][j] = v[u];   // M[R0 + i][C0 + j] = Linv[R0 + i][C0 + j]
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for ( int cc = 16 * part; cc < 16 * part + 16; cc++ ) {
-                if ( C0 + cc < n ) {
-                    const T m = sM[r][cc];
-#pragma unroll
-                    for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                        if ( q < nrhs ) acc[q] += m * sb[( C0 + cc ) * nrhs + q];
-                    }
-                }
-            }
-            __syncthreads();
+        for ( int u = 0; u < NU; u++ ) {
+            const int     i = tid & 63, j = ( tid >> 6 ) + 4 * u;
+            const int32_t gi = ( trans ? C0 : R0 ) + i, gj = ( trans ? R0 : C0 ) + j;
+            v[u] = ( gi < n && gj < n && gi >= gj ) ? LI[gi + (int64_t)gj * n] : (T)0;
         }
 #pragma unroll
-        for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[( part * 64 + r ) * PXD_MAXRHS + q] = acc[q];
-        __syncthreads();
-        if ( part == 0 && R0 + r < n ) {
-            for ( int q = 0; q < nrhs; q++ ) {
-                T v = (T)0;
-#pragma unroll
-                for ( int p2 = 0; p2 < 4; p2++ ) v += sred[( p2 * 64 + r ) * PXD_MAXRHS + q];
-                x[R0 + r + q * ldx] = v;
+        for ( int u = 0; u < NU; u++ ) {
+            const int i = tid & 63, j = ( tid >> 6 ) + 4 * u;
+            if ( trans ) {
+                sM[j][i] = v[u];
+            }
+            else {
+                sM[i][j] = v[u];
             }
         }
         __syncthreads();
+#pragma unroll
+        for ( int cc = 16 * part; cc < 16 * part + 16; cc++ ) {
+            if ( C0 + cc < n ) {
+                const T m = sM[r][cc];
+#pragma unroll
+                for ( int q = 0; q < PXD_MAXRHS; q++ ) {
+                    if ( q < nrhs ) acc[q] += m * sb[( C0 + cc ) * nrhs + q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[( part * 64 + r ) * PXD_MAXRHS + q] = acc[q];
+    __syncthreads();
+    if ( part == 0 && R0 + r < n ) {
+        for ( int q = 0; q < nrhs; q++ ) {
+            T v = (T)0;
+#pragma unroll
+            for ( int p2 = 0; p2 < 4; p2++ ) v += sred[( p2 * 64 + r ) * PXD_MAXRHS + q];
+            Yout[c.fcolnum + R0 + r + q * ldy] = v;
+        }
     }
 }
 
@@ -1223,8 +1297,8 @@ k_solve_diag_inv( const int32_t *__restrict__ list, const dcblk *__restrict__ cb
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-                     const dblok *__restrict__ bk, const T *__restrict__ L, T *__restrict__ X,
-                     int64_t ldx, int nrhs )
+                     const dblok *__restrict__ bk, const T *__restrict__ L, const T *X,
+                     int64_t ldx, T *Yt, int64_t ldt, int nrhs )
 {
     __shared__ T       sA[64][64 + 1];           // sA[col][row]
     __shared__ T       sx[64][PXD_MAXRHS];
@@ -1287,7 +1361,7 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
         if ( part == 0 && c0 + col < n ) {
             for ( int q = 0; q < nrhs; q++ ) {
                 T v = sred[0][col][q] + sred[1][col][q] + sred[2][col][q] + sred[3][col][q];
-                px_atomic_add( X + c.fcolnum + c0 + col + q * ldx, -v );
+                px_atomic_add( Yt + c.fcolnum + c0 + col + q * ldt, -v );
             }
         }
         __syncthreads();

@@ -89,8 +89,8 @@ k_solve_diag( const int32_t *__restrict__ list, const dcblk *__restrict__ cb,
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-                 const dblok *__restrict__ bk, const T *__restrict__ L, T *__restrict__ X,
-                 int64_t ldx, int nrhs )
+                 const dblok *__restrict__ bk, const T *__restrict__ L, T *X,
+                 int64_t ldx, int nrhs, const T *Xs, int64_t lds )
 {
     extern __shared__ unsigned char smem_raw[];
     T *sx = reinterpret_cast<T *>( smem_raw );    // [n][nrhs]
@@ -103,9 +103,10 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
     const int      tid = threadIdx.x;
     const int      r   = tid % 64, part = tid / 64;
 
+    // x_k, solved: in Xs (the diagonal solve's output) or in X itself
     for ( int idx = tid; idx < n * nrhs; idx += PXD_THREADS ) {
         int rr = idx % n, q = idx / n;
-        sx[rr * nrhs + q] = X[c.fcolnum + rr + q * ldx];
+        sx[rr * nrhs + q] = Xs[c.fcolnum + rr + q * lds];
     }
     __syncthreads();
 
@@ -404,4 +405,19 @@ k_convert( int64_t n, const TI *__restrict__ in, TO *__restrict__ out )
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if ( i < n ) out[i] = px_from_any<TO>( in[i] );
+}
+
+// LDL^T solve, between the sweeps: y_i <- y_i / D_i, D read from the
+// diagonal of the cblk's diagonal tile (solve_cblk_zdiag, kernels/solve_ztrsmsp.c:629-679)
+template <typename T>
+__global__ void
+k_diag_scale( int64_t n, const int32_t *__restrict__ col2cblk, const dcblk *__restrict__ cb,
+              const T *__restrict__ L, T *__restrict__ Y, int64_t ldy, int nrhs )
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n ) return;
+    const dcblk   c  = cb[col2cblk[i]];
+    const int32_t nk = c.lcolnum - c.fcolnum + 1;
+    const T       d  = L[c.coefoff + (int64_t)( i - c.fcolnum ) * ( nk + 1 )];
+    for ( int q = 0; q < nrhs; q++ ) Y[i + q * ldy] = px_div( Y[i + q * ldy], d );
 }

@@ -683,7 +683,15 @@ pastix_subtask_sopalin( pastix_data_t *pd )
     pd->solvmatr->diagthreshold = threshold;
 
     t0 = px_wtime();
-    rc = pxd_factorize( pd->dev, (int)iparm[IPARM_FACTORIZATION], fflt, threshold, &npiv );
+    {
+        /* for real matrices LL^T is LL^H (POTRF) and LDL^H is LDL^T (SYTRF),
+         * as in the reference's sopalinFacto table (sopalin/pastix_task_sopalin.c:51-58) */
+        int facto = (int)iparm[IPARM_FACTORIZATION];
+        int cplx  = ( fflt == SpmComplex32 || fflt == SpmComplex64 );
+        if ( !cplx && facto == PastixFactLLT ) facto = PastixFactLLH;
+        if ( !cplx && facto == PastixFactLDLH ) facto = PastixFactLDLT;
+        rc = pxd_factorize( pd->dev, facto, fflt, threshold, &npiv );
+    }
     if ( rc != PASTIX_SUCCESS ) {
         return rc;
     }

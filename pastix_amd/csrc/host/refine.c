@@ -320,6 +320,228 @@ sr_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     return rc;
 }
 
+/* per-column dot products (x, y) = sum conj(x_i) y_i, on the host */
+static void
+dots( ref_ctx_t *r, const void *x, const void *y, zc *out )
+{
+    pxd_vec_dot( r->dev, r->n, r->nrhs, x, y, r->dscal );
+    get_scalars( r, out, r->dscal, r->nrhs );
+}
+
+/* y += sign * a[c] * x for every column c */
+static void
+axpy_cols( ref_ctx_t *r, double sign, const zc *a, const void *x, void *y )
+{
+    put_scalars( r, a, r->dscal, r->nrhs );
+    pxd_vec_axpy( r->dev, r->n, r->nrhs, sign, r->dscal, x, y );
+}
+
+/*
+ * BiCGStab preconditioned by the factorization, following z_bicgstab_smp
+ * (refinement/z_refine_bicgstab.c:48-228): y = M^{-1} p, v = A y, alpha =
+ * (r, r2) / (v, r2), s = r - alpha v, z = M^{-1} s, t = A z, omega =
+ * (M^{-1} t, z) / (M^{-1} t, M^{-1} t), x += alpha y + omega z, r = s - omega
+ * t, beta = (r, r2) / (r_old, r2) * alpha / omega, p = r + beta (p - omega v);
+ * stop when ||r||_2 / ||b||_2 <= eps.  The scalars are complex for complex
+ * matrices (the reference keeps alpha and beta in double variables).  All
+ * columns run together, each with its own scalars and stopping point.
+ */
+static int
+bicgstab_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t itermax,
+              pastix_int_t *iters_out )
+{
+    pastix_int_t n = r->n, nr = r->nrhs, iters = 0;
+    size_t       blk = (size_t)n * nr * r->es;
+    void        *R, *R2, *P, *Yv, *V, *S, *Z, *Tv, *G2;
+    void       **vec[] = { &R, &R2, &P, &Yv, &V, &S, &Z, &Tv, &G2 };
+    double      *normb = px_calloc( nr, sizeof( double ) ), *target = px_calloc( nr, sizeof( double ) );
+    double      *rn = px_calloc( nr, sizeof( double ) );
+    int         *conv = px_calloc( nr, sizeof( int ) );
+    zc          *alpha = px_calloc( nr, sizeof( zc ) ), *beta = px_calloc( nr, sizeof( zc ) );
+    zc          *om = px_calloc( nr, sizeof( zc ) ), *t1 = px_calloc( nr, sizeof( zc ) );
+    zc          *t2 = px_calloc( nr, sizeof( zc ) );
+    double       normA = 0.;
+    int          rc = PASTIX_SUCCESS;
+
+    for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
+        if ( pxd_malloc( r->dev, blk, vec[i] ) ) {
+            return PASTIX_ERR_OUTOFMEMORY;
+        }
+    }
+    pxd_matrix_norm_inf( r->dev, &normA );
+    norms2( r, db, normb );
+    for ( pastix_int_t c = 0; c < nr; c++ ) {
+        if ( normb[c] == 0. ) normb[c] = 1.;
+    }
+    /* r = b - A x, r2 = r, p = r */
+    pxd_vec_copy( r->dev, n, nr, db, R );
+    pxd_spmv( r->dev, -1., dx, 1., R, nr, n );
+    pxd_vec_copy( r->dev, n, nr, R, R2 );
+    pxd_vec_copy( r->dev, n, nr, R, P );
+    int nconv = stop_targets( r, db, dx, R, eps, berr, normA, normb, target, conv );
+
+    while ( nconv < nr && iters < itermax ) {
+        iters++;
+        /* y = M^{-1} p, v = A y */
+        pxd_vec_copy( r->dev, n, nr, P, Yv );
+        if ( ( rc = pxd_solve( r->dev, nr, Yv, n ) ) != PASTIX_SUCCESS ) break;
+        pxd_spmv( r->dev, 1., Yv, 0., V, nr, n );
+        /* alpha = (r, r2) / (v, r2) */
+        dots( r, V, R2, t1 );
+        dots( r, R, R2, beta );
+        for ( pastix_int_t c = 0; c < nr; c++ ) {
+            alpha[c] = ( conv[c] || t1[c] == 0. ) ? 0. : beta[c] / t1[c];
+        }
+        /* s = r - alpha v;  z = M^{-1} s;  t = A z;  g2 = M^{-1} t */
+        pxd_vec_copy( r->dev, n, nr, R, S );
+        axpy_cols( r, -1., alpha, V, S );
+        pxd_vec_copy( r->dev, n, nr, S, Z );
+        if ( ( rc = pxd_solve( r->dev, nr, Z, n ) ) != PASTIX_SUCCESS ) break;
+        pxd_spmv( r->dev, 1., Z, 0., Tv, nr, n );
+        pxd_vec_copy( r->dev, n, nr, Tv, G2 );
+        if ( ( rc = pxd_solve( r->dev, nr, G2, n ) ) != PASTIX_SUCCESS ) break;
+        /* omega = (g2, z) / (g2, g2) */
+        dots( r, G2, Z, t1 );
+        dots( r, G2, G2, t2 );
+        for ( pastix_int_t c = 0; c < nr; c++ ) {
+            om[c] = ( conv[c] || t2[c] == 0. ) ? 0. : t1[c] / t2[c];
+        }
+        /* x += alpha y + omega z;  r = s - omega t (frozen columns keep r) */
+        axpy_cols( r, 1., alpha, Yv, dx );
+        axpy_cols( r, 1., om, Z, dx );
+        for ( pastix_int_t c = 0; c < nr; c++ ) t1[c] = conv[c] ? 0. : 1.;
+        axpy_cols( r, -1., t1, R, R );        /* r <- 0 on the active columns */
+        axpy_cols( r, 1., t1, S, R );         /* r <- s */
+        axpy_cols( r, -1., om, Tv, R );       /* r <- s - omega t */
+        /* beta = (r, r2) / (r_old, r2) * alpha / omega */
+        dots( r, R, R2, t2 );
+        for ( pastix_int_t c = 0; c < nr; c++ ) {
+            zc b = 0.;
+            if ( !conv[c] && beta[c] != 0. && om[c] != 0. ) {
+                b = t2[c] / beta[c] * ( alpha[c] / om[c] );
+            }
+            beta[c] = b;
+        }
+        /* p = r + beta (p - omega v) */
+        axpy_cols( r, -1., om, V, P );
+        for ( pastix_int_t c = 0; c < nr; c++ ) t1[c] = conv[c] ? 1. : beta[c];
+        put_scalars( r, t1, r->dscal, nr );
+        pxd_vec_scal( r->dev, n, nr, r->dscal, 0, P );
+        for ( pastix_int_t c = 0; c < nr; c++ ) t1[c] = conv[c] ? 0. : 1.;
+        axpy_cols( r, 1., t1, R, P );
+        /* stop on ||r|| (recursive residual) or on the backward error of the
+         * true residual; a breakdown ends the column */
+        if ( berr > 0. ) {
+            pxd_vec_copy( r->dev, n, nr, db, S );
+            pxd_spmv( r->dev, -1., dx, 1., S, nr, n );
+            nconv = stop_targets( r, db, dx, S, eps, berr, normA, normb, target, conv );
+        }
+        else {
+            norms2( r, R, rn );
+            nconv = 0;
+            for ( pastix_int_t c = 0; c < nr; c++ ) {
+                if ( rn[c] <= eps * normb[c] || alpha[c] == 0. || om[c] == 0. ) conv[c] = 1;
+                nconv += conv[c];
+            }
+        }
+    }
+    pxd_sync( r->dev );
+    *iters_out = iters;
+    for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
+        pxd_free( r->dev, *vec[i] );
+    }
+    free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( beta );
+    free( om ); free( t1 ); free( t2 );
+    return rc;
+}
+
+/*
+ * Preconditioned conjugate gradient following z_grad_smp
+ * (refinement/z_refine_grad.c:40-200), for symmetric (Hermitian) positive
+ * definite matrices: z = M^{-1} r, p = z; then alpha = (r, z) / (Ap, p),
+ * x += alpha p, r -= alpha Ap, z = M^{-1} r, beta = (r_new, z_new) / (r, z),
+ * p = z + beta p; stop when ||r||_2 / ||b||_2 <= eps.  Columns run together.
+ */
+static int
+cg_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t itermax,
+        pastix_int_t *iters_out )
+{
+    pastix_int_t n = r->n, nr = r->nrhs, iters = 0;
+    size_t       blk = (size_t)n * nr * r->es;
+    void        *R, *P, *Z, *Q;
+    void       **vec[] = { &R, &P, &Z, &Q };
+    double      *normb = px_calloc( nr, sizeof( double ) ), *target = px_calloc( nr, sizeof( double ) );
+    double      *rn = px_calloc( nr, sizeof( double ) );
+    int         *conv = px_calloc( nr, sizeof( int ) );
+    zc          *alpha = px_calloc( nr, sizeof( zc ) ), *rz = px_calloc( nr, sizeof( zc ) );
+    zc          *t1 = px_calloc( nr, sizeof( zc ) );
+    double       normA = 0.;
+    int          rc = PASTIX_SUCCESS;
+
+    for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
+        if ( pxd_malloc( r->dev, blk, vec[i] ) ) {
+            return PASTIX_ERR_OUTOFMEMORY;
+        }
+    }
+    pxd_matrix_norm_inf( r->dev, &normA );
+    norms2( r, db, normb );
+    for ( pastix_int_t c = 0; c < nr; c++ ) {
+        if ( normb[c] == 0. ) normb[c] = 1.;
+    }
+    pxd_vec_copy( r->dev, n, nr, db, R );
+    pxd_spmv( r->dev, -1., dx, 1., R, nr, n );
+    int nconv = stop_targets( r, db, dx, R, eps, berr, normA, normb, target, conv );
+    pxd_vec_copy( r->dev, n, nr, R, Z );
+    if ( nconv < nr && ( rc = pxd_solve( r->dev, nr, Z, n ) ) == PASTIX_SUCCESS ) {
+        pxd_vec_copy( r->dev, n, nr, Z, P );
+        dots( r, R, Z, rz );
+    }
+    while ( rc == PASTIX_SUCCESS && nconv < nr && iters < itermax ) {
+        iters++;
+        /* q = A p, alpha = (r, z) / (q, p) */
+        pxd_spmv( r->dev, 1., P, 0., Q, nr, n );
+        dots( r, Q, P, t1 );
+        for ( pastix_int_t c = 0; c < nr; c++ ) {
+            alpha[c] = ( conv[c] || t1[c] == 0. ) ? 0. : rz[c] / t1[c];
+        }
+        axpy_cols( r, 1., alpha, P, dx );
+        axpy_cols( r, -1., alpha, Q, R );
+        pxd_vec_copy( r->dev, n, nr, R, Z );
+        if ( ( rc = pxd_solve( r->dev, nr, Z, n ) ) != PASTIX_SUCCESS ) break;
+        dots( r, R, Z, t1 );
+        for ( pastix_int_t c = 0; c < nr; c++ ) {
+            zc beta = ( conv[c] || rz[c] == 0. ) ? 1. : t1[c] / rz[c];
+            rz[c]   = conv[c] ? rz[c] : t1[c];
+            t1[c]   = beta;
+        }
+        /* p = z + beta p (frozen columns keep p) */
+        put_scalars( r, t1, r->dscal, nr );
+        pxd_vec_scal( r->dev, n, nr, r->dscal, 0, P );
+        for ( pastix_int_t c = 0; c < nr; c++ ) t1[c] = conv[c] ? 0. : 1.;
+        axpy_cols( r, 1., t1, Z, P );
+        if ( berr > 0. ) {
+            pxd_vec_copy( r->dev, n, nr, db, Q );
+            pxd_spmv( r->dev, -1., dx, 1., Q, nr, n );
+            nconv = stop_targets( r, db, dx, Q, eps, berr, normA, normb, target, conv );
+        }
+        else {
+            norms2( r, R, rn );
+            nconv = 0;
+            for ( pastix_int_t c = 0; c < nr; c++ ) {
+                if ( rn[c] <= eps * normb[c] || alpha[c] == 0. ) conv[c] = 1;
+                nconv += conv[c];
+            }
+        }
+    }
+    pxd_sync( r->dev );
+    *iters_out = iters;
+    for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
+        pxd_free( r->dev, *vec[i] );
+    }
+    free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( rz ); free( t1 );
+    return rc;
+}
+
 int
 px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_int_t *iters )
 {
@@ -353,6 +575,12 @@ px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_
             switch ( pd->iparm[IPARM_REFINEMENT] ) {
             case PastixRefineSR:
                 rc = sr_run( &r, b, x, eps, pd->berr_target, pd->iparm[IPARM_ITERMAX], &it );
+                break;
+            case PastixRefineCG:
+                rc = cg_run( &r, b, x, eps, pd->berr_target, pd->iparm[IPARM_ITERMAX], &it );
+                break;
+            case PastixRefineBiCGSTAB:
+                rc = bicgstab_run( &r, b, x, eps, pd->berr_target, pd->iparm[IPARM_ITERMAX], &it );
                 break;
             case PastixRefineGMRES:
             default:

@@ -36,6 +36,11 @@ def lib():
         _lib.orc_factorize_lu.restype = ctypes.c_int
         _lib.orc_factorize_lu.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _i64, _p, _p, _p, _p, _p, _p,
                                           ctypes.c_double, _p, _p, _p]
+        _lib.orc_factorize_ldlt.restype = ctypes.c_int
+        _lib.orc_factorize_ldlt.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _i64, _p, _p, _p,
+                                            ctypes.c_double, _p, _p]
+        _lib.orc_solve_ldlt.restype = ctypes.c_int
+        _lib.orc_solve_ldlt.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _i64, _p, _i64]
         _lib.orc_solve_lu.restype = ctypes.c_int
         _lib.orc_solve_lu.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p, _i64]
     return _lib
@@ -116,6 +121,31 @@ class OracleLU:
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         lib().orc_solve_lu(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
                            _ptr(self.s["browtab"]), _ptr(self.L), _ptr(self.U), nrhs, _ptr(x), self.n)
+        return x
+
+
+class OracleLDLT:
+    """LDL^T factor with static pivoting computed by the oracle: the diagonal
+    tiles hold L (strictly below), D and D L^T; the other tiles hold L."""
+
+    def __init__(self, solver, bcsc, thresh, dtype=np.float64):
+        self.s = solver
+        self.cp, self.rt, self.val = bcsc
+        self.dtype = dtype
+        self.flt = 3 if dtype == np.float64 else 2
+        self.n = solver["nodenbr"]
+        self.L = np.zeros(solver["coefnbr"], dtype=dtype)
+        npiv = np.zeros(1, dtype=np.int64)
+        lib().orc_factorize_ldlt(self.flt, self.n, solver["cblknbr"], _ptr(solver["cblk"]), _ptr(solver["blok"]),
+                                 solver["coefnbr"], _ptr(self.cp), _ptr(self.rt), _ptr(self.val), thresh,
+                                 _ptr(self.L), _ptr(npiv))
+        self.npiv = int(npiv[0])
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        lib().orc_solve_ldlt(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
+                             _ptr(self.s["browtab"]), _ptr(self.L), nrhs, _ptr(x), self.n)
         return x
 
 

@@ -1,0 +1,56 @@
+"""LDL^T with static pivoting on the MI355X engine (SYTRF path) against the
+CPU oracle on a symmetric indefinite matrix (the 7-point Laplacian shifted by
+1.7): every tile including the D L^T upper part of the diagonal tiles
+(cpucblk_zdiff criterion, reference kernels/cpucblk_zdiff.c:93-100), the
+solve, and FP32 LDL^T + FP64 GMRES."""
+import numpy as np
+import pytest
+
+import pastix_amd as px
+import oracle_ref as orc
+from helpers import analyzed, cblk_diff, make_problem, static_pivot_threshold
+
+pytestmark = pytest.mark.gpu
+
+
+def _ldlt_case(nx, mixed=0):
+    spm = make_problem("lap7", nx, diag=4.3)
+    pd, iparm, dparm = analyzed(spm, nx, facto=px.PastixFactLDLT, mixed=mixed)
+    px.task_numfact(pd, spm)
+    return spm, pd, iparm, dparm
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx", [5, 9, 14])
+def test_sytrf_tiles_match_oracle(nx, dtype):
+    spm, pd, iparm, dparm = _ldlt_case(nx, mixed=int(dtype == np.float32))
+    s = px.solver_export(pd)
+    L = px.get_coeftab(pd, s["coefnbr"], dtype)
+    F = orc.OracleLDLT(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype)
+    assert iparm[px.IPARM_STATIC_PIVOTING] == F.npiv
+    assert cblk_diff(s, L, F.L, dtype, full_diag=True) <= 10.0
+    px.finalize(pd)
+
+
+@pytest.mark.parametrize("nx,nrhs", [(8, 1), (13, 3)])
+def test_ldlt_solve_backward_error(nx, nrhs):
+    spm, pd, iparm, dparm = _ldlt_case(nx)
+    x0, b = spm.gen_rhs(nrhs, seed=0)
+    x = np.array(b, order="F", copy=True)
+    px.task_solve(pd, spm, x)
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 1e-13
+    px.finalize(pd)
+
+
+def test_ldlt_mixed_gmres_reaches_1e12():
+    spm, pd, iparm, dparm = _ldlt_case(12, mixed=1)
+    x0, b = spm.gen_rhs(2, seed=6)
+    x = np.array(b, order="F", copy=True)
+    px.task_solve(pd, spm, x)
+    px.set_option(pd, "berr_target", 1e-12)
+    px.task_refine(pd, spm, b, x)
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 1e-12
+    assert 0 < iparm[px.IPARM_NBITER] < 30
+    px.finalize(pd)

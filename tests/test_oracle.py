@@ -198,3 +198,62 @@ def test_lu_static_pivot_known_answers():
     # a pivot that is exactly tau is kept, a positive tiny one becomes +tau
     F2 = orc.OracleLU(_one_cblk(2), _csc(np.array([[1e-30, 0.0], [0.0, 1.0]])), tau)
     assert F2.npiv == 1 and F2.L[0] == tau
+
+
+def _dense_ldlt_nopiv(A):
+    """LDL^T without pivoting: returns (unit lower L, d)."""
+    A = np.array(A, dtype=np.float64, copy=True)
+    n = A.shape[0]
+    Lm = np.eye(n)
+    d = np.zeros(n)
+    for k in range(n):
+        d[k] = A[k, k]
+        Lm[k + 1:, k] = A[k + 1:, k] / d[k]
+        A[k + 1:, k + 1:] -= d[k] * np.outer(Lm[k + 1:, k], Lm[k + 1:, k])
+    return Lm, d
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (6, 5, 4), (8, 8, 8)])
+def test_oracle_ldlt_equals_dense(dims):
+    # shifted Laplacian: symmetric indefinite (the smallest eigenvalue of the
+    # 7-point stencil is below 1.7 on these grids) and far from singular
+    spm = px.Spm.laplacian7(*dims, 4.3, 1.0)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLDLT)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd)
+    n = s["nodenbr"]
+    A = _dense(*bc, n)
+    ev = np.linalg.eigvalsh(A)
+    assert ev.min() < 0 < ev.max() and np.abs(ev).min() > 0.1
+    F = orc.OracleLDLT(s, bc, static_pivot_threshold(dparm, np.float64))
+    assert F.npiv == 0
+    Ld, d = _dense_ldlt_nopiv(A)
+    Lo = orc.dense_from_factor(s, F.L)
+    assert np.abs(np.tril(Lo, -1) - np.tril(Ld, -1)).max() <= 1e-11
+    assert np.abs(np.diag(Lo) - d).max() <= 1e-12 * np.abs(d).max()
+    # the upper part of the diagonal tiles holds D L^T
+    cb = s["cblk"].reshape(-1, 6)
+    for k in range(s["cblknbr"]):
+        nk = cb[k, 1] - cb[k, 0] + 1
+        T = F.L[cb[k, 4]:cb[k, 4] + nk * nk].reshape(nk, nk).T
+        assert np.allclose(np.triu(T, 1), np.triu((np.tril(T, -1) * np.diag(T)[None, :]).T, 1), atol=1e-11)
+    x0, b = spm.gen_rhs(2, seed=2)
+    xp = F.solve(permute_rhs(pd, b))
+    x = xp[px.order_export(pd)[0]]
+    assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max() * np.linalg.cond(A)
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 1e-13
+    px.finalize(pd)
+
+
+def test_ldlt_static_pivot_known_answers():
+    # |a_00| < tau -> sign(a_00) tau, no square root; D L^T above the diagonal
+    tau = 1e-8
+    A = np.array([[-1e-20, 2.0, 0.0], [2.0, 3.0, 1.0], [0.0, 1.0, 4.0]])
+    F = orc.OracleLDLT(_one_cblk(3), _csc(A), tau)
+    T = F.L.reshape(3, 3).T
+    assert F.npiv == 1
+    assert T[0, 0] == -tau
+    assert T[1, 0] == pytest.approx(2.0 / -tau)
+    assert T[0, 1] == pytest.approx(2.0)                       # (D L^T)[0,1] = d0 * l10
+    assert T[1, 1] == pytest.approx(3.0 - (2.0 / -tau) ** 2 * -tau)
