@@ -30,6 +30,7 @@
  * coefoff, brownum}, blok[5*bloknbr] = {frownum, lrownum, lcblknm, fcblknm,
  * coefind}, browtab[bloknbr-cblknbr].
  */
+#include <complex.h>
 #include <math.h>
 #include <pthread.h>
 #include <sched.h>
@@ -90,7 +91,7 @@ orc_blas( void )
     {                                                                                               \
         for ( int64_t k = 0; k < n; k++ ) {                                                         \
             T *akk = A + k * ( lda + 1 );                                                           \
-            if ( FABS( *akk ) < (T)tau ) {                                                          \
+            if ( FABS( *akk ) < ( __typeof__( FABS( *akk ) ) )tau ) {                               \
                 *akk = (T)tau;                                                                      \
                 ( *npiv )++;                                                                        \
             }                                                                                       \
@@ -451,14 +452,59 @@ orc_factorize( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_
 }
 
 /* ------------------------------------------------------------------------ */
+/* Complex arithmetic for the symmetric (non Hermitian) LDL^T and the LU:    */
+/* plain loops, C = alpha A op(B) + beta C with op = B or B^T (no conjugate) */
+
+#define ORC_GEMM_LOOP( T, NAME )                                                                     \
+    static void NAME( int tb, int64_t m, int64_t n, int64_t k, T al, const T *A, int64_t lda,       \
+                      const T *B, int64_t ldb, T be, T *C, int64_t ldc )                             \
+    {                                                                                               \
+        for ( int64_t j = 0; j < n; j++ ) {                                                         \
+            for ( int64_t i = 0; i < m; i++ ) {                                                     \
+                T acc = 0;                                                                          \
+                for ( int64_t l = 0; l < k; l++ ) {                                                 \
+                    T b = ( tb == ORC_TRANS ) ? B[j + l * ldb] : B[l + j * ldb];                    \
+                    acc += A[i + l * lda] * b;                                                      \
+                }                                                                                   \
+                C[i + j * ldc] = al * acc + be * C[i + j * ldc];                                    \
+            }                                                                                       \
+        }                                                                                           \
+    }
+ORC_GEMM_LOOP( double complex, zgemm_w )
+ORC_GEMM_LOOP( float complex, cgemm_w )
+
+/* lower triangle of the matrix (complex input) into the L tiles */
+#define ORC_FILLC( T, SFX )                                                                          \
+    static void fill_##SFX( int64_t n, const int64_t *cblk, const int64_t *blok,                    \
+                            const int64_t *colptr, const int64_t *rowtab, const double complex *val, \
+                            const int64_t *col2cblk, T *L )                                         \
+    {                                                                                               \
+        for ( int64_t j = 0; j < n; j++ ) {                                                         \
+            int64_t k = col2cblk[j];                                                                \
+            int64_t b = CB( k, C_FBLOK ), lb = CB( k + 1, C_FBLOK );                                \
+            for ( int64_t e = colptr[j]; e < colptr[j + 1]; e++ ) {                                 \
+                int64_t i = rowtab[e];                                                              \
+                if ( i < j ) continue;                                                              \
+                while ( b < lb && ( BK( b, B_LROW ) < i || BK( b, B_FROW ) > i ) ) b++;             \
+                if ( b >= lb ) continue;                                                            \
+                int64_t rows = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                               \
+                L[CB( k, C_COEFOFF ) + BK( b, B_COEFIND ) + ( i - BK( b, B_FROW ) ) +               \
+                  ( j - CB( k, C_FCOL ) ) * rows] = (T)val[e];                                      \
+            }                                                                                       \
+        }                                                                                           \
+    }
+ORC_FILLC( double complex, z )
+ORC_FILLC( float complex, c )
+
+/* ------------------------------------------------------------------------ */
 /* LU with static pivoting (kernels/core_zgetrfsp.c), sequential in cblk     */
 /* order (every source of cblk k precedes k).  The U side holds U^T in the   */
 /* layout of L (ucoeftab).                                                   */
 
-#define ORC_LU_TYPED( T, SFX, GEMM, FABS )                                                          \
+#define ORC_LU_TYPED( T, SFX, GEMM, FABS, VT, NEG )                                                \
     /* U tiles: A[j][i] at (i, j) for i > j (the strict upper part, transposed) */                  \
     static void fillu_##SFX( int64_t n, const int64_t *cblk, const int64_t *blok,                   \
-                             const int64_t *tcolptr, const int64_t *trowtab, const double *tval,    \
+                             const int64_t *tcolptr, const int64_t *trowtab, const VT *tval,        \
                              const int64_t *col2cblk, T *U )                                        \
     {                                                                                               \
         for ( int64_t j = 0; j < n; j++ ) {                                                         \
@@ -482,8 +528,8 @@ orc_factorize( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_
         int64_t mn = ( m < n ) ? m : n;                                                             \
         for ( int64_t k = 0; k < mn; k++ ) {                                                        \
             T *akk = A + k * ( lda + 1 );                                                           \
-            if ( FABS( *akk ) < (T)tau ) {                                                          \
-                *akk = ( *akk < (T)0 ) ? (T)-tau : (T)tau;                                          \
+            if ( FABS( *akk ) < ( __typeof__( FABS( *akk ) ) )tau ) {                               \
+                *akk = NEG( *akk ) ? (T)-tau : (T)tau;                                              \
                 ( *npiv )++;                                                                        \
             }                                                                                       \
             T a = (T)1 / *akk;                                                                      \
@@ -631,18 +677,28 @@ orc_factorize( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_
         }                                                                                           \
     }
 
-ORC_LU_TYPED( double, d, dgemm_w, fabs )
-ORC_LU_TYPED( float, s, sgemm_w, fabsf )
+#define ORC_NEG_R( x ) ( ( x ) < 0 )
+#define ORC_NEG_C( x ) ( creal( x ) < 0 )
+ORC_LU_TYPED( double, d, dgemm_w, fabs, double, ORC_NEG_R )
+ORC_LU_TYPED( float, s, sgemm_w, fabsf, double, ORC_NEG_R )
+ORC_LU_TYPED( double complex, z, zgemm_w, cabs, double complex, ORC_NEG_C )
+ORC_LU_TYPED( float complex, c, cgemm_w, cabsf, double complex, ORC_NEG_C )
 
 /*
  * Fill + LU factorization with static pivoting.  L and U (coefnbr entries
  * each, factor type flt) are overwritten; tcolptr/trowtab/tval is the CSC of
  * A^T (for a symmetric matrix, the CSC itself).
  */
+static size_t
+orc_es( int flt )
+{
+    return ( flt == 3 ) ? 8 : ( flt == 2 ) ? 4 : ( flt == 5 ) ? 16 : 8;
+}
+
 int
 orc_factorize_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
-                  int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const double *val,
-                  const int64_t *tcolptr, const int64_t *trowtab, const double *tval, double tau,
+                  int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const void *val,
+                  const int64_t *tcolptr, const int64_t *trowtab, const void *tval, double tau,
                   void *L, void *U, int64_t *npiv )
 {
     int64_t *col2cblk = malloc( ( n > 0 ? n : 1 ) * sizeof( int64_t ) );
@@ -651,20 +707,19 @@ orc_factorize_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int
     for ( int64_t k = 0; k < nc; k++ ) {
         for ( int64_t j = CB( k, C_FCOL ); j <= CB( k, C_LCOL ); j++ ) col2cblk[j] = k;
     }
-    if ( flt == 3 ) {
-        memset( L, 0, coefnbr * sizeof( double ) );
-        memset( U, 0, coefnbr * sizeof( double ) );
-        fill_d( n, cblk, blok, colptr, rowtab, val, col2cblk, (double *)L );
-        fillu_d( n, cblk, blok, tcolptr, trowtab, tval, col2cblk, (double *)U );
-        for ( int64_t k = 0; k < nc; k++ ) lu_cblk_d( cblk, blok, k, (double *)L, (double *)U, &piv, tau );
+    memset( L, 0, coefnbr * orc_es( flt ) );
+    memset( U, 0, coefnbr * orc_es( flt ) );
+#define ORC_LU_RUN( T, SFX, VT )                                                                     \
+    fill_##SFX( n, cblk, blok, colptr, rowtab, (const VT *)val, col2cblk, (T *)L );                 \
+    fillu_##SFX( n, cblk, blok, tcolptr, trowtab, (const VT *)tval, col2cblk, (T *)U );             \
+    for ( int64_t k = 0; k < nc; k++ ) lu_cblk_##SFX( cblk, blok, k, (T *)L, (T *)U, &piv, tau );
+    switch ( flt ) {
+    case 3: ORC_LU_RUN( double, d, double ) break;
+    case 2: ORC_LU_RUN( float, s, double ) break;
+    case 5: ORC_LU_RUN( double complex, z, double complex ) break;
+    default: ORC_LU_RUN( float complex, c, double complex ) break;
     }
-    else {
-        memset( L, 0, coefnbr * sizeof( float ) );
-        memset( U, 0, coefnbr * sizeof( float ) );
-        fill_s( n, cblk, blok, colptr, rowtab, val, col2cblk, (float *)L );
-        fillu_s( n, cblk, blok, tcolptr, trowtab, tval, col2cblk, (float *)U );
-        for ( int64_t k = 0; k < nc; k++ ) lu_cblk_s( cblk, blok, k, (float *)L, (float *)U, &piv, tau );
-    }
+#undef ORC_LU_RUN
     if ( npiv ) *npiv = piv;
     free( col2cblk );
     return 0;
@@ -673,19 +728,34 @@ orc_factorize_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int
 /* x <- A^{-1} x with an LU factor (double x, factor in flt). */
 int
 orc_solve_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
-              const int64_t *browtab, const void *L, const void *U, int64_t nrhs, double *x, int64_t ldx )
+              const int64_t *browtab, const void *L, const void *U, int64_t nrhs, void *x, int64_t ldx )
 {
     orc_blas_init();
     if ( flt == 3 ) {
-        lusolve_d( nc, cblk, blok, browtab, (const double *)L, (const double *)U, nrhs, x, ldx );
+        lusolve_d( nc, cblk, blok, browtab, (const double *)L, (const double *)U, nrhs, (double *)x, ldx );
     }
-    else {
-        float *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+    else if ( flt == 5 ) {
+        lusolve_z( nc, cblk, blok, browtab, (const double complex *)L, (const double complex *)U, nrhs,
+                   (double complex *)x, ldx );
+    }
+    else if ( flt == 2 ) {
+        float  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        double *xd = (double *)x;
         for ( int64_t q = 0; q < nrhs; q++ )
-            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)xd[i + q * ldx];
         lusolve_s( nc, cblk, blok, browtab, (const float *)L, (const float *)U, nrhs, xs, n );
         for ( int64_t q = 0; q < nrhs; q++ )
-            for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
+    else {
+        float complex  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float complex ) );
+        double complex *xd = (double complex *)x;
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float complex)xd[i + q * ldx];
+        lusolve_c( nc, cblk, blok, browtab, (const float complex *)L, (const float complex *)U, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double complex)xs[i + q * n];
         free( xs );
     }
     return 0;
@@ -696,14 +766,14 @@ orc_solve_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t
 /* sequential in cblk order.  The diagonal tile holds L (strictly below), D  */
 /* and D L^T (above); the off-diagonal tiles hold L.                         */
 
-#define ORC_LDLT_TYPED( T, SFX, GEMM, FABS )                                                        \
+#define ORC_LDLT_TYPED( T, SFX, GEMM, FABS, NEG )                                                   \
     /* unblocked (core_zsytf2sp): copy the column to the row, scale, update */                      \
     static void sytf2_##SFX( int64_t n, T *A, int64_t lda, int64_t *npiv, double tau )              \
     {                                                                                               \
         for ( int64_t k = 0; k < n; k++ ) {                                                         \
             T *akk = A + k * ( lda + 1 );                                                           \
-            if ( FABS( *akk ) < (T)tau ) {                                                          \
-                *akk = ( *akk < (T)0 ) ? (T)-tau : (T)tau;                                          \
+            if ( FABS( *akk ) < ( __typeof__( FABS( *akk ) ) )tau ) {                               \
+                *akk = NEG( *akk ) ? (T)-tau : (T)tau;                                              \
                 ( *npiv )++;                                                                        \
             }                                                                                       \
             T a = (T)1 / *akk;                                                                      \
@@ -820,32 +890,35 @@ orc_solve_lu( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t
         }                                                                                           \
     }
 
-ORC_LDLT_TYPED( double, d, dgemm_w, fabs )
-ORC_LDLT_TYPED( float, s, sgemm_w, fabsf )
+ORC_LDLT_TYPED( double, d, dgemm_w, fabs, ORC_NEG_R )
+ORC_LDLT_TYPED( float, s, sgemm_w, fabsf, ORC_NEG_R )
+ORC_LDLT_TYPED( double complex, z, zgemm_w, cabs, ORC_NEG_C )
+ORC_LDLT_TYPED( float complex, c, cgemm_w, cabsf, ORC_NEG_C )
 
 /* Fill + LDL^T factorization with static pivoting (flt 3 double, 2 float). */
 int
 orc_factorize_ldlt( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
-                    int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const double *val,
+                    int64_t coefnbr, const int64_t *colptr, const int64_t *rowtab, const void *val,
                     double tau, void *L, int64_t *npiv )
 {
     int64_t *col2cblk = malloc( ( n > 0 ? n : 1 ) * sizeof( int64_t ) );
     int64_t  piv = 0;
-    size_t   es = ( flt == 3 ) ? sizeof( double ) : sizeof( float );
-    void    *W = calloc( coefnbr > 0 ? coefnbr : 1, es );
+    void    *W = calloc( coefnbr > 0 ? coefnbr : 1, orc_es( flt ) );
     orc_blas_init();
     for ( int64_t k = 0; k < nc; k++ ) {
         for ( int64_t j = CB( k, C_FCOL ); j <= CB( k, C_LCOL ); j++ ) col2cblk[j] = k;
     }
-    memset( L, 0, coefnbr * es );
-    if ( flt == 3 ) {
-        fill_d( n, cblk, blok, colptr, rowtab, val, col2cblk, (double *)L );
-        for ( int64_t k = 0; k < nc; k++ ) ldlt_cblk_d( cblk, blok, k, (double *)L, (double *)W, &piv, tau );
+    memset( L, 0, coefnbr * orc_es( flt ) );
+#define ORC_LDLT_RUN( T, SFX, VT )                                                                   \
+    fill_##SFX( n, cblk, blok, colptr, rowtab, (const VT *)val, col2cblk, (T *)L );                 \
+    for ( int64_t k = 0; k < nc; k++ ) ldlt_cblk_##SFX( cblk, blok, k, (T *)L, (T *)W, &piv, tau );
+    switch ( flt ) {
+    case 3: ORC_LDLT_RUN( double, d, double ) break;
+    case 2: ORC_LDLT_RUN( float, s, double ) break;
+    case 5: ORC_LDLT_RUN( double complex, z, double complex ) break;
+    default: ORC_LDLT_RUN( float complex, c, double complex ) break;
     }
-    else {
-        fill_s( n, cblk, blok, colptr, rowtab, val, col2cblk, (float *)L );
-        for ( int64_t k = 0; k < nc; k++ ) ldlt_cblk_s( cblk, blok, k, (float *)L, (float *)W, &piv, tau );
-    }
+#undef ORC_LDLT_RUN
     if ( npiv ) *npiv = piv;
     free( W );
     free( col2cblk );
@@ -855,19 +928,33 @@ orc_factorize_ldlt( int flt, int64_t n, int64_t nc, const int64_t *cblk, const i
 /* x <- A^{-1} x with an LDL^T factor (double x, factor in flt). */
 int
 orc_solve_ldlt( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
-                const int64_t *browtab, const void *L, int64_t nrhs, double *x, int64_t ldx )
+                const int64_t *browtab, const void *L, int64_t nrhs, void *x, int64_t ldx )
 {
     orc_blas_init();
     if ( flt == 3 ) {
-        ldltsolve_d( nc, cblk, blok, browtab, (const double *)L, nrhs, x, ldx );
+        ldltsolve_d( nc, cblk, blok, browtab, (const double *)L, nrhs, (double *)x, ldx );
     }
-    else {
-        float *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+    else if ( flt == 5 ) {
+        ldltsolve_z( nc, cblk, blok, browtab, (const double complex *)L, nrhs, (double complex *)x, ldx );
+    }
+    else if ( flt == 2 ) {
+        float  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        double *xd = (double *)x;
         for ( int64_t q = 0; q < nrhs; q++ )
-            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)x[i + q * ldx];
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)xd[i + q * ldx];
         ldltsolve_s( nc, cblk, blok, browtab, (const float *)L, nrhs, xs, n );
         for ( int64_t q = 0; q < nrhs; q++ )
-            for ( int64_t i = 0; i < n; i++ ) x[i + q * ldx] = (double)xs[i + q * n];
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
+    else {
+        float complex  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float complex ) );
+        double complex *xd = (double complex *)x;
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float complex)xd[i + q * ldx];
+        ldltsolve_c( nc, cblk, blok, browtab, (const float complex *)L, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double complex)xs[i + q * n];
         free( xs );
     }
     return 0;

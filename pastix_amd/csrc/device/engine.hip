@@ -18,6 +18,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <limits>
+#include <type_traits>
 #include <vector>
 #include "kernels_solve.hpp"
 #include "kernels_mfma.hpp"
@@ -369,9 +371,11 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     // fill: zero + scatter of the lower triangle of P A P^T (LU: and of the
     // strict upper triangle, transposed, into the U tiles)
     PXD_CHECK( hipMemsetAsync( L, 0, ctx->coefnbr * sizeof( T ), s ) );
-    double ovf = ( sizeof( T ) == 4 ) ? (double)FLT_MAX : DBL_MAX;
-    hipLaunchKernelGGL( ( k_fill<T, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
-                        ctx->n, ctx->dcolptr, ctx->drowtab, (const double *)ctx->dval, ctx->dcol2cblk,
+    typedef typename px_real<T>::type R;
+    typedef typename std::conditional<std::is_same<T, R>::value, double, pxcd>::type TI;   // matrix values
+    const double ovf = (double)std::numeric_limits<R>::max();
+    hipLaunchKernelGGL( ( k_fill<T, TI> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
+                        ctx->n, ctx->dcolptr, ctx->drowtab, (const TI *)ctx->dval, ctx->dcol2cblk,
                         ctx->dcb, ctx->dbk, L, 0, ovf, ctx->dscratch + 1 );
     launches += 2;
     if ( lu ) {
@@ -380,8 +384,8 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         const bool     tr = ( ctx->dtval != nullptr );
         const int64_t *tc = tr ? ctx->dtcolptr : ctx->dcolptr;
         const int64_t *tt = tr ? ctx->dtrowtab : ctx->drowtab;
-        const double  *tv = (const double *)( tr ? ctx->dtval : ctx->dval );
-        hipLaunchKernelGGL( ( k_fill<T, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
+        const TI      *tv = (const TI *)( tr ? ctx->dtval : ctx->dval );
+        hipLaunchKernelGGL( ( k_fill<T, TI> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
                             ctx->n, tc, tt, tv, ctx->dcol2cblk, ctx->dcb, ctx->dbk, U, 1, ovf, ctx->dscratch + 1 );
         launches += 2;
     }
@@ -423,7 +427,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                 hipLaunchKernelGGL( ( k_potrf2<T, true> ), dim3( np ), dim3( PXD_THREADS ), sh, st, ctx->dlist + lv.c0,
                                     ctx->dcb, L, DI, thresh, ctx->dscratch );
             }
-            else {
+            else if constexpr ( std::is_same<T, R>::value ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
                                     ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch );
             }
@@ -434,7 +438,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                 launches++;
             }
         }
-        else {
+        else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
             hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
             launches++;
@@ -457,13 +461,15 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
                                 ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt );
         }
-        else if ( ctx->trsm_rt == 32 ) {
-            hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ), st,
-                                ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
-        }
-        else {
-            hipLaunchKernelGGL( ( k_trsm<T, 16> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 16 ), st,
-                                ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+        else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
+            if ( ctx->trsm_rt == 32 ) {
+                hipLaunchKernelGGL( ( k_trsm<T, 32> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 32 ),
+                                    st, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_trsm<T, 16> ), dim3( nt ), dim3( PXD_THREADS ), trsm_lds<T>( lv.maxn, 16 ),
+                                    st, ctx->dtrsm + lv.t0, ctx->dcb, ctx->dbk, L, (const T *)DI );
+            }
         }
         launches++;
     };
@@ -482,9 +488,11 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             launches++;
         }
         else if ( nu ) {
-            hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0, ctx->dcb,
-                                ctx->dbk, L );
-            launches++;
+            if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
+                hipLaunchKernelGGL( ( k_update<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
+                                    ctx->dcb, ctx->dbk, L );
+                launches++;
+            }
         }
     };
 
@@ -610,7 +618,11 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         return 0;
     }
 
-    // scalar kernels, in place
+    // scalar kernels (real factors), in place
+    if constexpr ( !std::is_same<T, typename px_real<T>::type>::value ) {
+        return 3;
+    }
+    else {
     for ( int li = 0; li < nl; li++ ) {
         const level_t &lv = ctx->levels[li];
         unsigned       np = (unsigned)( lv.c1 - lv.c0 );
@@ -635,14 +647,32 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
                             ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 1, 0 );
     }
+    }
     PXD_CHECK( hipGetLastError() );
     return 0;
+}
+
+// dynamic-LDS limits of the matrix-core kernels (all four arithmetics)
+template <typename T>
+void
+set_lds_attributes_mfma( size_t lds )
+{
+    lds -= 16384;   // room for the kernels' static LDS
+    if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
+        (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds );
+    }
+    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
 }
 
 template <typename T>
 void
 set_lds_attributes( size_t lds )
 {
+    set_lds_attributes_mfma<T>( lds );
     lds -= 16384;   // room for the kernels' static LDS
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
@@ -652,6 +682,44 @@ set_lds_attributes( size_t lds )
     (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+}
+
+// Solve of nrhs vectors of type TV (double or complex double) with a factor
+// of type T: directly when the types agree, else through vectors converted
+// to the factor precision and back, PXD_MAXRHS columns at a time.
+template <typename TV, typename T>
+int
+solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
+{
+    int rc = 0;
+    if constexpr ( std::is_same<TV, T>::value ) {
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
+            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+            rc = solve_sweeps<T>( ctx, (T *)dx + q0 * ldx, ldx, nq );
+        }
+    }
+    else {
+        size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, PXD_MAXRHS ) * sizeof( T );
+        if ( ctx->dvecs_bytes < need ) {
+            if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
+            PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
+            ctx->dvecs_bytes = need;
+        }
+        T *xs = (T *)ctx->dvecs;
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
+            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+            for ( int q = 0; q < nq; q++ ) {
+                hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
+                                    ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
+            }
+            rc = solve_sweeps<T>( ctx, xs, ctx->n, nq );
+            for ( int q = 0; q < nq; q++ ) {
+                hipLaunchKernelGGL( ( k_convert<TV, T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
+                                    ctx->n, xs + q * ctx->n, (TV *)dx + ( q0 + q ) * ldx );
+            }
+        }
+    }
+    return rc;
 }
 
 }  // namespace
@@ -694,6 +762,8 @@ pxd_create( pxd_ctx_t **pctx, int device )
     PXD_CHECK( hipMalloc( &ctx->dred, 4096 ) );
     set_lds_attributes<double>( ctx->lds_max );
     set_lds_attributes<float>( ctx->lds_max );
+    set_lds_attributes_mfma<pxcd>( ctx->lds_max );
+    set_lds_attributes_mfma<pxcf>( ctx->lds_max );
     (void)hipGetLastError();   // an attribute refused above must not be reported later
     if ( getenv( "PXD_VERBOSE" ) ) {
         fprintf( stderr, "pastix_amd device: %s, %d CUs, LDS/block %zu, L2 %d\n", prop.gcnArchName,
@@ -951,8 +1021,12 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
     if ( factotype == FACT_SYTRF && !ctx->kernels ) {
         return 7;   // LDL^T runs on the matrix-core kernels only
     }
-    if ( factor_flttype != FT_DOUBLE && factor_flttype != FT_FLOAT ) {
+    const bool cplx = ( factor_flttype == FT_CPLX64 || factor_flttype == FT_CPLX32 );
+    if ( factor_flttype != FT_DOUBLE && factor_flttype != FT_FLOAT && !cplx ) {
         return 3;
+    }
+    if ( cplx && ( factotype == FACT_POTRF || !ctx->kernels ) ) {
+        return 3;   // complex: symmetric LDL^T and LU on the matrix-core kernels
     }
     if ( factotype == FACT_GETRF && ( ( ctx->dtval == nullptr && ctx->mtx == 111 ) || !ctx->kernels ) ) {
         // LU of a general matrix needs its transpose for the U side, and the
@@ -984,7 +1058,9 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
     }
     ctx->fflt  = factor_flttype;
     ctx->stats.coef_bytes = bytes * ( lu ? 2 : 1 );
-    const int bm = ( factor_flttype == FT_DOUBLE ) ? upd_rows<double>::value : upd_rows<float>::value;
+    const int bm = ( factor_flttype == FT_DOUBLE ) ? upd_rows<double>::value
+                 : ( factor_flttype == FT_FLOAT ) ? upd_rows<float>::value
+                 : ( factor_flttype == FT_CPLX64 ) ? upd_rows<pxcd>::value : upd_rows<pxcf>::value;
     if ( ctx->tasks_dirty || ctx->upd_bm != bm || ctx->facto != factotype ) {
         ctx->upd_bm      = bm;
         ctx->facto       = factotype;
@@ -994,10 +1070,12 @@ pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype, double thresho
             return rc;
         }
     }
-    if ( factor_flttype == FT_DOUBLE ) {
-        return factorize<double>( ctx, threshold, nbpivots );
+    switch ( factor_flttype ) {
+    case FT_DOUBLE: return factorize<double>( ctx, threshold, nbpivots );
+    case FT_FLOAT: return factorize<float>( ctx, threshold, nbpivots );
+    case FT_CPLX64: return factorize<pxcd>( ctx, threshold, nbpivots );
+    default: return factorize<pxcf>( ctx, threshold, nbpivots );
     }
-    return factorize<float>( ctx, threshold, nbpivots );
 }
 
 int
@@ -1024,33 +1102,11 @@ pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
     PXD_CHECK( hipEventCreate( &e1 ) );
     PXD_CHECK( hipEventRecord( e0, ctx->stream ) );
     int rc = 0;
-    if ( ctx->fflt == FT_DOUBLE ) {
-        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
-            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
-            rc = solve_sweeps<double>( ctx, (double *)dx + q0 * ldx, ldx, nq );
-        }
-    }
-    else {
-        // single precision factor: convert the vectors down, sweep, convert back
-        size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, PXD_MAXRHS ) * sizeof( float );
-        if ( ctx->dvecs_bytes < need ) {
-            if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
-            PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
-            ctx->dvecs_bytes = need;
-        }
-        float *xs = (float *)ctx->dvecs;
-        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
-            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
-            for ( int q = 0; q < nq; q++ ) {
-                hipLaunchKernelGGL( ( k_convert<float, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0,
-                                    ctx->stream, ctx->n, (const double *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
-            }
-            rc = solve_sweeps<float>( ctx, xs, ctx->n, nq );
-            for ( int q = 0; q < nq; q++ ) {
-                hipLaunchKernelGGL( ( k_convert<double, float> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0,
-                                    ctx->stream, ctx->n, xs + q * ctx->n, (double *)dx + ( q0 + q ) * ldx );
-            }
-        }
+    switch ( ctx->fflt ) {
+    case FT_DOUBLE: rc = solve_vectors<double, double>( ctx, nrhs, dx, ldx ); break;
+    case FT_FLOAT: rc = solve_vectors<double, float>( ctx, nrhs, dx, ldx ); break;
+    case FT_CPLX64: rc = solve_vectors<pxcd, pxcd>( ctx, nrhs, dx, ldx ); break;
+    default: rc = solve_vectors<pxcd, pxcf>( ctx, nrhs, dx, ldx ); break;
     }
     PXD_CHECK( hipEventRecord( e1, ctx->stream ) );
     PXD_CHECK( hipEventSynchronize( e1 ) );

@@ -93,7 +93,8 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     T *A  = L + c.coefoff;            // n x n, ld n
     T *LI = Linv + c.dinvoff;         // n x n, ld n
     int piv = 0;
-    const T tau = (T)thresh;
+    typedef typename px_real<T>::type R;
+    const R tau = (R)thresh;
 
     for ( int32_t kb = 0; kb < n; kb += PXD_NB ) {
         const int32_t bs = min( PXD_NB, n - kb );
@@ -118,8 +119,8 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         for ( int cc = 0; cc < bs; cc++ ) {
             T d = sA[cc][cc];
             if ( px_abs( d ) < tau ) {
-                // Cholesky: tau (core_zpotf2sp); LDL^T: sign(d) tau (core_zsytf2sp)
-                d = ( LDLT && d < (T)0 ) ? -tau : tau;
+                // Cholesky: tau (core_zpotf2sp); LDL^T: sign(re d) tau (core_zsytf2sp)
+                d = ( LDLT && px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
                 if ( tid == 0 ) {
                     piv++;
                 }
@@ -143,7 +144,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             int rr = idx % bs, cc = idx / bs;
             if ( rr > cc ) {
                 const T a = sA[cc][rr];
-                if ( LDLT ) {
+                if constexpr ( LDLT ) {
                     sA[rr][cc] = a;
                     sA[cc][rr] = a / sD[cc];
                 }
@@ -154,7 +155,12 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         }
         __syncthreads();
         if ( tid < bs ) {
-            sA[tid][tid] = LDLT ? sD[tid] : px_sqrt( sD[tid] );
+            if constexpr ( LDLT ) {
+                sA[tid][tid] = sD[tid];
+            }
+            else {
+                sA[tid][tid] = px_sqrt( sD[tid] );
+            }
         }
         if ( tid < PXD_NB ) {
             sDi[tid] = ( tid < bs ) ? (T)1 / sD[tid] : (T)0;
@@ -214,10 +220,10 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                 const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
                 if ( rr < m ) {
                     // LDL^T: the product is L21 D; L21 = product / D
-                    sP[rr][c0] = acc0[v];   // zero beyond bs: sI is zero there
-                    sP[rr][c1] = acc1[v];
-                    if ( c0 < bs ) A[( m0 + rr ) + (int64_t)( kb + c0 ) * n] = LDLT ? acc0[v] * sDi[c0] : acc0[v];
-                    if ( c1 < bs ) A[( m0 + rr ) + (int64_t)( kb + c1 ) * n] = LDLT ? acc1[v] * sDi[c1] : acc1[v];
+                    sP[rr][c0] = M::get( acc0, v );   // zero beyond bs: sI is zero there
+                    sP[rr][c1] = M::get( acc1, v );
+                    if ( c0 < bs ) A[( m0 + rr ) + (int64_t)( kb + c0 ) * n] = LDLT ? M::get( acc0, v ) * sDi[c0] : M::get( acc0, v );
+                    if ( c1 < bs ) A[( m0 + rr ) + (int64_t)( kb + c1 ) * n] = LDLT ? M::get( acc1, v ) * sDi[c1] : M::get( acc1, v );
                 }
             }
         }
@@ -293,7 +299,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                                 const int32_t r  = 32 * ti[g] + 16 * a + M::row( lane, v );
                                 const int32_t cc = 32 * tj[g] + 16 * b + ( lane & 15 );
                                 if ( tv[g] && r < m && cc < m && r >= cc ) {
-                                    A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[g][a][b][v] - acc[g][a][b][v];
+                                    A[( m0 + r ) + (int64_t)( m0 + cc ) * n] = old[g][a][b][v] - M::get( acc[g][a][b], v );
                                 }
                             }
             }
@@ -338,8 +344,8 @@ small_trsm_mfma( T *base, int64_t rs, int64_t ks, int32_t rows, int32_t bs, cons
         for ( int v = 0; v < 4; v++ ) {
             const int32_t rr = rb + M::row( lane, v );
             const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
-            if ( rr < rows && c0 < bs ) base[rr * rs + c0 * ks] = acc0[v];
-            if ( rr < rows && c1 < bs ) base[rr * rs + c1 * ks] = acc1[v];
+            if ( rr < rows && c0 < bs ) base[rr * rs + c0 * ks] = M::get( acc0, v );
+            if ( rr < rows && c1 < bs ) base[rr * rs + c1 * ks] = M::get( acc1, v );
         }
     }
 }
@@ -374,7 +380,8 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     T *IL = DL + c.dinvoff;
     T *IU = DU + c.dinvoff;
     int     piv = 0;
-    const T tau = (T)thresh;
+    typedef typename px_real<T>::type R;
+    const R tau = (R)thresh;
     const int64_t nn = (int64_t)n * n;
 
     // A_kk = L part + (U part)^T, 8 elements per thread in flight
@@ -421,7 +428,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         for ( int cc = 0; cc < bs; cc++ ) {
             T d = sA[cc][cc];
             if ( px_abs( d ) < tau ) {
-                d = ( d < (T)0 ) ? -tau : tau;
+                d = ( px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
                 if ( tid == 0 ) {
                     piv++;
                 }
@@ -626,8 +633,8 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
                 const int32_t rr = rb + M::row( lane, v );
-                acc0[v] = ( rr < nr ) ? sR[rr][lane & 15] : (T)0;
-                acc1[v] = ( rr < nr ) ? sR[rr][16 + ( lane & 15 )] : (T)0;
+                M::set( acc0, v, ( rr < nr ) ? sR[rr][lane & 15] : (T)0 );
+                M::set( acc1, v, ( rr < nr ) ? sR[rr][16 + ( lane & 15 )] : (T)0 );
             }
 #pragma unroll
             for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
@@ -639,8 +646,8 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
             for ( int v = 0; v < 4; v++ ) {
                 const int32_t rr = rb + M::row( lane, v );
                 if ( rr < nr ) {
-                    sR[rr][lane & 15]        = acc0[v];
-                    sR[rr][16 + ( lane & 15 )] = acc1[v];
+                    sR[rr][lane & 15]        = M::get( acc0, v );
+                    sR[rr][16 + ( lane & 15 )] = M::get( acc1, v );
                 }
             }
         }
@@ -672,8 +679,8 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
                 const int r = 16 * bi + M::row( lane, v ), cc = 16 * bc + ( lane & 15 );
-                sX[r][cc] = acc[v];
-                if ( r < bk && cc < bj ) LI[( K + r ) + (int64_t)( J + cc ) * n] = acc[v];
+                sX[r][cc] = M::get( acc, v );
+                if ( r < bk && cc < bj ) LI[( K + r ) + (int64_t)( J + cc ) * n] = M::get( acc, v );
             }
         }
         __syncthreads();
@@ -730,70 +737,79 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     }
     __syncthreads();
 
-    acc_t acc[PXD_TRSM_NT];
-#pragma unroll
-    for ( int j = 0; j < PXD_TRSM_NT; j++ ) acc[j] = M::zero();
-
     const int     lr    = tid & 63;          // row this thread loads
     const int64_t lbase = sBase[lr];
     const int32_t lld   = sLd[lr];
 
-    // the next stage's loads are issued before the current stage's products
-    constexpr int NLI = ( BK * PXD_TRSM_NT * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
-    T             rb[BK / 4], rl[NLI];
-    auto          load = [&]( int32_t l0 ) {
+    // Column passes of NTP 16-column tiles, the highest first: Linv is
+    // lower triangular, so a pass reads only B columns up to its own last
+    // one and may overwrite its columns in place before the lower passes run.
+    constexpr int NTP = ( sizeof( T ) <= 8 ) ? PXD_TRSM_NT / 2 : 6;
+    constexpr int NLI = ( BK * NTP * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
+    const int     npass = ( nt + NTP - 1 ) / NTP;
+    for ( int pass = npass - 1; pass >= 0; pass-- ) {
+        const int32_t cb0  = pass * NTP * 16;                  // first column of the pass
+        const int32_t lend = min( n, cb0 + NTP * 16 );         // Linv[c][l] = 0 for l > c
+        acc_t acc[NTP];
 #pragma unroll
-        for ( int u = 0; u < BK / 4; u++ ) {
-            const int32_t l = l0 + ( tid >> 6 ) + 4 * u;
-            rb[u] = ( lbase >= 0 && l < n ) ? L[lbase + (int64_t)l * lld] : (T)0;
-        }
+        for ( int j = 0; j < NTP; j++ ) acc[j] = M::zero();
+
+        // the next stage's loads are issued before the current stage's products
+        T    rb[BK / 4], rl[NLI];
+        auto load = [&]( int32_t l0 ) {
 #pragma unroll
-        for ( int u = 0; u < NLI; u++ ) {
-            const int     idx = tid + u * PXD_THREADS;
-            const int     cc = idx % ( PXD_TRSM_NT * 16 ), kk = idx / ( PXD_TRSM_NT * 16 );
-            const int32_t l = l0 + kk;
-            rl[u] = ( kk < BK && cc < n && l < n && 16 * ( cc >> 4 ) + 15 >= l ) ? LI[cc + (int64_t)l * n] : (T)0;
-        }
-    };
-    load( 0 );
-    for ( int32_t l0 = 0; l0 < n; l0 += BK ) {
+            for ( int u = 0; u < BK / 4; u++ ) {
+                const int32_t l = l0 + ( tid >> 6 ) + 4 * u;
+                rb[u] = ( lbase >= 0 && l < lend ) ? L[lbase + (int64_t)l * lld] : (T)0;
+            }
 #pragma unroll
-        for ( int u = 0; u < BK / 4; u++ ) sB[( tid >> 6 ) + 4 * u][lr] = rb[u];
+            for ( int u = 0; u < NLI; u++ ) {
+                const int     idx = tid + u * PXD_THREADS;
+                const int     ci = idx % ( NTP * 16 ), kk = idx / ( NTP * 16 );
+                const int32_t cc = cb0 + ci, l = l0 + kk;
+                rl[u] = ( kk < BK && cc < n && l < lend && 16 * ( cc >> 4 ) + 15 >= l ) ? LI[cc + (int64_t)l * n]
+                                                                                       : (T)0;
+            }
+        };
+        load( 0 );
+        for ( int32_t l0 = 0; l0 < lend; l0 += BK ) {
 #pragma unroll
-        for ( int u = 0; u < NLI; u++ ) {
-            const int idx = tid + u * PXD_THREADS;
-            if ( idx < BK * PXD_TRSM_NT * 16 ) sLi[idx / ( PXD_TRSM_NT * 16 )][idx % ( PXD_TRSM_NT * 16 )] = rl[u];
-        }
-        __syncthreads();
-        if ( l0 + BK < n ) {
-            load( l0 + BK );
-        }
+            for ( int u = 0; u < BK / 4; u++ ) sB[( tid >> 6 ) + 4 * u][lr] = rb[u];
 #pragma unroll
-        for ( int k4 = 0; k4 < BK; k4 += 4 ) {
-            const T a = sB[k4 + ( lane >> 4 )][16 * w + ( lane & 15 )];
+            for ( int u = 0; u < NLI; u++ ) {
+                const int idx = tid + u * PXD_THREADS;
+                if ( idx < BK * NTP * 16 ) sLi[idx / ( NTP * 16 )][idx % ( NTP * 16 )] = rl[u];
+            }
+            __syncthreads();
+            if ( l0 + BK < lend ) {
+                load( l0 + BK );
+            }
 #pragma unroll
-            for ( int j = 0; j < PXD_TRSM_NT; j++ ) {
-                if ( j < nt && 16 * j + 15 >= l0 + k4 ) {
-                    const T b = sLi[k4 + ( lane >> 4 )][16 * j + ( lane & 15 )];
-                    acc[j] = M::mma( a, b, acc[j] );
+            for ( int k4 = 0; k4 < BK; k4 += 4 ) {
+                const T a = sB[k4 + ( lane >> 4 )][16 * w + ( lane & 15 )];
+#pragma unroll
+                for ( int j = 0; j < NTP; j++ ) {
+                    if ( cb0 + 16 * j < n && cb0 + 16 * j + 15 >= l0 + k4 ) {
+                        const T b = sLi[k4 + ( lane >> 4 )][16 * j + ( lane & 15 )];
+                        acc[j] = M::mma( a, b, acc[j] );
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-    }
 #pragma unroll
-    for ( int j = 0; j < PXD_TRSM_NT; j++ ) {
-        if ( j < nt ) {
-            const int cc = 16 * j + ( lane & 15 );
+        for ( int j = 0; j < NTP; j++ ) {
+            const int cc = cb0 + 16 * j + ( lane & 15 );
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
                 const int r = 16 * w + M::row( lane, v );
                 if ( sBase[r] >= 0 && cc < n ) {
                     // LDL^T: L_off = B L_kk^{-T} D^{-1}
-                    L[sBase[r] + (int64_t)cc * sLd[r]] = ldlt ? acc[j][v] * sDi[cc] : acc[j][v];
+                    L[sBase[r] + (int64_t)cc * sLd[r]] = ldlt ? M::get( acc[j], v ) * sDi[cc] : M::get( acc[j], v );
                 }
             }
         }
+        __syncthreads();   // the pass's stores precede the next pass's loads
     }
 }
 
@@ -941,7 +957,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                 const int cc = wc + 16 * j + ( lane & 15 );
                 T *dst = p[i][v] + (int64_t)cc * sDstLd[r];
                 if ( atomic ) {
-                    if ( pv[i][j][v] ) px_atomic_add( dst, -acc[i][j][v] );
+                    if ( pv[i][j][v] ) px_atomic_add( dst, -M::get( acc[i][j], v ) );
                 }
                 else {
                     old[i][j][v] = pv[i][j][v] ? *dst : (T)0;
@@ -956,7 +972,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                 for ( int v = 0; v < 4; v++ ) {
                     const int r  = wr + 16 * i + M::row( lane, v );
                     const int cc = wc + 16 * j + ( lane & 15 );
-                    if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - acc[i][j][v];
+                    if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - M::get( acc[i][j], v );
                 }
     }
 }
@@ -992,7 +1008,7 @@ update_scatter( T *L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int l
             const int cc = 16 * j + ( lane & 15 );
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
-                if ( pv[v] && cc < nvalid ) px_atomic_add( p[v] + (int64_t)cc * pl[v], -acc[j][v] );
+                if ( pv[v] && cc < nvalid ) px_atomic_add( p[v] + (int64_t)cc * pl[v], -M::get( acc[j], v ) );
             }
         }
         return;
@@ -1009,7 +1025,7 @@ update_scatter( T *L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int l
         const int cc = 16 * j + ( lane & 15 );
 #pragma unroll
         for ( int v = 0; v < 4; v++ ) {
-            if ( pv[v] && cc < nvalid ) p[v][(int64_t)cc * pl[v]] = old[j][v] - acc[j][v];
+            if ( pv[v] && cc < nvalid ) p[v][(int64_t)cc * pl[v]] = old[j][v] - M::get( acc[j], v );
         }
     }
 }
@@ -1018,6 +1034,8 @@ update_scatter( T *L, const typename mfma16<T>::acc_t ( &acc )[4], int r0, int l
 template <typename T> struct upd_rows;
 template <> struct upd_rows<float>  { static constexpr int value = 256; };
 template <> struct upd_rows<double> { static constexpr int value = 128; };
+template <> struct upd_rows<pxcf>   { static constexpr int value = 128; };
+template <> struct upd_rows<pxcd>   { static constexpr int value = 64; };
 
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS, 3 )
@@ -1186,7 +1204,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
 #pragma unroll
         for ( int i = 0; i < RB; i++ )
 #pragma unroll
-            for ( int v = 0; v < 4; v++ ) sE[( 16 * i + M::row( lane, v ) ) * 17 + lr] = acc[i][j][v];
+            for ( int v = 0; v < 4; v++ ) sE[( 16 * i + M::row( lane, v ) ) * 17 + lr] = M::get( acc[i][j], v );
         __syncthreads();
         uint32_t off[CPL];
         T        val[CPL];

@@ -19,6 +19,8 @@ template <> struct mfma16<float> {
         return __builtin_amdgcn_mfma_f32_16x16x4f32( a, b, c, 0, 0, 0 );
     }
     __device__ static inline int row( int lane, int v ) { return 4 * ( lane >> 4 ) + v; }
+    __device__ static inline float get( const acc_t &c, int v ) { return c[v]; }
+    __device__ static inline void  set( acc_t &c, int v, float x ) { c[v] = x; }
 };
 
 template <> struct mfma16<double> {
@@ -29,7 +31,37 @@ template <> struct mfma16<double> {
         return __builtin_amdgcn_mfma_f64_16x16x4f64( a, b, c, 0, 0, 0 );
     }
     __device__ static inline int row( int lane, int v ) { return ( lane >> 4 ) + 4 * v; }
+    __device__ static inline double get( const acc_t &c, int v ) { return c[v]; }
+    __device__ static inline void   set( acc_t &c, int v, double x ) { c[v] = x; }
 };
+
+// Complex 16x16x4 products as four real ones: (ar + i ai)(br + i bi)
+// accumulated into separate real and imaginary accumulators (no conjugation:
+// the complex factorizations here are symmetric, not Hermitian).
+template <typename C, typename R> struct mfma16_cplx {
+    typedef typename mfma16<R>::acc_t racc;
+    struct acc_t {
+        racc re, im;
+    };
+    __device__ static inline acc_t zero() { return acc_t{ mfma16<R>::zero(), mfma16<R>::zero() }; }
+    __device__ static inline acc_t mma( C a, C b, acc_t c )
+    {
+        c.re = mfma16<R>::mma( a.re, b.re, c.re );
+        c.re = mfma16<R>::mma( -a.im, b.im, c.re );
+        c.im = mfma16<R>::mma( a.re, b.im, c.im );
+        c.im = mfma16<R>::mma( a.im, b.re, c.im );
+        return c;
+    }
+    __device__ static inline int row( int lane, int v ) { return mfma16<R>::row( lane, v ); }
+    __device__ static inline C    get( const acc_t &c, int v ) { return C( c.re[v], c.im[v] ); }
+    __device__ static inline void set( acc_t &c, int v, C x )
+    {
+        c.re[v] = x.re;
+        c.im[v] = x.im;
+    }
+};
+template <> struct mfma16<pxcf> : mfma16_cplx<pxcf, float> {};
+template <> struct mfma16<pxcd> : mfma16_cplx<pxcd, double> {};
 
 // 16-byte LDS fragments: KV consecutive k values of one row
 template <typename T> struct lds_vec;
@@ -40,6 +72,19 @@ template <> struct lds_vec<float> {
 template <> struct lds_vec<double> {
     typedef double type __attribute__( ( ext_vector_type( 2 ) ) );
     static constexpr int n = 2;
+};
+template <typename C, int N> struct cvec {
+    C e[N];
+    __device__ inline C       &operator[]( int i ) { return e[i]; }
+    __device__ inline const C &operator[]( int i ) const { return e[i]; }
+} __attribute__( ( aligned( 16 ) ) );
+template <> struct lds_vec<pxcf> {
+    typedef cvec<pxcf, 2> type;
+    static constexpr int n = 2;
+};
+template <> struct lds_vec<pxcd> {
+    typedef cvec<pxcd, 1> type;
+    static constexpr int n = 1;
 };
 
 // Task of workgroup b among nb: the workgroups that the dispatcher sends to
@@ -70,6 +115,37 @@ template <> struct bufio<float> {
     __device__ static inline void store( float v, __amdgpu_buffer_rsrc_t r, uint32_t off )
     {
         __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned, v ), r, off, 0, 0 );
+    }
+};
+template <> struct bufio<pxcf> {
+    typedef unsigned u2 __attribute__( ( ext_vector_type( 2 ) ) );
+    __device__ static inline pxcf load( __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        u2 v = __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 );
+        return pxcf( __builtin_bit_cast( float, v.x ), __builtin_bit_cast( float, v.y ) );
+    }
+    __device__ static inline void store( pxcf v, __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        u2 w;
+        w.x = __builtin_bit_cast( unsigned, v.re );
+        w.y = __builtin_bit_cast( unsigned, v.im );
+        __builtin_amdgcn_raw_buffer_store_b64( w, r, off, 0, 0 );
+    }
+};
+template <> struct bufio<pxcd> {
+    typedef unsigned u4 __attribute__( ( ext_vector_type( 4 ) ) );
+    typedef unsigned u2 __attribute__( ( ext_vector_type( 2 ) ) );
+    __device__ static inline pxcd load( __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        u4 v = __builtin_amdgcn_raw_buffer_load_b128( r, off, 0, 0 );
+        u2 a = { v.x, v.y }, b = { v.z, v.w };
+        return pxcd( __builtin_bit_cast( double, a ), __builtin_bit_cast( double, b ) );
+    }
+    __device__ static inline void store( pxcd v, __amdgpu_buffer_rsrc_t r, uint32_t off )
+    {
+        u2 a = __builtin_bit_cast( u2, v.re ), b = __builtin_bit_cast( u2, v.im );
+        u4 w = { a.x, a.y, b.x, b.y };
+        __builtin_amdgcn_raw_buffer_store_b128( w, r, off, 0, 0 );
     }
 };
 template <> struct bufio<double> {

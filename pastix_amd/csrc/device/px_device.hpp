@@ -21,9 +21,13 @@
 
 struct pxcd {
     double re, im;
+    pxcd() = default;
+    __host__ __device__ constexpr pxcd( double r, double i = 0. ) : re( r ), im( i ) {}
 };
 struct pxcf {
     float re, im;
+    pxcf() = default;
+    __host__ __device__ constexpr pxcf( float r, float i = 0.f ) : re( r ), im( i ) {}
 };
 
 // ---------------------------------------------------------------------------
@@ -55,6 +59,16 @@ __host__ __device__ inline pxcf operator-( pxcf a ) { return pxcf{ -a.re, -a.im 
 
 __host__ __device__ inline pxcd operator*( pxcd a, double b ) { return pxcd{ a.re * b, a.im * b }; }
 __host__ __device__ inline pxcf operator*( pxcf a, float b ) { return pxcf{ a.re * b, a.im * b }; }
+__host__ __device__ inline pxcd &operator+=( pxcd &a, pxcd b ) { a = a + b; return a; }
+__host__ __device__ inline pxcd &operator-=( pxcd &a, pxcd b ) { a = a - b; return a; }
+__host__ __device__ inline pxcf &operator+=( pxcf &a, pxcf b ) { a = a + b; return a; }
+__host__ __device__ inline pxcf &operator-=( pxcf &a, pxcf b ) { a = a - b; return a; }
+
+// real part (the sign of a static pivot follows it, core_zsytf2sp)
+__host__ __device__ inline double px_re( double a ) { return a; }
+__host__ __device__ inline float  px_re( float a ) { return a; }
+__host__ __device__ inline double px_re( pxcd a ) { return a.re; }
+__host__ __device__ inline float  px_re( pxcf a ) { return a.re; }
 
 __host__ __device__ inline pxcd px_div( pxcd a, pxcd b )
 {
@@ -75,6 +89,8 @@ __host__ __device__ inline pxcf px_div( pxcf a, pxcf b )
     float r = b.re / b.im, d = b.im + b.re * r;
     return pxcf{ ( a.re * r + a.im ) / d, ( a.im * r - a.re ) / d };
 }
+__host__ __device__ inline pxcd operator/( pxcd a, pxcd b ) { return px_div( a, b ); }
+__host__ __device__ inline pxcf operator/( pxcf a, pxcf b ) { return px_div( a, b ); }
 __host__ __device__ inline double px_div( double a, double b ) { return a / b; }
 __host__ __device__ inline float  px_div( float a, float b ) { return a / b; }
 

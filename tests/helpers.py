@@ -60,15 +60,16 @@ def cblk_diff(solver, La, Lb, dtype, full_diag=False):
     for k in range(solver["cblknbr"]):
         n = cb[k, 1] - cb[k, 0] + 1
         off0, off1 = cb[k, 4], cb[k + 1, 4]
-        a = La[off0:off1].astype(np.float64)
-        b = Lb[off0:off1].astype(np.float64)
+        wide = np.complex128 if np.iscomplexobj(Lb) else np.float64
+        a = La[off0:off1].astype(wide)
+        b = Lb[off0:off1].astype(wide)
         # the diagonal tile: compare its lower triangle only
         d_a = a[:n * n].reshape(n, n).T
         d_b = b[:n * n].reshape(n, n).T
         if not full_diag:
             d_a, d_b = np.tril(d_a), np.tril(d_b)
         diff = max(np.abs(d_a - d_b).max(), np.abs(a[n * n:] - b[n * n:]).max() if off1 - off0 > n * n else 0.0)
-        nrm = np.sqrt((d_b ** 2).sum() + (b[n * n:] ** 2).sum())
+        nrm = np.sqrt((np.abs(d_b) ** 2).sum() + (np.abs(b[n * n:]) ** 2).sum())
         if nrm > 0:
             worst = max(worst, diff / (nrm * eps))
     return worst

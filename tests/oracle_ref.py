@@ -97,6 +97,9 @@ def transpose_csc(cp, rt, val):
     return T.indptr.astype(np.int64), T.indices.astype(np.int64), np.ascontiguousarray(T.data)
 
 
+_FLT = {np.dtype(np.float64): 3, np.dtype(np.float32): 2, np.dtype(np.complex128): 5, np.dtype(np.complex64): 4}
+
+
 class OracleLU:
     """LU factor with static pivoting computed by the oracle: L tiles (unit
     lower + the U part of the diagonal tiles) and U^T tiles."""
@@ -106,7 +109,7 @@ class OracleLU:
         self.cp, self.rt, self.val = bcsc
         self.tcp, self.trt, self.tval = transpose_csc(*bcsc)
         self.dtype = dtype
-        self.flt = 3 if dtype == np.float64 else 2
+        self.flt = _FLT[np.dtype(dtype)]
         self.n = solver["nodenbr"]
         self.L = np.zeros(solver["coefnbr"], dtype=dtype)
         self.U = np.zeros(solver["coefnbr"], dtype=dtype)
@@ -117,7 +120,7 @@ class OracleLU:
         self.npiv = int(npiv[0])
 
     def solve(self, b):
-        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        x = np.array(b, dtype=np.complex128 if self.flt in (4, 5) else np.float64, order="F", copy=True)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         lib().orc_solve_lu(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
                            _ptr(self.s["browtab"]), _ptr(self.L), _ptr(self.U), nrhs, _ptr(x), self.n)
@@ -132,7 +135,7 @@ class OracleLDLT:
         self.s = solver
         self.cp, self.rt, self.val = bcsc
         self.dtype = dtype
-        self.flt = 3 if dtype == np.float64 else 2
+        self.flt = _FLT[np.dtype(dtype)]
         self.n = solver["nodenbr"]
         self.L = np.zeros(solver["coefnbr"], dtype=dtype)
         npiv = np.zeros(1, dtype=np.int64)
@@ -142,7 +145,7 @@ class OracleLDLT:
         self.npiv = int(npiv[0])
 
     def solve(self, b):
-        x = np.array(b, dtype=np.float64, order="F", copy=True)
+        x = np.array(b, dtype=np.complex128 if self.flt in (4, 5) else np.float64, order="F", copy=True)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         lib().orc_solve_ldlt(self.flt, self.n, self.s["cblknbr"], _ptr(self.s["cblk"]), _ptr(self.s["blok"]),
                              _ptr(self.s["browtab"]), _ptr(self.L), nrhs, _ptr(x), self.n)

@@ -147,7 +147,7 @@ def test_plrnt_jump_matches_sequential_lcg():
 
 def _dense_lu_nopiv(A):
     """Doolittle LU without pivoting, packed (unit L strictly below, U on and above)."""
-    A = np.array(A, dtype=np.float64, copy=True)
+    A = np.array(A, copy=True)
     n = A.shape[0]
     for k in range(n):
         A[k + 1:, k] /= A[k, k]
@@ -202,10 +202,10 @@ def test_lu_static_pivot_known_answers():
 
 def _dense_ldlt_nopiv(A):
     """LDL^T without pivoting: returns (unit lower L, d)."""
-    A = np.array(A, dtype=np.float64, copy=True)
+    A = np.array(A, copy=True)
     n = A.shape[0]
-    Lm = np.eye(n)
-    d = np.zeros(n)
+    Lm = np.eye(n, dtype=A.dtype)
+    d = np.zeros(n, dtype=A.dtype)
     for k in range(n):
         d[k] = A[k, k]
         Lm[k + 1:, k] = A[k + 1:, k] / d[k]
@@ -257,3 +257,47 @@ def test_ldlt_static_pivot_known_answers():
     assert T[1, 0] == pytest.approx(2.0 / -tau)
     assert T[0, 1] == pytest.approx(2.0)                       # (D L^T)[0,1] = d0 * l10
     assert T[1, 1] == pytest.approx(3.0 - (2.0 / -tau) ** 2 * -tau)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (6, 5, 4), (8, 7, 6)])
+def test_oracle_complex_symmetric_ldlt_equals_dense(dims):
+    # A = L7 - (0.5 + 0.1i) I: complex symmetric, not Hermitian; the
+    # factorization uses plain transposes (core_zsytrfsp)
+    spm = px.Spm.helmholtz7(*dims, 0.5, 0.1)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLDLT)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd, complex_=True)
+    n = s["nodenbr"]
+    A = _dense(*bc, n)
+    assert np.abs(A - A.T).max() == 0 and np.abs(A - A.conj().T).max() > 0.1
+    F = orc.OracleLDLT(s, bc, static_pivot_threshold(dparm, np.float64), np.complex128)
+    assert F.npiv == 0
+    Ld, d = _dense_ldlt_nopiv(A)
+    Lo = orc.dense_from_factor(s, F.L)
+    assert np.abs(np.tril(Lo, -1) - np.tril(Ld, -1)).max() <= 1e-11
+    assert np.abs(np.diag(Lo) - d).max() <= 1e-12 * np.abs(d).max()
+    assert np.abs((np.tril(Ld, -1) + np.eye(n)) @ np.diag(d) @ (np.tril(Ld, -1) + np.eye(n)).T - A).max() <= 1e-11
+    x0, b = spm.gen_rhs(2, seed=3)
+    xp = F.solve(permute_rhs(pd, b))
+    x = xp[px.order_export(pd)[0]]
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 1e-13
+    # the single precision factor
+    F32 = orc.OracleLDLT(s, bc, static_pivot_threshold(dparm, np.float32), np.complex64)
+    L32 = orc.dense_from_factor(s, F32.L).astype(np.complex128)
+    assert np.abs(np.tril(L32, -1) - np.tril(Ld, -1)).max() <= 1e-4
+    px.finalize(pd)
+
+
+def test_oracle_complex_lu_equals_dense():
+    dims = (5, 4, 4)
+    spm = px.Spm.helmholtz7(*dims, 0.5, 0.1)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLU)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd, complex_=True)
+    A = _dense(*bc, s["nodenbr"])
+    F = orc.OracleLU(s, bc, static_pivot_threshold(dparm, np.float64), np.complex128)
+    D = _dense_lu_nopiv(A.astype(np.complex128))
+    packed = np.tril(orc.dense_from_factor(s, F.L), -1) + np.triu(orc.dense_from_factor(s, F.U).T)
+    assert np.abs(packed - D).max() <= 1e-12 * np.abs(D).max()
+    px.finalize(pd)
