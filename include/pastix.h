@@ -121,6 +121,7 @@ int  pastixCheckParam( const pastix_int_t *iparm, const double *dparm );
  * reference, which exposes its SolverMatrix only through internal headers).
  */
 #include "pastix_amd_engine.h"
+#include "pastix_amd_models.h"
 
 #ifdef __cplusplus
 }

@@ -154,6 +154,43 @@ int pxd_vec_gemv  ( pxd_ctx_t *ctx, int64_t n, int64_t nrhs, int64_t m, const vo
 /* Residual-norm helpers: ||A||_inf of the uploaded matrix. */
 int pxd_matrix_norm_inf( pxd_ctx_t *ctx, double *norm );
 
+/*
+ * Kernel timing log of the last factorization run with the option
+ * "model_log" = 1: one record per panel, per TRSM of a cblk and per blok
+ * update, as the reference's kernel_trace_stop records them
+ * (kernels/kernels_trace.h:134-177; written out as model.csv lines
+ * Kernel;M;N;K;Time by kernels/kernels_trace.c:222-249).  ktype is a
+ * pastix_ktype_t (pastix/kernels_enums.h).  time is in seconds: the span from
+ * the first workgroup's start to the last one's end, with the task alone on
+ * the device.  Copies at most max records; *count gets the total.
+ */
+typedef struct pxd_model_entry_s {
+    int32_t ktype, m, n, k;
+    double  time;
+} pxd_model_entry_t;
+int pxd_model_log( const pxd_ctx_t *ctx, pxd_model_entry_t *entries, int64_t max, int64_t *count );
+
+/*
+ * Learned cost model (no counterpart in the reference, whose costs are the
+ * polynomials of common/models.h:38-83): a multilayer perceptron with 8
+ * inputs, two tanh layers of 32 units and a linear output, trained on the GPU
+ * by full-batch Adam on the mean squared error of the uploaded rows.
+ * Parameters (pxd_mlp_nparams() floats): W1[8][32], b1[32], W2[32][32],
+ * b2[32], w3[32], b3.  X is row-major n x 8.
+ */
+typedef struct pxd_mlp_s pxd_mlp_t;
+int  pxd_mlp_nparams   ( void );
+int  pxd_mlp_create    ( pxd_mlp_t **mlp, int device, uint64_t seed );
+void pxd_mlp_destroy   ( pxd_mlp_t *mlp );
+int  pxd_mlp_get_params( pxd_mlp_t *mlp, float *params );
+int  pxd_mlp_set_params( pxd_mlp_t *mlp, const float *params );   /* also resets Adam */
+int  pxd_mlp_set_data  ( pxd_mlp_t *mlp, const float *X, const float *t, int64_t n );
+/* loss = sum_r (y_r - t_r)^2 / n over the uploaded rows, and its gradient */
+int  pxd_mlp_loss_grad ( pxd_mlp_t *mlp, double *loss, float *grad );
+/* steps Adam steps (beta 0.9, 0.999, eps 1e-8) at rate lr; loss after them */
+int  pxd_mlp_train     ( pxd_mlp_t *mlp, int steps, double lr, double *loss );
+int  pxd_mlp_predict   ( pxd_mlp_t *mlp, const float *X, int64_t n, float *y );
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@
 #include <vector>
 #include "kernels_solve.hpp"
 #include "kernels_mfma.hpp"
+#include "pastix/kernels_enums.h"
 
 namespace {
 
@@ -37,7 +38,17 @@ struct level_t {
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
     int64_t d0, d1;     // solve diagonal tasks (64-row chunks of the diagonal blocks)
     int64_t i0, i1;     // block-column inversion tasks
+    int64_t g0, g1;     // blok updates (upd_group)
     int32_t maxn;
+};
+
+// The tiles of one blok update (source cblk, facing blok b, side), in one of
+// the two tile lists, and its GEMM sizes as the reference logs them.
+struct upd_group {
+    int32_t cblk, blok;
+    int32_t M, N, K;
+    int32_t big, side;
+    int64_t t0, t1;
 };
 
 }  // namespace
@@ -101,8 +112,19 @@ struct pxd_ctx_s {
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
 
+    // host copies of the panel, TRSM and inversion lists, and the blok
+    // updates in launch order (kernel timing log)
+    std::vector<int32_t>   hlist;
+    std::vector<row_task>  htrsm;
+    std::vector<inv_task>  hinvt;
+    std::vector<upd_group> groups;
+
     // options / stats
     int          timing = 0;
+    // 1: the factorization runs one stream, launches every blok update on
+    // its own and records per-workgroup times into model_entries
+    int          model_log = 0;
+    std::vector<pxd_model_entry_t> model_entries;
     pxd_stats_t  stats{};
 };
 
@@ -214,6 +236,7 @@ build_schedule( pxd_ctx_t *ctx )
     struct iv { int32_t side, F, f, l; std::vector<upd_task> *list; int64_t first, last; };
     const bool lu = ( ctx->facto == FACT_GETRF );
     std::vector<iv> ivs;
+    ctx->groups.clear();
 
     for ( int32_t L = 0; L < nlev; L++ ) {
         level_t &lv = ctx->levels[L];
@@ -224,6 +247,7 @@ build_schedule( pxd_ctx_t *ctx )
         lv.f0   = solv.size();
         lv.d0   = diag.size();
         lv.i0   = invt.size();
+        lv.g0   = ctx->groups.size();
         lv.maxn = 0;
         ivs.clear();
         for ( int32_t k : bylev[L] ) {
@@ -278,6 +302,8 @@ build_schedule( pxd_ctx_t *ctx )
                             }
                             ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &updb, first,
                                                (int64_t)updb.size() } );
+                            ctx->groups.push_back( upd_group{ k, b, M, N, c.lcolnum - c.fcolnum + 1, 1, side,
+                                                              first, (int64_t)updb.size() } );
                         }
                         else {
                             int64_t first = upd.size();
@@ -288,6 +314,8 @@ build_schedule( pxd_ctx_t *ctx )
                             }
                             ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &upd, first,
                                                (int64_t)upd.size() } );
+                            ctx->groups.push_back( upd_group{ k, b, M, N, c.lcolnum - c.fcolnum + 1, 0, side,
+                                                              first, (int64_t)upd.size() } );
                         }
                     }
                 }
@@ -322,12 +350,16 @@ build_schedule( pxd_ctx_t *ctx )
         lv.f1 = solv.size();
         lv.d1 = diag.size();
         lv.i1 = invt.size();
+        lv.g1 = ctx->groups.size();
     }
     ctx->nupd  = upd.size() + updb.size();
     ctx->ntrsm = trsm.size();
     ctx->nsolv = solv.size();
     ctx->stats.levels       = nlev;
     ctx->stats.update_tiles = upd.size() + updb.size();
+    ctx->hlist = list;
+    ctx->htrsm = trsm;
+    ctx->hinvt = invt;
     if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
          upload( &ctx->dupdb, updb ) ||
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
@@ -343,6 +375,89 @@ ev_ms( hipEvent_t a, hipEvent_t b )
     float ms = 0.f;
     (void)hipEventElapsedTime( &ms, a, b );
     return ms;
+}
+
+// Records of the kernel timing log from the workgroup stamps of one
+// factorization (layout of the stamp buffer: panels, inversions L and U
+// sides, TRSM tiles L and U sides, 64 x 64 update tiles, tall update tiles).
+// A panel is logged as (n, 0, 0) like cpucblk_zpotrfsp1d
+// (kernels/core_zpotrfsp.c:258); the TRSM of a cblk as (stride - n, n, 0) like
+// cpucblk_ztrsmsp (kernels/core_ztrsmsp.c:366-397), with its block-column
+// inversions counted in since they replace the reference's triangular solve;
+// a blok update as (M, N, K) like the GEMMCblk2d2d record
+// (kernels/gpu_zgemmsp.c:242-251).  A task's time is the span from its first
+// workgroup's start to its last workgroup's end.
+void
+model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_list, int64_t n_inv,
+                   int64_t n_trsm, int64_t n_upd, double sec_per_tick )
+{
+    const bool      lu    = ( ctx->facto == FACT_GETRF );
+    const int       nside = lu ? 2 : 1;
+    const uint64_t *sp    = hs.data();
+    const uint64_t *si    = sp + 2 * n_list;
+    const uint64_t *sm    = si + 4 * n_inv;
+    const uint64_t *su    = sm + 4 * n_trsm;
+    const uint64_t *sb    = su + 2 * n_upd;
+    // span of the stamped tasks [a, b) of one buffer; false when none ran
+    auto span = [&]( const uint64_t *base, int64_t a, int64_t b, double &t ) {
+        uint64_t lo = UINT64_MAX, hi = 0;
+        for ( int64_t i = a; i < b; i++ ) {
+            if ( base[2 * i + 1] == 0 ) {
+                continue;
+            }
+            lo = std::min( lo, base[2 * i] );
+            hi = std::max( hi, base[2 * i + 1] );
+        }
+        if ( hi == 0 ) {
+            return false;
+        }
+        t += (double)( hi - lo ) * sec_per_tick;
+        return true;
+    };
+    // tasks of each cblk in the inversion and TRSM lists (contiguous runs)
+    const int64_t        nc = ctx->cblknbr;
+    std::vector<int64_t> ib( nc + 1, 0 ), ie( nc + 1, 0 ), tb( nc + 1, 0 ), te( nc + 1, 0 );
+    for ( int64_t i = (int64_t)ctx->hinvt.size() - 1; i >= 0; i-- ) {
+        const int32_t k = ctx->hinvt[i].cblk;
+        if ( ie[k] == 0 ) ie[k] = i + 1;
+        ib[k] = i;
+    }
+    for ( int64_t i = (int64_t)ctx->htrsm.size() - 1; i >= 0; i-- ) {
+        const int32_t k = ctx->htrsm[i].cblk;
+        if ( te[k] == 0 ) te[k] = i + 1;
+        tb[k] = i;
+    }
+    const int kpanel = lu ? PastixKernelGETRF : ( ctx->facto == FACT_SYTRF ) ? PastixKernelSYTRF : PastixKernelPOTRF;
+    std::vector<pxd_model_entry_t> &out = ctx->model_entries;
+    out.clear();
+    for ( const level_t &lv : ctx->levels ) {
+        for ( int64_t i = lv.c0; i < lv.c1; i++ ) {
+            const int32_t k = ctx->hlist[i];
+            const dcblk  &c = ctx->hcb[k];
+            const int32_t n = c.lcolnum - c.fcolnum + 1;
+            double        t = 0.;
+            if ( span( sp, i, i + 1, t ) ) {
+                out.push_back( pxd_model_entry_t{ kpanel, n, 0, 0, t } );
+            }
+            if ( c.stride == n ) {
+                continue;
+            }
+            for ( int side = 0; side < nside; side++ ) {
+                double tt = 0.;
+                span( si + 2 * side * n_inv, ib[k], ie[k], tt );
+                if ( span( sm + 2 * side * n_trsm, tb[k], te[k], tt ) ) {
+                    out.push_back( pxd_model_entry_t{ PastixKernelTRSMCblk2d, c.stride - n, n, 0, tt } );
+                }
+            }
+        }
+        for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
+            const upd_group &gr = ctx->groups[g];
+            double           t  = 0.;
+            if ( span( gr.big ? sb : su, gr.t0, gr.t1, t ) ) {
+                out.push_back( pxd_model_entry_t{ PastixKernelGEMMCblk2d2d, gr.M, gr.N, gr.K, t } );
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -395,6 +510,32 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         ctx->stats.fill_seconds = 1e-3 * ev_ms( e0, ea );
     }
 
+    // kernel timing log: stamps of every panel, inversion, TRSM and update
+    // workgroup (LU: inversions and TRSM of both sides)
+    const bool mlog = ctx->model_log && ctx->kernels;
+    uint64_t *stamps_all = nullptr, *st_panel = nullptr, *st_inv = nullptr, *st_trsm = nullptr;
+    uint64_t *st_upd = nullptr, *st_updb = nullptr;
+    const int64_t n_list = ctx->hlist.size(), n_inv = ctx->hinvt.size(), n_trsm = ctx->htrsm.size();
+    int64_t n_upd = 0, n_updb = 0;
+    for ( const level_t &lv : ctx->levels ) {
+        n_upd  = std::max( n_upd, lv.u1 );
+        n_updb = std::max( n_updb, lv.b1 );
+    }
+    const int64_t n_st = n_list + 2 * n_inv + 2 * n_trsm + n_upd + n_updb;
+    if ( mlog ) {
+        PXD_CHECK( hipMalloc( (void **)&stamps_all, 2 * sizeof( uint64_t ) * std::max<int64_t>( 1, n_st ) ) );
+        PXD_CHECK( hipMemsetAsync( stamps_all, 0, 2 * sizeof( uint64_t ) * std::max<int64_t>( 1, n_st ), s ) );
+        st_panel = stamps_all;
+        st_inv   = st_panel + 2 * n_list;
+        st_trsm  = st_inv + 4 * n_inv;
+        st_upd   = st_trsm + 4 * n_trsm;
+        st_updb  = st_upd + 2 * n_upd;
+    }
+    // stamps of the tasks [i0, ...) of a launch (side 1: the U side's copy)
+    auto stp = [&]( uint64_t *base, int64_t i0, int side, int64_t nside ) -> uint64_t * {
+        return base ? base + 2 * ( i0 + side * nside ) : nullptr;
+    };
+
     if ( ctx->kernels ) {
         PXD_CHECK( hipMemsetAsync( DI, 0, ctx->dinvnbr * sizeof( T ), s ) );
         launches++;
@@ -411,13 +552,13 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         size_t trtri_lds = (size_t)std::max( 1, lv.maxn - PXD_NB ) * ( PXD_NB + 1 ) * sizeof( T );
         if ( lu ) {
             hipLaunchKernelGGL( ( k_getrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
-                                ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch );
+                                ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ) );
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ) );
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU, stp( st_inv, lv.i0, 1, n_inv ) );
                 launches += 2;
             }
         }
@@ -425,16 +566,17 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             const size_t sh = (size_t)std::max( 1, lv.maxn ) * ( PXD_NB + 1 ) * sizeof( T );
             if ( ldlt ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, true> ), dim3( np ), dim3( PXD_THREADS ), sh, st, ctx->dlist + lv.c0,
-                                    ctx->dcb, L, DI, thresh, ctx->dscratch );
+                                    ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ) );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
-                                    ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch );
+                                    ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
+                                    stp( st_panel, lv.c0, 0, 0 ) );
             }
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ) );
                 launches++;
             }
         }
@@ -452,14 +594,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         if ( lu ) {
             // L_off <- L_off U_kk^{-1};  U^T_off <- U^T_off L_kk^{-T}
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DU, 0 );
+                                ctx->dcb, ctx->dbk, L, (const T *)DU, 0, stp( st_trsm, lv.t0, 0, n_trsm ) );
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, U, (const T *)DI, 0 );
+                                ctx->dcb, ctx->dbk, U, (const T *)DI, 0, stp( st_trsm, lv.t0, 1, n_trsm ) );
             launches++;
         }
         else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt );
+                                ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt, stp( st_trsm, lv.t0, 0, n_trsm ) );
         }
         else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
             if ( ctx->trsm_rt == 32 ) {
@@ -479,12 +621,12 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         unsigned nb = (unsigned)( b1 - b0 );
         if ( nb ) {
             hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt );
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ) );
             launches++;
         }
         if ( nu && ctx->kernels ) {
             hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
-                                ctx->dcb, ctx->dbk, L, U, ldlt );
+                                ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_upd, u0, 0, 0 ) );
             launches++;
         }
         else if ( nu ) {
@@ -496,7 +638,24 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         }
     };
 
-    if ( ctx->timing ) {
+    if ( mlog ) {
+        // one stream; every blok update is a launch of its own, so that its
+        // workgroups' times are those of the update alone on the device
+        for ( const level_t &lv : ctx->levels ) {
+            panel( lv, s );
+            trsm( lv, s );
+            for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
+                const upd_group &gr = ctx->groups[g];
+                if ( gr.big ) {
+                    update( 0, 0, gr.t0, gr.t1, s );
+                }
+                else {
+                    update( gr.t0, gr.t1, 0, 0, s );
+                }
+            }
+        }
+    }
+    else if ( ctx->timing ) {
         // one stream, every phase timed on its own
         for ( const level_t &lv : ctx->levels ) {
             PXD_CHECK( hipEventRecord( ea, s ) );
@@ -546,6 +705,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     int h[4];
     PXD_CHECK( hipMemcpy( h, ctx->dscratch, 4 * sizeof( int ), hipMemcpyDeviceToHost ) );
     *nbpivots = h[0];
+    if ( mlog ) {
+        std::vector<uint64_t> hs( 2 * n_st );
+        PXD_CHECK( hipMemcpy( hs.data(), stamps_all, hs.size() * sizeof( uint64_t ), hipMemcpyDeviceToHost ) );
+        PXD_CHECK( hipFree( stamps_all ) );
+        int rate_khz = 0;
+        PXD_CHECK( hipDeviceGetAttribute( &rate_khz, hipDeviceAttributeWallClockRate, ctx->device ) );
+        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, 1. / ( 1e3 * std::max( 1, rate_khz ) ) );
+    }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
     (void)hipEventDestroy( ea );
@@ -812,9 +979,26 @@ pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats )
 }
 
 int
+pxd_model_log( const pxd_ctx_t *ctx, pxd_model_entry_t *entries, int64_t max, int64_t *count )
+{
+    const int64_t n = (int64_t)ctx->model_entries.size();
+    if ( count ) {
+        *count = n;
+    }
+    if ( entries && max > 0 ) {
+        std::memcpy( entries, ctx->model_entries.data(), sizeof( pxd_model_entry_t ) * std::min( n, max ) );
+    }
+    return 0;
+}
+
+int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
+    if ( k == "model_log" ) {
+        ctx->model_log = ( value != 0. );
+        return 0;
+    }
     if ( k == "timing" ) {
         ctx->timing = ( value != 0. );
         return 0;

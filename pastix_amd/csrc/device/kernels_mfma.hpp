@@ -17,6 +17,26 @@ struct inv_task {
     int32_t J;
 };
 
+// Per-workgroup timing for the kernel timing log: when stamps is not null,
+// task i of the launch writes its start and end times (constant-rate wall
+// clock ticks) to stamps[2i] and stamps[2i+1].
+__device__ inline uint64_t
+stamp_begin( const uint64_t *stamps )
+{
+    return stamps ? (uint64_t)wall_clock64() : 0;
+}
+__device__ inline void
+stamp_end( uint64_t *stamps, int64_t i, uint64_t t0 )
+{
+    if ( stamps ) {
+        __syncthreads();
+        if ( threadIdx.x == 0 ) {
+            stamps[2 * i]     = t0;
+            stamps[2 * i + 1] = (uint64_t)wall_clock64();
+        }
+    }
+}
+
 // Largest i in [0, n) with a[i] <= key (a ascending, a[0] <= key).
 __device__ inline int32_t
 lds_search_le( const int32_t *a, int32_t n, int32_t key )
@@ -76,7 +96,8 @@ search_grow( const search_keys &sk, const dblok *__restrict__ bk, int32_t ffb, i
 template <typename T, bool LDLT>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L,
-          T *__restrict__ Linv, double thresh, int *__restrict__ npiv )
+          T *__restrict__ Linv, double thresh, int *__restrict__ npiv,
+          uint64_t *__restrict__ stamps )
 {
     __shared__ T sA[PXD_NB][PXD_NB + 1];   // sA[col][row]
     __shared__ T sI[PXD_NB][PXD_NB + 1];   // inverse, sI[col][row]
@@ -87,6 +108,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     T ( *sP )[PXD_NB + 1] = reinterpret_cast<T ( * )[PXD_NB + 1]>( smem_raw );
 
     const int32_t k   = list[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const dcblk   c   = cb[k];
     const int32_t n   = c.lcolnum - c.fcolnum + 1;
     const int     tid = threadIdx.x;
@@ -309,6 +331,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     if ( tid == 0 && piv > 0 ) {
         atomicAdd( npiv, piv );
     }
+    stamp_end( stamps, blockIdx.x, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -362,7 +385,8 @@ small_trsm_mfma( T *base, int64_t rs, int64_t ks, int32_t rows, int32_t bs, cons
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L, T *__restrict__ U,
-          T *__restrict__ DL, T *__restrict__ DU, double thresh, int *__restrict__ npiv )
+          T *__restrict__ DL, T *__restrict__ DU, double thresh, int *__restrict__ npiv,
+          uint64_t *__restrict__ stamps )
 {
     __shared__ T sA[PXD_NB][PXD_NB + 1];    // sA[col][row]
     __shared__ T sIL[PXD_NB][PXD_NB + 1];   // L11^{-1}, [col][row]
@@ -372,6 +396,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     __shared__ T sPj[PXD_NB][64 + 1];
 
     const int32_t k   = list[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const dcblk   c   = cb[k];
     const int32_t n   = c.lcolnum - c.fcolnum + 1;
     const int     tid = threadIdx.x;
@@ -572,6 +597,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     if ( tid == 0 && piv > 0 ) {
         atomicAdd( npiv, piv );
     }
+    stamp_end( stamps, blockIdx.x, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +609,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ L,
-         T *__restrict__ Linv )
+         T *__restrict__ Linv, uint64_t *__restrict__ stamps )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -593,6 +619,7 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
     T ( *sR )[PXD_NB + 1] = reinterpret_cast<T ( * )[PXD_NB + 1]>( smem_raw );   // rows below block J
 
     const inv_task t    = tasks[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const dcblk    c    = cb[t.cblk];
     const int32_t  n    = c.lcolnum - c.fcolnum + 1;
     const int      tid  = threadIdx.x;
@@ -688,6 +715,7 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
             sweep( K + PXD_NB - r0, K, bk );
         }
     }
+    stamp_end( stamps, blockIdx.x, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -699,7 +727,8 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-             const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv, int ldlt )
+             const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv, int ldlt,
+             uint64_t *__restrict__ stamps )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -711,6 +740,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ T       sDi[PXD_TRSM_NT * 16];   // LDL^T: 1 / D of the cblk
 
     const row_task t    = tasks[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const dcblk    c    = cb[t.cblk];
     const int32_t  n    = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb   = cb[t.cblk + 1].fblok;
@@ -811,6 +841,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
         }
         __syncthreads();   // the pass's stores precede the next pass's loads
     }
+    stamp_end( stamps, blockIdx.x, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -821,7 +852,8 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-               const dblok *__restrict__ bk, T *L, T *U, int ldlt )
+               const dblok *__restrict__ bk, T *L, T *U, int ldlt,
+               uint64_t *__restrict__ stamps )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -836,6 +868,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ T           sDk[PXD_TRSM_NT * 16];   // LDL^T: D of the source cblk
 
     const upd_task t    = tasks[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const int      tid  = threadIdx.x;
     const int      lane = tid & 63, w = tid >> 6;
     const dcblk    c    = cb[t.cblk];
@@ -975,6 +1008,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                     if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - M::get( acc[i][j], v );
                 }
     }
+    stamp_end( stamps, blockIdx.x, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,7 +1074,8 @@ template <> struct upd_rows<pxcd>   { static constexpr int value = 64; };
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS, 3 )
 k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
-              const dblok *__restrict__ bk, T *L, T *U, int ldlt )
+              const dblok *__restrict__ bk, T *L, T *U, int ldlt,
+              uint64_t *__restrict__ stamps )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -1062,7 +1097,9 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     __shared__ search_keys sk;
     __shared__ T           sDk[PXD_TRSM_NT * 16];   // LDL^T: D of the source cblk
 
-    const upd_task t    = tasks[xcd_task( blockIdx.x, ntasks )];
+    const int32_t  ti   = xcd_task( blockIdx.x, ntasks );
+    const upd_task t    = tasks[ti];
+    const uint64_t t_stamp = stamp_begin( stamps );
     const int      tid  = threadIdx.x;
     const int      lane = tid & 63, w = tid >> 6;
     const dcblk    c    = cb[t.cblk];
@@ -1229,6 +1266,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
         }
         __syncthreads();
     }
+    stamp_end( stamps, ti, t_stamp );
 }
 
 // ---------------------------------------------------------------------------

@@ -136,6 +136,7 @@ pastixInitWithAffinity( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_in
     if ( iparm[IPARM_THREAD_NBR] <= 0 ) {
         iparm[IPARM_THREAD_NBR] = pd->nthreads;
     }
+    pastixModelsLoad( pd );   /* common/api.c:854 */
     *pastix_data = pd;
 }
 
@@ -185,6 +186,8 @@ pastixFinalize( pastix_data_t **pastix_data )
     if ( pd->dev ) {
         pxd_destroy( pd->dev );
     }
+    pastixModelsFree( pd->cpu_models );
+    pastixModelsFree( pd->gpu_models );
     free( pd->dir_global );
     free( pd );
     *pastix_data = NULL;

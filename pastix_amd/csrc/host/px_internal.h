@@ -192,6 +192,8 @@ struct pastix_data_s {
     char            *dir_global;
     double           berr_target;        /* >0: refinement stops on backward error */
     int              refine_nrhs_batch;   /* 1: reference behaviour (one RHS at a time) */
+    pastix_model_t  *cpu_models;          /* kernel cost models (common/pastixdata.h:106-107) */
+    pastix_model_t  *gpu_models;
 };
 
 int px_device_ready( pastix_data_t *pd );

@@ -183,3 +183,50 @@ def dense_from_factor(solver, L):
         fr = bk[b, 0]
         D[fr:fr + t.shape[0], fc:fc + t.shape[1]] = t
     return np.tril(D)
+
+
+# ---------------------------------------------------------------------------
+# Learned cost model: float64 restatement of the perceptron whose forward and
+# backward passes run on the GPU (8 inputs, two tanh layers of 32 units, one
+# linear output; parameters W1[8][32], b1, W2[32][32], b2, w3, b3 in one array).
+
+MLP_F, MLP_H = 8, 32
+
+
+def mlp_unpack(p):
+    p = np.asarray(p, dtype=np.float64)
+    F, H = MLP_F, MLP_H
+    o = 0
+    W1 = p[o:o + F * H].reshape(F, H); o += F * H
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    w3 = p[o:o + H]; o += H
+    b3 = p[o]; o += 1
+    assert o == p.size
+    return W1, b1, W2, b2, w3, b3
+
+
+def mlp_forward(p, X):
+    W1, b1, W2, b2, w3, b3 = mlp_unpack(p)
+    h1 = np.tanh(X @ W1 + b1)
+    h2 = np.tanh(h1 @ W2 + b2)
+    return h2 @ w3 + b3
+
+
+def mlp_loss_grad(p, X, t):
+    """Mean squared error sum_r (y_r - t_r)^2 / n and its gradient."""
+    X = np.asarray(X, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    n = X.shape[0]
+    W1, b1, W2, b2, w3, b3 = mlp_unpack(p)
+    h1 = np.tanh(X @ W1 + b1)
+    h2 = np.tanh(h1 @ W2 + b2)
+    y = h2 @ w3 + b3
+    e = y - t
+    dy = 2.0 * e / n
+    g2 = np.outer(dy, w3) * (1.0 - h2 ** 2)
+    g1 = (g2 @ W2.T) * (1.0 - h1 ** 2)
+    grad = np.concatenate([(X.T @ g1).ravel(), g1.sum(0), (h1.T @ g2).ravel(), g2.sum(0),
+                           h2.T @ dy, [dy.sum()]])
+    return float((e ** 2).sum() / n), grad

@@ -9,14 +9,14 @@ import pastix_amd as px
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = ["include/pastix.h", "include/pastix/order.h", "include/spm.h",
-           "include/pastix_amd_device.h", "include/pastix_amd_engine.h"]
+           "include/pastix_amd_device.h", "include/pastix_amd_engine.h", "include/pastix_amd_models.h"]
 
 
 def declared_functions(path):
     text = open(os.path.join(ROOT, path)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"#.*", "", text)
-    names = re.findall(r"^\s*(?:const\s+)?(?:int|void|double|size_t|char\s*\*|void\s*\*)\s*\**\s*"
+    names = re.findall(r"^\s*(?:const\s+)?(?:int|void|double|size_t|char\s*\*|void\s*\*|pastix_model_t)\s*\**\s*"
                        r"([A-Za-z_]\w*)\s*\(", text, flags=re.M)
     return sorted(set(names))
 
