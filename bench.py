@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=60, help="grid edge of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
+                    help="engine option for both paths (e.g. prio=0, upd_big_rows=256)")
     ap.add_argument("--profile-stats", action="store_true", help="per-kernel-class event timing pass")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the timed region, one device copy of this many MiB (calibrates the "
@@ -55,7 +57,7 @@ def log(*a):
 class Run:
     """One analysed problem with the matrix and the RHS resident on the device."""
 
-    def __init__(self, px, spm, n, nrhs, mixed, berr):
+    def __init__(self, px, spm, n, nrhs, mixed, berr, opts=()):
         self.px = px
         self.spm = spm
         iparm, dparm = px.initParam()
@@ -73,6 +75,9 @@ class Run:
         if rc:
             raise RuntimeError("bcsc upload failed (%d)" % rc)
         px.set_option(self.pd, "berr_target", berr)
+        for kv in opts:
+            k, v = kv.split("=")
+            px.set_option(self.pd, k, float(v))
         self.nrhs = nrhs
         self.N = spm.n
         self.x_true, b = spm.gen_rhs(nrhs, seed=0)
@@ -202,7 +207,7 @@ def main():
         if label == "fp64" and args.no_fp64:
             continue
         log("step: analyse + upload (%s)" % label)
-        run = Run(px, spm, n, args.nrhs, mixed, args.berr)
+        run = Run(px, spm, n, args.nrhs, mixed, args.berr, args.opt)
         log("step: analysis %.2f s; timing %d steps (%s)" % (run.t_analyze, args.steps, label))
         r = measure(run, args.steps, args.warmup)
         r["thflops"] = run.dparm[px.DPARM_FACT_THFLOPS]

@@ -121,6 +121,9 @@ struct pxd_ctx_s {
 
     // options / stats
     int          timing = 0;
+    // 1: panels, inverses, TRSM and the updates into the next level raise
+    // their waves' issue priority (s_setprio) over the bulk updates
+    int          prio = 1;
     // 1: the factorization runs one stream, launches every blok update on
     // its own and records per-workgroup times into model_entries
     int          model_log = 0;
@@ -552,13 +555,13 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         size_t trtri_lds = (size_t)std::max( 1, lv.maxn - PXD_NB ) * ( PXD_NB + 1 ) * sizeof( T );
         if ( lu ) {
             hipLaunchKernelGGL( ( k_getrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
-                                ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ) );
+                                ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ) );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio );
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU, stp( st_inv, lv.i0, 1, n_inv ) );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU, stp( st_inv, lv.i0, 1, n_inv ), ctx->prio );
                 launches += 2;
             }
         }
@@ -566,17 +569,17 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             const size_t sh = (size_t)std::max( 1, lv.maxn ) * ( PXD_NB + 1 ) * sizeof( T );
             if ( ldlt ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, true> ), dim3( np ), dim3( PXD_THREADS ), sh, st, ctx->dlist + lv.c0,
-                                    ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ) );
+                                    ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
                                     ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
-                                    stp( st_panel, lv.c0, 0, 0 ) );
+                                    stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
             }
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ) );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio );
                 launches++;
             }
         }
@@ -594,14 +597,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         if ( lu ) {
             // L_off <- L_off U_kk^{-1};  U^T_off <- U^T_off L_kk^{-T}
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DU, 0, stp( st_trsm, lv.t0, 0, n_trsm ) );
+                                ctx->dcb, ctx->dbk, L, (const T *)DU, 0, stp( st_trsm, lv.t0, 0, n_trsm ), ctx->prio );
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, U, (const T *)DI, 0, stp( st_trsm, lv.t0, 1, n_trsm ) );
+                                ctx->dcb, ctx->dbk, U, (const T *)DI, 0, stp( st_trsm, lv.t0, 1, n_trsm ), ctx->prio );
             launches++;
         }
         else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt, stp( st_trsm, lv.t0, 0, n_trsm ) );
+                                ctx->dcb, ctx->dbk, L, (const T *)DI, ldlt, stp( st_trsm, lv.t0, 0, n_trsm ), ctx->prio );
         }
         else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
             if ( ctx->trsm_rt == 32 ) {
@@ -616,17 +619,17 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         launches++;
     };
     // update tiles [u0, u1) of the 64 x 64 list and [b0, b1) of the tall list
-    auto update = [&]( int64_t u0, int64_t u1, int64_t b0, int64_t b1, hipStream_t st ) {
+    auto update = [&]( int64_t u0, int64_t u1, int64_t b0, int64_t b1, hipStream_t st, int prio ) {
         unsigned nu = (unsigned)( u1 - u0 );
         unsigned nb = (unsigned)( b1 - b0 );
         if ( nb ) {
             hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ) );
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
             launches++;
         }
         if ( nu && ctx->kernels ) {
             hipLaunchKernelGGL( ( k_update_mfma<T> ), dim3( nu ), dim3( PXD_THREADS ), 0, st, ctx->dupd + u0,
-                                ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_upd, u0, 0, 0 ) );
+                                ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_upd, u0, 0, 0 ), prio );
             launches++;
         }
         else if ( nu ) {
@@ -647,10 +650,10 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
                 const upd_group &gr = ctx->groups[g];
                 if ( gr.big ) {
-                    update( 0, 0, gr.t0, gr.t1, s );
+                    update( 0, 0, gr.t0, gr.t1, s, 0 );
                 }
                 else {
-                    update( gr.t0, gr.t1, 0, 0, s );
+                    update( gr.t0, gr.t1, 0, 0, s, 0 );
                 }
             }
         }
@@ -669,7 +672,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_trsm += 1e-3 * ev_ms( ea, eb );
             PXD_CHECK( hipEventRecord( ea, s ) );
-            update( lv.u0, lv.u1, lv.b0, lv.b1, s );
+            update( lv.u0, lv.u1, lv.b0, lv.b1, s, 0 );
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_upd += 1e-3 * ev_ms( ea, eb );
@@ -689,9 +692,9 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             trsm( lv, s2 );
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
             PXD_CHECK( hipStreamWaitEvent( s, paneldone, 0 ) );
-            update( lv.u0, lv.uc, lv.b0, lv.bc, s );
+            update( lv.u0, lv.uc, lv.b0, lv.bc, s, ctx->prio );
             PXD_CHECK( hipEventRecord( ready, s ) );
-            update( lv.uc, lv.u1, lv.bc, lv.b1, s );
+            update( lv.uc, lv.u1, lv.bc, lv.b1, s, 0 );
         }
     }
     PXD_CHECK( hipGetLastError() );
@@ -995,6 +998,10 @@ int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
+    if ( k == "prio" ) {
+        ctx->prio = ( value != 0. );
+        return 0;
+    }
     if ( k == "model_log" ) {
         ctx->model_log = ( value != 0. );
         return 0;

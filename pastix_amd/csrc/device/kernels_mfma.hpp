@@ -17,6 +17,18 @@ struct inv_task {
     int32_t J;
 };
 
+// Waves of a kernel on the critical path of the factorization (panels,
+// inverses, TRSM and the updates that the next level waits for) raise their
+// instruction-issue priority, so that they keep their pace on CUs that they
+// share with the bulk update kernels of the other stream.
+__device__ inline void
+wave_priority( int prio )
+{
+    if ( prio ) {
+        __builtin_amdgcn_s_setprio( 3 );
+    }
+}
+
 // Per-workgroup timing for the kernel timing log: when stamps is not null,
 // task i of the launch writes its start and end times (constant-rate wall
 // clock ticks) to stamps[2i] and stamps[2i+1].
@@ -97,7 +109,7 @@ template <typename T, bool LDLT>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L,
           T *__restrict__ Linv, double thresh, int *__restrict__ npiv,
-          uint64_t *__restrict__ stamps )
+          uint64_t *__restrict__ stamps, int prio )
 {
     __shared__ T sA[PXD_NB][PXD_NB + 1];   // sA[col][row]
     __shared__ T sI[PXD_NB][PXD_NB + 1];   // inverse, sI[col][row]
@@ -109,6 +121,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 
     const int32_t k   = list[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const dcblk   c   = cb[k];
     const int32_t n   = c.lcolnum - c.fcolnum + 1;
     const int     tid = threadIdx.x;
@@ -241,11 +254,17 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                 const int32_t rr = rb + M::row( lane, v );
                 const int     c0 = lane & 15, c1 = 16 + ( lane & 15 );
                 if ( rr < m ) {
-                    // LDL^T: the product is L21 D; L21 = product / D
+                    // LDL^T: the product is L21 D; L21 = product / D goes below
+                    // the block and the product itself, transposed, to the
+                    // block's rows right of it (D L^T, core_zsytrfsp.c:176-186)
                     sP[rr][c0] = M::get( acc0, v );   // zero beyond bs: sI is zero there
                     sP[rr][c1] = M::get( acc1, v );
                     if ( c0 < bs ) A[( m0 + rr ) + (int64_t)( kb + c0 ) * n] = LDLT ? M::get( acc0, v ) * sDi[c0] : M::get( acc0, v );
                     if ( c1 < bs ) A[( m0 + rr ) + (int64_t)( kb + c1 ) * n] = LDLT ? M::get( acc1, v ) * sDi[c1] : M::get( acc1, v );
+                    if constexpr ( LDLT ) {
+                        if ( c0 < bs ) A[( kb + c0 ) + (int64_t)( m0 + rr ) * n] = M::get( acc0, v );
+                        if ( c1 < bs ) A[( kb + c1 ) + (int64_t)( m0 + rr ) * n] = M::get( acc1, v );
+                    }
                 }
             }
         }
@@ -386,7 +405,7 @@ template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L, T *__restrict__ U,
           T *__restrict__ DL, T *__restrict__ DU, double thresh, int *__restrict__ npiv,
-          uint64_t *__restrict__ stamps )
+          uint64_t *__restrict__ stamps, int prio )
 {
     __shared__ T sA[PXD_NB][PXD_NB + 1];    // sA[col][row]
     __shared__ T sIL[PXD_NB][PXD_NB + 1];   // L11^{-1}, [col][row]
@@ -397,6 +416,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 
     const int32_t k   = list[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const dcblk   c   = cb[k];
     const int32_t n   = c.lcolnum - c.fcolnum + 1;
     const int     tid = threadIdx.x;
@@ -609,7 +629,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ L,
-         T *__restrict__ Linv, uint64_t *__restrict__ stamps )
+         T *__restrict__ Linv, uint64_t *__restrict__ stamps, int prio )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -620,6 +640,7 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 
     const inv_task t    = tasks[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const dcblk    c    = cb[t.cblk];
     const int32_t  n    = c.lcolnum - c.fcolnum + 1;
     const int      tid  = threadIdx.x;
@@ -728,7 +749,7 @@ template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
              const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv, int ldlt,
-             uint64_t *__restrict__ stamps )
+             uint64_t *__restrict__ stamps, int prio )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -741,6 +762,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 
     const row_task t    = tasks[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const dcblk    c    = cb[t.cblk];
     const int32_t  n    = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb   = cb[t.cblk + 1].fblok;
@@ -853,7 +875,7 @@ template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                const dblok *__restrict__ bk, T *L, T *U, int ldlt,
-               uint64_t *__restrict__ stamps )
+               uint64_t *__restrict__ stamps, int prio )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -869,6 +891,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 
     const upd_task t    = tasks[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const int      tid  = threadIdx.x;
     const int      lane = tid & 63, w = tid >> 6;
     const dcblk    c    = cb[t.cblk];
@@ -1075,7 +1098,7 @@ template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS, 3 )
 k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
               const dblok *__restrict__ bk, T *L, T *U, int ldlt,
-              uint64_t *__restrict__ stamps )
+              uint64_t *__restrict__ stamps, int prio )
 {
     typedef mfma16<T>          M;
     typedef typename M::acc_t  acc_t;
@@ -1100,6 +1123,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const int32_t  ti   = xcd_task( blockIdx.x, ntasks );
     const upd_task t    = tasks[ti];
     const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
     const int      tid  = threadIdx.x;
     const int      lane = tid & 63, w = tid >> 6;
     const dcblk    c    = cb[t.cblk];

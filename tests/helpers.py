@@ -13,6 +13,20 @@ def make_problem(kind, nx, ny=None, nz=None, **kw):
         return px.Spm.convdiff27(nx, ny, nz, 1.0, 0.5, kw.get("seed", 0))
     if kind == "helmholtz7":
         return px.Spm.helmholtz7(nx, ny, nz, 0.5, 0.1)
+    if kind == "lap7sd":
+        # symmetric indefinite and diagonally dominant: the 7-point stencil
+        # with diagonal +7 / -7 on even / odd vertices (plus a complex part
+        # when asked).  LDL^T without pivoting keeps every Schur complement
+        # diagonally dominant, so the factor has no element growth and two
+        # implementations agree to rounding.
+        base = px.Spm.laplacian7(nx, ny, nz, 6.0, 1.0)
+        cp, rp, v = base.arrays()
+        v = v.astype(np.complex128) if kw.get("complex_") else v.copy()
+        for j in range(nx * ny * nz):
+            for p in range(cp[j], cp[j + 1]):
+                if rp[p] == j:
+                    v[p] = (7.0 if j % 2 == 0 else -7.0) + (0.3j if kw.get("complex_") else 0.0)
+        return px.Spm.from_csc(nx * ny * nz, cp, rp, v, mtxtype=base.s.mtxtype)
     raise ValueError(kind)
 
 
