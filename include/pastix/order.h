@@ -10,6 +10,7 @@
 
 #include <stdint.h>
 #include <stdio.h>
+#include "spm.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -45,6 +46,9 @@ void pastixOrderExit ( pastix_order_t *ordeptr );
 void pastixOrderBase ( pastix_order_t *ordeptr, pastix_int_t baseval );
 int  pastixOrderCheck( const pastix_order_t *ordeptr );
 int  pastixOrderCopy ( pastix_order_t *ordedst, const pastix_order_t *ordesrc );
+/* Expand a node ordering to the spm->dof unknowns of each node (reference
+ * order/order.c:396-469; constant dof only). */
+void pastixOrderExpand( pastix_order_t *ordeptr, const spmatrix_t *spm );
 
 /* Nested dissection of an nx x ny x nz grid (reference order/order_grids.c:306). */
 int  pastixOrderGrid ( pastix_order_t **myorder, pastix_int_t nx, pastix_int_t ny, pastix_int_t nz );

@@ -106,6 +106,7 @@ _sig("pastix_task_solve_and_refine", _I, _p, pastix_int, pastix_int, _p, pastix_
 _sig("pastixOrderGrid", _I, ctypes.POINTER(ctypes.POINTER(pastix_order_t)), pastix_int, pastix_int, pastix_int)
 _sig("pastixOrderExit", None, ctypes.POINTER(pastix_order_t))
 _sig("pastixOrderCheck", _I, ctypes.POINTER(pastix_order_t))
+_sig("pastixOrderExpand", None, ctypes.POINTER(pastix_order_t), ctypes.POINTER(spmatrix_t))
 _sig("pastixOrderSaveFile", _I, ctypes.POINTER(pastix_order_t), ctypes.c_char_p)
 _sig("pastixOrderLoadFile", _I, ctypes.POINTER(pastix_order_t), ctypes.c_char_p)
 _sig("spmInit", None, ctypes.POINTER(spmatrix_t))
@@ -306,6 +307,14 @@ class Order:
         _check("pastixOrderGrid", lib.pastixOrderGrid(ctypes.byref(o.ptr), nx, ny, nz))
         return o
 
+    def expand(self, dof):
+        """Expand a node ordering to dof unknowns per node (pastixOrderExpand)."""
+        stub = spmatrix_t()
+        stub.dof = dof
+        stub.baseval = 0
+        lib.pastixOrderExpand(self.ptr, ctypes.byref(stub))
+        return self
+
     def arrays(self):
         o = self.ptr.contents
         n, c = o.vertnbr, o.cblknbr
@@ -372,11 +381,14 @@ def subtask_blend(pastix_data):
     _check("pastix_subtask_blend", lib.pastix_subtask_blend(pastix_data))
 
 
-def analyze_grid(pastix_data, spm, nx, ny, nz):
+def analyze_grid(pastix_data, spm, nx, ny, nz, dof=1):
     """Analysis with the grid nested dissection given as a personal ordering,
-    the sequence of the reference's example/ordering_grid.c:107-123."""
-    iparm = None
+    the sequence of the reference's example/ordering_grid.c:107-123.  With
+    dof > 1 the matrix holds dof unknowns per grid node (numbered node-major)
+    and the node ordering is expanded to them (pastixOrderExpand)."""
     order = Order.grid(nx, ny, nz)
+    if dof > 1:
+        order.expand(dof)
     subtask_order(pastix_data, spm, order)
     subtask_symbfact(pastix_data)
     subtask_reordering(pastix_data)

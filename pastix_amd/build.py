@@ -6,7 +6,7 @@
   benchmark's CPU baseline (plain C + pthreads; never linked into the
   product).
 
-Run ``python -m pastix_amd.build`` (or ``__graft_entry__.build()``).  Objects
+Run ``python pastix_amd/build.py`` (or ``__graft_entry__.build()``).  Objects
 go to ``_build/`` (git-ignored); the .so files stay in-tree so that they
 travel with the repository snapshot.
 """

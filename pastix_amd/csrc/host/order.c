@@ -213,6 +213,55 @@ pastixOrderCopy( pastix_order_t *dst, const pastix_order_t *src )
     return PASTIX_SUCCESS;
 }
 
+/*
+ * Expand an ordering of the nodes of a matrix with spm->dof unknowns per
+ * node to its unknowns (reference order/order.c:396-469): node v at position
+ * i becomes the spm->dof consecutive positions of its unknowns dof*v ..
+ * dof*v+dof-1, so every supernode range, before and after splitting, is
+ * scaled by dof; the tree is unchanged.  The ordering ends with base 0, as
+ * the reference's does.  Variable dof (spm->dof <= 0) is not supported here.
+ */
+void
+pastixOrderExpand( pastix_order_t *ordeptr, const spmatrix_t *spm )
+{
+    pastix_int_t  dof = spm->dof, nv, n, i, j;
+    pastix_int_t *peri;
+    if ( dof <= 0 ) {
+        px_print_error( "pastixOrderExpand: variable dof is not supported" );
+        return;
+    }
+    pastixOrderBase( ordeptr, 0 );
+    if ( dof == 1 ) {
+        return;
+    }
+    nv   = ordeptr->vertnbr;
+    n    = nv * dof;
+    peri = px_malloc( n * sizeof( pastix_int_t ) );
+    for ( i = 0; i < nv; i++ ) {
+        for ( j = 0; j < dof; j++ ) {
+            peri[i * dof + j] = ordeptr->peritab[i] * dof + j;
+        }
+    }
+    free( ordeptr->peritab );
+    ordeptr->peritab = peri;
+    free( ordeptr->permtab );
+    ordeptr->permtab = px_malloc( n * sizeof( pastix_int_t ) );
+    for ( i = 0; i < n; i++ ) {
+        ordeptr->permtab[peri[i]] = i;
+    }
+    if ( ordeptr->rangtab ) {
+        for ( i = 0; i <= ordeptr->cblknbr; i++ ) {
+            ordeptr->rangtab[i] *= dof;
+        }
+    }
+    if ( ordeptr->sndetab ) {
+        for ( i = 0; i <= ordeptr->sndenbr; i++ ) {
+            ordeptr->sndetab[i] *= dof;
+        }
+    }
+    ordeptr->vertnbr = n;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Grid nested dissection.                                                  */
 

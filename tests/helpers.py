@@ -13,6 +13,9 @@ def make_problem(kind, nx, ny=None, nz=None, **kw):
         return px.Spm.convdiff27(nx, ny, nz, 1.0, 0.5, kw.get("seed", 0))
     if kind == "helmholtz7":
         return px.Spm.helmholtz7(nx, ny, nz, 0.5, 0.1)
+    if kind == "vlap7":
+        # configuration 5: dof x dof blocks per node, lognormal coefficients
+        return px.Spm.vlaplacian7(nx, ny, nz, kw.get("dof", 3), kw.get("sigma", 1.0), kw.get("seed", 0))
     if kind == "lap7sd":
         # symmetric indefinite and diagonally dominant: the 7-point stencil
         # with diagonal +7 / -7 on even / odd vertices (plus a complex part
@@ -30,7 +33,7 @@ def make_problem(kind, nx, ny=None, nz=None, **kw):
     raise ValueError(kind)
 
 
-def analyzed(spm, nx, ny=None, nz=None, facto=None, mixed=0, extra=None):
+def analyzed(spm, nx, ny=None, nz=None, facto=None, mixed=0, extra=None, dof=1):
     """pastix_data after the grid analysis (example/ordering_grid.c flow)."""
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
@@ -42,7 +45,7 @@ def analyzed(spm, nx, ny=None, nz=None, facto=None, mixed=0, extra=None):
     for k, v in (extra or {}).items():
         iparm[k] = v
     pd = px.init(iparm, dparm)
-    px.analyze_grid(pd, spm, nx, ny, nz)
+    px.analyze_grid(pd, spm, nx, ny, nz, dof=dof)
     return pd, iparm, dparm
 
 

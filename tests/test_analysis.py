@@ -272,3 +272,34 @@ def test_level_order_puts_children_before_fathers():
     for k, f in enumerate(tree):
         assert f == -1 or f > k
     px.finalize(pd)
+
+
+def test_order_expand_for_multiple_dof():
+    """pastixOrderExpand (reference order/order.c:396-469): every node's dof
+    unknowns take consecutive positions in the node's place, ranges scale by
+    dof, the tree is kept; and the analysis of the expanded matrix is the
+    node analysis with every column and row range scaled by dof (what the
+    reference's symbol expansion gives) while no cblk needs a split."""
+    from helpers import make_problem, analyzed
+    o = px.Order.grid(6, 5, 4)
+    p0, i0, r0, t0 = o.arrays()
+    o.expand(3)
+    p, i, r, t = o.arrays()
+    assert np.array_equal(i, (3 * i0[:, None] + np.arange(3)).ravel())
+    assert np.array_equal(p[i], np.arange(len(i)))
+    assert np.array_equal(r, 3 * r0) and np.array_equal(t, t0)
+    assert px.lib.pastixOrderCheck(o.ptr) == 0
+    for nx in (6, 8):
+        a = make_problem("lap7", nx)
+        pa, _, _ = analyzed(a, nx)
+        b = make_problem("vlap7", nx)
+        pb, _, _ = analyzed(b, nx, dof=3)
+        sa, sb = px.solver_export(pa), px.solver_export(pb)
+        assert sb["nodenbr"] == 3 * sa["nodenbr"] and sb["cblknbr"] == sa["cblknbr"]
+        ca, cb = sa["cblk"].reshape(-1, 6), sb["cblk"].reshape(-1, 6)
+        assert np.array_equal(cb[:-1, 0], 3 * ca[:-1, 0]) and np.array_equal(cb[:-1, 1] + 1, 3 * (ca[:-1, 1] + 1))
+        ba, bb = sa["blok"].reshape(-1, 5), sb["blok"].reshape(-1, 5)
+        assert np.array_equal(bb[:, 0], 3 * ba[:, 0]) and np.array_equal(bb[:, 1] + 1, 3 * (ba[:, 1] + 1))
+        assert np.array_equal(bb[:, 2:4], ba[:, 2:4])
+        px.finalize(pa)
+        px.finalize(pb)
