@@ -29,6 +29,16 @@ wave_priority( int prio )
     }
 }
 
+// Row i >= column j of entry idx of a row-major packed lower triangle.
+__device__ inline void
+tri_index( int idx, int &i, int &j )
+{
+    i = (int)( ( sqrtf( 8.f * (float)idx + 1.f ) - 1.f ) * 0.5f );
+    if ( ( i + 1 ) * ( i + 2 ) / 2 <= idx ) i++;
+    if ( i * ( i + 1 ) / 2 > idx ) i--;
+    j = idx - i * ( i + 1 ) / 2;
+}
+
 // Per-workgroup timing for the kernel timing log: when stamps is not null,
 // task i of the launch writes its start and end times (constant-rate wall
 // clock ticks) to stamps[2i] and stamps[2i+1].
@@ -146,11 +156,15 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             for ( int u = 0; u < NU; u++ ) {
                 const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
                 sA[cc][r] = v[u];
-                sI[cc][r] = (T)0;
+                sI[cc][r] = ( r == cc && r < bs ) ? (T)1 : (T)0;   // W = I
             }
         }
         __syncthreads();
 
+        // unblocked elimination A = U D U^T (U unit lower, multipliers
+        // u_rc = a_rc / d_c).  The same row operations applied to W = I leave
+        // W = U^{-1} (sI[col][row]): step cc subtracts u_r,cc * W[cc][:] from
+        // every row r > cc, in the same barrier step as the trailing update.
         for ( int cc = 0; cc < bs; cc++ ) {
             T d = sA[cc][cc];
             if ( px_abs( d ) < tau ) {
@@ -160,12 +174,19 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                     piv++;
                 }
             }
-            const int m = bs - cc - 1;
-            const T   r = (T)1 / d;
-            for ( int idx = tid; idx < m * m; idx += PXD_THREADS ) {
-                int rr = cc + 1 + idx % m, c2 = cc + 1 + idx / m;
-                if ( rr >= c2 ) {
+            const int m   = bs - cc - 1;
+            const int nsq = m * ( m + 1 ) / 2;   // lower triangle of the trailing block
+            const T   r   = (T)1 / d;
+            for ( int idx = tid; idx < nsq + m * ( cc + 1 ); idx += PXD_THREADS ) {
+                if ( idx < nsq ) {
+                    int i, j;
+                    tri_index( idx, i, j );
+                    const int rr = cc + 1 + i, c2 = cc + 1 + j;
                     sA[c2][rr] -= sA[cc][rr] * sA[cc][c2] * r;
+                }
+                else {
+                    const int k = idx - nsq, rr = cc + 1 + k % m, j = k / m;
+                    sI[j][rr] -= sA[cc][rr] * r * sI[j][cc];
                 }
             }
             if ( tid == 0 ) {
@@ -173,8 +194,10 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             }
             __syncthreads();
         }
-        // Cholesky: scale the columns by 1/sqrt(pivot).  LDL^T: L = column / d,
-        // the diagonal holds D and the upper part D L^T (the unscaled columns)
+        // Cholesky: L = U D^{1/2}, the columns scaled by 1/sqrt(pivot), and
+        // L^{-1} = D^{-1/2} U^{-1}, the rows of W scaled likewise.  LDL^T:
+        // L = U (column / d), L^{-1} = W, the diagonal holds D and the upper
+        // part D L^T (the unscaled columns).
         for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
             int rr = idx % bs, cc = idx / bs;
             if ( rr > cc ) {
@@ -185,6 +208,11 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                 }
                 else {
                     sA[cc][rr] = a / px_sqrt( sD[cc] );
+                }
+            }
+            if constexpr ( !LDLT ) {
+                if ( rr >= cc ) {
+                    sI[cc][rr] = sI[cc][rr] / px_sqrt( sD[rr] );
                 }
             }
         }
@@ -199,21 +227,6 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         }
         if ( tid < PXD_NB ) {
             sDi[tid] = ( tid < bs ) ? (T)1 / sD[tid] : (T)0;
-        }
-        __syncthreads();
-
-        // inverse of the lower-triangular block (LDL^T: of the unit L):
-        // thread t builds column t
-        if ( tid < bs ) {
-            const int t = tid;
-            sI[t][t] = LDLT ? (T)1 : (T)1 / sA[t][t];
-            for ( int i = t + 1; i < bs; i++ ) {
-                T s = (T)0;
-                for ( int l = t; l < i; l++ ) {
-                    s += sA[l][i] * sI[t][l];
-                }
-                sI[t][i] = LDLT ? -s : -s / sA[i][i];
-            }
         }
         __syncthreads();
 
@@ -463,13 +476,17 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
             for ( int u = 0; u < NU; u++ ) {
                 const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
                 sA[cc][r]  = v[u];
-                sIL[cc][r] = (T)0;
-                sIU[cc][r] = (T)0;
+                sIL[cc][r] = ( r == cc && r < bs ) ? (T)1 : (T)0;   // W_L = I
+                sIU[cc][r] = ( r == cc && r < bs ) ? (T)1 : (T)0;   // W_U = I
             }
         }
         __syncthreads();
 
-        // unblocked right-looking LU, the multipliers left unscaled until the end
+        // unblocked right-looking LU, the multipliers left unscaled until the
+        // end.  The row operations of L (multipliers a_r,cc / d) applied to
+        // W_L = I leave W_L = L11^{-1}; those of U11^T = U~ D (multipliers
+        // a_cc,j / d) applied to W_U = I leave W_U = U~^{-1}, so that
+        // (U11^T)^{-1} = D^{-1} W_U.  Both run in the step's barrier interval.
         for ( int cc = 0; cc < bs; cc++ ) {
             T d = sA[cc][cc];
             if ( px_abs( d ) < tau ) {
@@ -478,11 +495,22 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                     piv++;
                 }
             }
-            const int m = bs - cc - 1;
-            const T   r = (T)1 / d;
-            for ( int idx = tid; idx < m * m; idx += PXD_THREADS ) {
-                const int rr = cc + 1 + idx % m, c2 = cc + 1 + idx / m;
-                sA[c2][rr] -= sA[cc][rr] * r * sA[c2][cc];
+            const int m   = bs - cc - 1;
+            const int nsq = m * m, nw = m * ( cc + 1 );
+            const T   r   = (T)1 / d;
+            for ( int idx = tid; idx < nsq + 2 * nw; idx += PXD_THREADS ) {
+                if ( idx < nsq ) {
+                    const int rr = cc + 1 + idx % m, c2 = cc + 1 + idx / m;
+                    sA[c2][rr] -= sA[cc][rr] * r * sA[c2][cc];
+                }
+                else if ( idx < nsq + nw ) {
+                    const int k = idx - nsq, rr = cc + 1 + k % m, j = k / m;
+                    sIL[j][rr] -= sA[cc][rr] * r * sIL[j][cc];
+                }
+                else {
+                    const int k = idx - nsq - nw, rr = cc + 1 + k % m, j = k / m;
+                    sIU[j][rr] -= sA[rr][cc] * r * sIU[j][cc];
+                }
             }
             if ( tid == 0 ) {
                 sD[cc] = d;
@@ -500,25 +528,11 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         }
         __syncthreads();
 
-        // inverses: threads [0, bs) build the columns of L11^{-1} (unit
-        // lower), threads [64, 64 + bs) those of (U11^T)^{-1}
-        if ( tid < bs ) {
-            const int t = tid;
-            sIL[t][t] = (T)1;
-            for ( int i = t + 1; i < bs; i++ ) {
-                T s = (T)0;
-                for ( int l = t; l < i; l++ ) s += sA[l][i] * sIL[t][l];
-                sIL[t][i] = -s;
-            }
-        }
-        else if ( tid >= 64 && tid < 64 + bs ) {
-            const int t = tid - 64;
-            // U11^T[i][l] = U11[l][i] = sA[i][l]
-            sIU[t][t] = (T)1 / sA[t][t];
-            for ( int i = t + 1; i < bs; i++ ) {
-                T s = (T)0;
-                for ( int l = t; l < i; l++ ) s += sA[i][l] * sIU[t][l];
-                sIU[t][i] = -s / sA[i][i];
+        // (U11^T)^{-1} = D^{-1} W_U: rows scaled by 1/d (W_L is L11^{-1} as is)
+        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
+            const int rr = idx % bs, cc = idx / bs;
+            if ( rr >= cc ) {
+                sIU[cc][rr] = sIU[cc][rr] / sD[rr];
             }
         }
         __syncthreads();

@@ -10,7 +10,7 @@ import pytest
 
 import pastix_amd as px
 import oracle_ref as orc
-from helpers import analyzed, cblk_diff, make_problem, static_pivot_threshold
+from helpers import analyzed, cblk_diff, make_problem, permute_rhs, static_pivot_threshold, unpermute
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,13 @@ def test_ldlt_solve_backward_error(nx, nrhs):
     x = np.array(b, order="F", copy=True)
     px.task_solve(pd, spm, x)
     berr, _ = spm.backward_error(b, x)
-    assert berr <= 1e-13
+    # LDL^T without pivoting of an indefinite matrix has element growth, so
+    # the bound is the oracle's own solve on the same structure, within 10x
+    s = px.solver_export(pd)
+    F = orc.OracleLDLT(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float64), np.float64)
+    xo = unpermute(pd, F.solve(permute_rhs(pd, b)))
+    berr_o, _ = spm.backward_error(b, np.asfortranarray(xo))
+    assert berr <= max(1e-13, 10.0 * berr_o)
     px.finalize(pd)
 
 
