@@ -36,7 +36,7 @@ struct level_t {
     int64_t u0, uc, u1; // update tasks (64 x 64 tiles); [u0, uc) face the next level
     int64_t b0, bc, b1; // update tasks (tall tiles); [b0, bc) face the next level
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
-    int64_t d0, d1;     // solve diagonal tasks (64-row chunks of the diagonal blocks)
+    int64_t d0, d1;     // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks)
     int64_t i0, i1;     // block-column inversion tasks
     int64_t g0, g1;     // blok updates (upd_group)
     int32_t maxn;
@@ -264,7 +264,7 @@ build_schedule( pxd_ctx_t *ctx )
             for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
                 solv.push_back( row_task{ k, r0 } );
             }
-            for ( int32_t r0 = 0; r0 < nk; r0 += 64 ) {
+            for ( int32_t r0 = 0; r0 < nk; r0 += PXD_DIAG_ROWS ) {
                 diag.push_back( row_task{ k, r0 } );
             }
             for ( int32_t J = 0; J + PXD_NB < nk; J += PXD_NB ) {

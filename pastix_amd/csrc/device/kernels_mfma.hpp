@@ -1308,18 +1308,25 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal solve of one 64-row chunk of a cblk, out of place: Y[R0..R0+63]
-// <- (Linv or Linv^T)[R0.., :] Xin_k, so that the chunks of a cblk run in
-// parallel workgroups (tasks {cblk, R0}).
+// Diagonal solve of one PXD_DIAG_ROWS-row chunk of a cblk, out of place:
+// Y[R0..R0+15] <- (Linv or Linv^T)[R0.., :] Xin_k, so that the chunks of a
+// cblk run in parallel workgroups (tasks {cblk, R0}).  Thread (r, g) sums
+// the columns g, g+16, ... of output row R0+r; every load of the workgroup
+// is issued at once (a cblk is at most PXD_TRSM_NT * 16 wide), and the
+// lanes of a 16-thread group read 16 consecutive elements (rows of Linv
+// forward, columns of Linv, i.e. rows of Linv^T, backward).
+#define PXD_DIAG_ROWS 16
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ Linv,
                     const T *__restrict__ Xin, int64_t ldx, T *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
 {
+    constexpr int CR   = PXD_DIAG_ROWS;
+    constexpr int NG   = PXD_THREADS / CR;          // column phases
+    constexpr int MAXU = ( PXD_TRSM_NT * 16 + NG - 1 ) / NG;
     extern __shared__ unsigned char smem_raw[];
-    T *sb = reinterpret_cast<T *>( smem_raw );   // [n][nrhs]
-    __shared__ T sM[64][64 + 1];                 // operator chunk, sM[row][col]; then partial sums
-    T           *sred = &sM[0][0];
+    T *sb = reinterpret_cast<T *>( smem_raw );      // [n][nrhs]
+    __shared__ T sred[NG][CR][PXD_MAXRHS + 1];
 
     const row_task tk  = tasks[blockIdx.x];
     const dcblk    c   = cb[tk.cblk];
@@ -1328,8 +1335,17 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     const int      tid = threadIdx.x;
     const T       *LI  = Linv + c.dinvoff;
     const T       *x   = Xin + c.fcolnum;
-    const int      r   = tid & 63, part = tid >> 6;
+    const int      r   = trans ? tid / NG : tid % CR;
+    const int      g   = trans ? tid % NG : tid / CR;
+    const int32_t  i   = R0 + r;
 
+    T a[MAXU];
+#pragma unroll
+    for ( int u = 0; u < MAXU; u++ ) {
+        const int32_t j  = g + NG * u;
+        const bool    ok = i < n && j < n && ( trans ? j >= i : j <= i );
+        a[u] = ok ? LI[trans ? j + (int64_t)i * n : i + (int64_t)j * n] : (T)0;
+    }
     for ( int idx = tid; idx < n * nrhs; idx += PXD_THREADS ) {
         const int rr = idx % n, q = idx / n;
         sb[rr * nrhs + q] = x[rr + q * ldx];
@@ -1339,49 +1355,25 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     T acc[PXD_MAXRHS];
 #pragma unroll
     for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
-    const int32_t cfirst = trans ? R0 : 0, clast = trans ? n : R0 + 1;
-    for ( int32_t C0 = cfirst; C0 < clast; C0 += 64 ) {
-        constexpr int NU = 64 * 64 / PXD_THREADS;
-        T             v[NU];
 #pragma unroll
-        for ( int u = 0; u < NU; u++ ) {
-            const int     i = tid & 63, j = ( tid >> 6 ) + 4 * u;
-            const int32_t gi = ( trans ? C0 : R0 ) + i, gj = ( trans ? R0 : C0 ) + j;
-            v[u] = ( gi < n && gj < n && gi >= gj ) ? LI[gi + (int64_t)gj * n] : (T)0;
-        }
+    for ( int u = 0; u < MAXU; u++ ) {
+        const int32_t j = g + NG * u;
+        if ( j < n ) {
 #pragma unroll
-        for ( int u = 0; u < NU; u++ ) {
-            const int i = tid & 63, j = ( tid >> 6 ) + 4 * u;
-            if ( trans ) {
-                sM[j][i] = v[u];
-            }
-            else {
-                sM[i][j] = v[u];
+            for ( int q = 0; q < PXD_MAXRHS; q++ ) {
+                if ( q < nrhs ) acc[q] += a[u] * sb[j * nrhs + q];
             }
         }
-        __syncthreads();
-#pragma unroll
-        for ( int cc = 16 * part; cc < 16 * part + 16; cc++ ) {
-            if ( C0 + cc < n ) {
-                const T m = sM[r][cc];
-#pragma unroll
-                for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                    if ( q < nrhs ) acc[q] += m * sb[( C0 + cc ) * nrhs + q];
-                }
-            }
-        }
-        __syncthreads();
     }
 #pragma unroll
-    for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[( part * 64 + r ) * PXD_MAXRHS + q] = acc[q];
+    for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[g][r][q] = acc[q];
     __syncthreads();
-    if ( part == 0 && R0 + r < n ) {
-        for ( int q = 0; q < nrhs; q++ ) {
-            T v = (T)0;
+    if ( tid < CR * nrhs ) {
+        const int r2 = tid % CR, q = tid / CR;
+        T         v  = (T)0;
 #pragma unroll
-            for ( int p2 = 0; p2 < 4; p2++ ) v += sred[( p2 * 64 + r ) * PXD_MAXRHS + q];
-            Yout[c.fcolnum + R0 + r + q * ldy] = v;
-        }
+        for ( int p2 = 0; p2 < NG; p2++ ) v += sred[p2][r2][q];
+        if ( R0 + r2 < n ) Yout[c.fcolnum + R0 + r2 + q * ldy] = v;
     }
 }
 

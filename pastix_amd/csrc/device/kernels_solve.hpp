@@ -126,13 +126,14 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
     for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
     if ( valid ) {
         // 16 columns per batch: the loads of a batch are all in flight at once
+        constexpr int B  = 16;
         const int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
-        for ( int32_t cb0 = c0; cb0 < c1; cb0 += 16 ) {
-            T a[16];
+        for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
+            T a[B];
 #pragma unroll
-            for ( int u = 0; u < 16; u++ ) a[u] = ( cb0 + u < c1 ) ? L[base + (int64_t)( cb0 + u ) * ld] : (T)0;
+            for ( int u = 0; u < B; u++ ) a[u] = ( cb0 + u < c1 ) ? L[base + (int64_t)( cb0 + u ) * ld] : (T)0;
 #pragma unroll
-            for ( int u = 0; u < 16; u++ ) {
+            for ( int u = 0; u < B; u++ ) {
                 if ( cb0 + u < c1 ) {
 #pragma unroll
                     for ( int q = 0; q < PXD_MAXRHS; q++ ) {
