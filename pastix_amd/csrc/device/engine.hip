@@ -39,6 +39,7 @@ struct level_t {
     int64_t d0, d1;     // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks)
     int64_t i0, i1;     // block-column inversion tasks
     int64_t g0, g1;     // blok updates (upd_group)
+    int64_t q0, q1;     // multi-workgroup panel tasks (32-row blocks of the diagonal tiles)
     int32_t maxn;
 };
 
@@ -75,6 +76,9 @@ struct pxd_ctx_s {
     int                  tasks_dirty = 0;   // options changed the task lists
     std::vector<level_t> levels;
     int32_t  *dlist = nullptr;
+    pan_task *dpan  = nullptr;
+    int      *dflags = nullptr;     // progress counters of the panel tasks
+    int64_t   npan  = 0;
     row_task *dtrsm = nullptr;
     upd_task *dupd  = nullptr;
     upd_task *dupdb = nullptr;
@@ -115,6 +119,7 @@ struct pxd_ctx_s {
     // host copies of the panel, TRSM and inversion lists, and the blok
     // updates in launch order (kernel timing log)
     std::vector<int32_t>   hlist;
+    std::vector<pan_task>  hpan;
     std::vector<row_task>  htrsm;
     std::vector<inv_task>  hinvt;
     std::vector<upd_group> groups;
@@ -124,6 +129,9 @@ struct pxd_ctx_s {
     // 1: panels, inverses, TRSM and the updates into the next level raise
     // their waves' issue priority (s_setprio) over the bulk updates
     int          prio = 1;
+    // 1: Cholesky diagonal tiles factored by one workgroup per 32-row block
+    // (k_potrf_mw); 0: one workgroup per cblk (k_potrf2)
+    int          panel_mw = 1;
     // 1: the factorization runs one stream, launches every blok update on
     // its own and records per-workgroup times into model_entries
     int          model_log = 0;
@@ -230,6 +238,7 @@ build_schedule( pxd_ctx_t *ctx )
     }
 
     std::vector<int32_t>  list;
+    std::vector<pan_task> pan;
     std::vector<row_task> trsm, solv, diag;
     std::vector<inv_task> invt;
     std::vector<upd_task> upd, updb;
@@ -244,6 +253,7 @@ build_schedule( pxd_ctx_t *ctx )
     for ( int32_t L = 0; L < nlev; L++ ) {
         level_t &lv = ctx->levels[L];
         lv.c0   = list.size();
+        lv.q0   = pan.size();
         lv.t0   = trsm.size();
         lv.u0   = upd.size();
         lv.b0   = updb.size();
@@ -258,6 +268,12 @@ build_schedule( pxd_ctx_t *ctx )
             int32_t      nk = c.lcolnum - c.fcolnum + 1;
             lv.maxn = std::max( lv.maxn, nk );
             list.push_back( k );
+            {
+                const int32_t f0 = (int32_t)pan.size();
+                for ( int32_t pb = 0; PXD_NB * pb < nk; pb++ ) {
+                    pan.push_back( pan_task{ k, pb, f0, 0 } );
+                }
+            }
             for ( int32_t r0 = nk; r0 < c.stride; r0 += ctx->trsm_rt ) {
                 trsm.push_back( row_task{ k, r0 } );
             }
@@ -347,6 +363,7 @@ build_schedule( pxd_ctx_t *ctx )
             i = j;
         }
         lv.c1 = list.size();
+        lv.q1 = pan.size();
         lv.t1 = trsm.size();
         lv.u1 = upd.size();
         lv.b1 = updb.size();
@@ -361,9 +378,16 @@ build_schedule( pxd_ctx_t *ctx )
     ctx->stats.levels       = nlev;
     ctx->stats.update_tiles = upd.size() + updb.size();
     ctx->hlist = list;
+    ctx->hpan  = pan;
+    ctx->npan  = (int64_t)pan.size();
+    if ( ctx->dflags ) {
+        PXD_CHECK( hipFree( ctx->dflags ) );
+        ctx->dflags = nullptr;
+    }
+    PXD_CHECK( hipMalloc( (void **)&ctx->dflags, sizeof( int ) * std::max<int64_t>( 1, ctx->npan ) ) );
     ctx->htrsm = trsm;
     ctx->hinvt = invt;
-    if ( upload( &ctx->dlist, list ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
+    if ( upload( &ctx->dlist, list ) || upload( &ctx->dpan, pan ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
          upload( &ctx->dupdb, updb ) ||
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
         return 6;
@@ -392,7 +416,7 @@ ev_ms( hipEvent_t a, hipEvent_t b )
 // workgroup's start to its last workgroup's end.
 void
 model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_list, int64_t n_inv,
-                   int64_t n_trsm, int64_t n_upd, double sec_per_tick )
+                   int64_t n_trsm, int64_t n_upd, double sec_per_tick, bool mw )
 {
     const bool      lu    = ( ctx->facto == FACT_GETRF );
     const int       nside = lu ? 2 : 1;
@@ -433,13 +457,22 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
     const int kpanel = lu ? PastixKernelGETRF : ( ctx->facto == FACT_SYTRF ) ? PastixKernelSYTRF : PastixKernelPOTRF;
     std::vector<pxd_model_entry_t> &out = ctx->model_entries;
     out.clear();
+    // multi-workgroup panels: the tasks of cblk k in the panel-task list
+    std::vector<int64_t> qb( nc + 1, 0 ), qe( nc + 1, 0 );
+    if ( mw ) {
+        for ( int64_t i = (int64_t)ctx->hpan.size() - 1; i >= 0; i-- ) {
+            const int32_t k = ctx->hpan[i].cblk;
+            if ( qe[k] == 0 ) qe[k] = i + 1;
+            qb[k] = i;
+        }
+    }
     for ( const level_t &lv : ctx->levels ) {
         for ( int64_t i = lv.c0; i < lv.c1; i++ ) {
             const int32_t k = ctx->hlist[i];
             const dcblk  &c = ctx->hcb[k];
             const int32_t n = c.lcolnum - c.fcolnum + 1;
             double        t = 0.;
-            if ( span( sp, i, i + 1, t ) ) {
+            if ( mw ? span( sp, qb[k], qe[k], t ) : span( sp, i, i + 1, t ) ) {
                 out.push_back( pxd_model_entry_t{ kpanel, n, 0, 0, t } );
             }
             if ( c.stride == n ) {
@@ -484,6 +517,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     int64_t launches = 0;
 
     PXD_CHECK( hipMemsetAsync( ctx->dscratch, 0, 4 * sizeof( int ), s ) );
+    PXD_CHECK( hipMemsetAsync( ctx->dflags, 0, sizeof( int ) * std::max<int64_t>( 1, ctx->npan ), s ) );
     PXD_CHECK( hipEventRecord( e0, s ) );
 
     // fill: zero + scatter of the lower triangle of P A P^T (LU: and of the
@@ -518,7 +552,9 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     const bool mlog = ctx->model_log && ctx->kernels;
     uint64_t *stamps_all = nullptr, *st_panel = nullptr, *st_inv = nullptr, *st_trsm = nullptr;
     uint64_t *st_upd = nullptr, *st_updb = nullptr;
-    const int64_t n_list = ctx->hlist.size(), n_inv = ctx->hinvt.size(), n_trsm = ctx->htrsm.size();
+    const bool    use_mw = ctx->panel_mw && ctx->kernels && !lu && !ldlt;
+    const int64_t n_list = use_mw ? (int64_t)ctx->hpan.size() : (int64_t)ctx->hlist.size();
+    const int64_t n_inv = ctx->hinvt.size(), n_trsm = ctx->htrsm.size();
     int64_t n_upd = 0, n_updb = 0;
     for ( const level_t &lv : ctx->levels ) {
         n_upd  = std::max( n_upd, lv.u1 );
@@ -572,9 +608,20 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                                     ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
-                hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
-                                    ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
-                                    stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
+                if ( ctx->panel_mw ) {
+                    // one workgroup per 32-row block of each diagonal tile
+                    const unsigned nq  = (unsigned)( lv.q1 - lv.q0 );
+                    const size_t   shq = (size_t)( ( std::max( 1, lv.maxn ) + PXD_NB - 1 ) / PXD_NB ) * PXD_NB *
+                                       ( PXD_NB + 1 ) * sizeof( T );
+                    hipLaunchKernelGGL( ( k_potrf_mw<T> ), dim3( nq ), dim3( PXD_THREADS ), shq, st,
+                                        ctx->dpan + lv.q0, ctx->dcb, L, DI, thresh, ctx->dscratch, ctx->dflags,
+                                        ctx->dscratch + 2, stp( st_panel, lv.q0, 0, 0 ), ctx->prio );
+                }
+                else {
+                    hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
+                                        ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
+                                        stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
+                }
             }
             launches++;
             if ( ni ) {
@@ -714,7 +761,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         PXD_CHECK( hipFree( stamps_all ) );
         int rate_khz = 0;
         PXD_CHECK( hipDeviceGetAttribute( &rate_khz, hipDeviceAttributeWallClockRate, ctx->device ) );
-        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, 1. / ( 1e3 * std::max( 1, rate_khz ) ) );
+        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, 1. / ( 1e3 * std::max( 1, rate_khz ) ), use_mw );
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
@@ -723,6 +770,10 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     if ( h[1] > 0 ) {
         fprintf( stderr, "pastix_amd device: %d matrix entries overflow the factor precision\n", h[1] );
         return 7; /* PASTIX_ERR_BADPARAMETER */
+    }
+    if ( h[2] != 0 ) {
+        fprintf( stderr, "pastix_amd device: a panel workgroup timed out waiting for its neighbour\n" );
+        return 6; /* PASTIX_ERR_INTERNAL */
     }
     return 0;
 }
@@ -836,6 +887,9 @@ set_lds_attributes_mfma( size_t lds )
     (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
+        (void)hipFuncSetAttribute( (const void *)k_potrf_mw<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    }
 }
 
 template <typename T>
@@ -951,7 +1005,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
     for ( void *p : ptrs ) {
@@ -998,6 +1052,10 @@ int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
+    if ( k == "panel_mw" ) {
+        ctx->panel_mw = ( value != 0. );
+        return 0;
+    }
     if ( k == "prio" ) {
         ctx->prio = ( value != 0. );
         return 0;

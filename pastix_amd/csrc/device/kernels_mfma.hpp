@@ -367,6 +367,257 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 }
 
 // ---------------------------------------------------------------------------
+// Multi-workgroup Cholesky of the diagonal tiles.  Task {cblk, p, flag0}:
+// the workgroup owns the 32-row block p of the cblk's n x n diagonal tile,
+// i.e. the 32 x 32 tiles T(p, 0..p), kept in LDS from the start.  For
+// k = 0..p-1 it waits until block k has factored its diagonal tile
+// (flag[k] > k), forms L(p,k) = T(p,k) L(k,k)^{-T} with the inverse that
+// block k wrote to Linv, publishes it (flag[p] = k+1), and subtracts
+// L(p,k) L(j,k)^T from its tiles T(p,j), j = k+1..p, each L(j,k) once
+// block j has published it.  For k = p it factors T(p,p) like one block of
+// k_potrf2 (static pivot, inverse built by the elimination) and publishes
+// flag[p] = p+1.  The critical path is the chain of diagonal tiles instead
+// of the n/32 sequential block steps of a single workgroup.
+//
+// Publication: a workgroup-scope release by every thread, a barrier, then a
+// device-scope release and the flag store by thread 0.  Waits: thread 0
+// polls the flag, then a device-scope acquire and a barrier.  Every wait is
+// bounded by a wall-clock deadline; on expiry the kernel records it in *err
+// and goes on, so every wave reaches the end.
+struct pan_task {
+    int32_t cblk;
+    int32_t p;
+    int32_t flag0;
+    int32_t pad;
+};
+#define PXD_PANEL_WAIT_TICKS 200000000ull   // 2 s of the 100 MHz constant clock
+
+__device__ inline void
+panel_wait( const int *f, int target, int *err, int tid )
+{
+    if ( tid == 0 ) {
+        // after one timeout anywhere every later wait falls through at once
+        const uint64_t t0 = (uint64_t)wall_clock64();
+        while ( __hip_atomic_load( f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) < target &&
+                __hip_atomic_load( err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == 0 ) {
+            if ( (uint64_t)wall_clock64() - t0 > PXD_PANEL_WAIT_TICKS ) {
+                __hip_atomic_store( err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+                break;
+            }
+            __builtin_amdgcn_s_sleep( 1 );
+        }
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup" );
+}
+
+__device__ inline void
+panel_publish( int *f, int value, int tid )
+{
+    __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup" );
+    __syncthreads();
+    if ( tid == 0 ) {
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
+        __hip_atomic_store( f, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+    }
+}
+
+// P[r][c] = sum_l X[r][l] Y[c][l] for 32 x 32 tiles held as s[col][row]:
+// wave w computes the 16 x 16 quadrant (w >> 1, w & 1) on the matrix cores.
+template <typename T>
+__device__ inline typename mfma16<T>::acc_t
+tile_xyt( const T ( *sX )[PXD_NB + 1], const T ( *sY )[PXD_NB + 1], int lane, int w )
+{
+    typedef mfma16<T> M;
+    typename M::acc_t acc = M::zero();
+    const int qr = w >> 1, qc = w & 1;
+#pragma unroll
+    for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+        const int l = 4 * s4 + ( lane >> 4 );
+        acc = M::mma( sX[l][16 * qr + ( lane & 15 )], sY[l][16 * qc + ( lane & 15 )], acc );
+    }
+    return acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T *__restrict__ L,
+            T *__restrict__ Linv, double thresh, int *__restrict__ npiv, int *flags, int *err,
+            uint64_t *__restrict__ stamps, int prio )
+{
+    typedef mfma16<T>         M;
+    typedef typename M::acc_t acc_t;
+    typedef T                 tile_t[PXD_NB][PXD_NB + 1];   // [col][row]
+    __shared__ tile_t sI;                // inverse of a diagonal block
+    __shared__ tile_t sO;                // L(j,k) of another block
+    __shared__ T      sD[PXD_NB];
+    extern __shared__ unsigned char smem_raw[];
+    tile_t *sT = reinterpret_cast<tile_t *>( smem_raw );   // T(p, 0..p)
+
+    const pan_task tk      = tasks[blockIdx.x];
+    const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
+    const dcblk   c    = cb[tk.cblk];
+    const int32_t n    = c.lcolnum - c.fcolnum + 1;
+    const int     p    = tk.p;
+    const int32_t R0   = PXD_NB * p;
+    const int32_t bs   = min( PXD_NB, n - R0 );
+    const int     tid  = threadIdx.x;
+    const int     lane = tid & 63, w = tid >> 6;
+    T            *A    = L + c.coefoff;
+    T            *LI   = Linv + c.dinvoff;
+    int          *flag = flags + tk.flag0;
+    constexpr int NU   = PXD_NB * PXD_NB / PXD_THREADS;
+    int           piv  = 0;
+    typedef typename px_real<T>::type R;
+    const R tau = (R)thresh;
+
+    // the block row: T(p, j) for j = 0..p (lower part only for j = p)
+    for ( int j0 = 0; j0 <= p; j0 += 4 ) {
+        T v[4][NU];
+#pragma unroll
+        for ( int jj = 0; jj < 4; jj++ )
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB, j = j0 + jj;
+                const bool ok = j <= p && r < bs && ( j < p || cc <= r );
+                v[jj][u] = ok ? A[( R0 + r ) + (int64_t)( PXD_NB * j + cc ) * n] : (T)0;
+            }
+#pragma unroll
+        for ( int jj = 0; jj < 4; jj++ )
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                if ( j0 + jj <= p ) sT[j0 + jj][cc][r] = v[jj][u];
+            }
+    }
+
+    for ( int k = 0; k < p; k++ ) {
+        // L(p,k) = T(p,k) L(k,k)^{-T}
+        panel_wait( flag + k, k + 1, err, tid );
+        {
+            T v[NU];
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                v[u] = LI[( PXD_NB * k + r ) + (int64_t)( PXD_NB * k + cc ) * n];
+            }
+#pragma unroll
+            for ( int u = 0; u < NU; u++ ) {
+                const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                sI[cc][r] = v[u];
+            }
+        }
+        __syncthreads();
+        {
+            acc_t acc = tile_xyt<T>( sT[k], sI, lane, w );
+            __syncthreads();   // every wave has read T(p,k)
+            const int qr = w >> 1, qc = w & 1;
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) {
+                const int r = 16 * qr + M::row( lane, v ), cc = 16 * qc + ( lane & 15 );
+                const T   x = ( r < bs ) ? M::get( acc, v ) : (T)0;
+                sT[k][cc][r] = x;
+                if ( r < bs ) A[( R0 + r ) + (int64_t)( PXD_NB * k + cc ) * n] = x;
+            }
+        }
+        panel_publish( flag + p, k + 1, tid );
+        __syncthreads();   // sT[k] complete for every wave
+        // T(p,j) -= L(p,k) L(j,k)^T, j = k+1..p
+        for ( int j = k + 1; j <= p; j++ ) {
+            const tile_t *Y = &sT[k];
+            if ( j < p ) {
+                panel_wait( flag + j, k + 1, err, tid );
+                T v[NU];
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                    v[u] = A[( PXD_NB * j + r ) + (int64_t)( PXD_NB * k + cc ) * n];
+                }
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
+                    sO[cc][r] = v[u];
+                }
+                __syncthreads();
+                Y = &sO;
+            }
+            acc_t     acc = tile_xyt<T>( sT[k], *Y, lane, w );
+            const int qr = w >> 1, qc = w & 1;
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) {
+                const int r = 16 * qr + M::row( lane, v ), cc = 16 * qc + ( lane & 15 );
+                sT[j][cc][r] -= M::get( acc, v );
+            }
+            __syncthreads();   // sO is reused by the next j
+        }
+    }
+
+    // T(p,p): elimination with the inverse built alongside (as in k_potrf2)
+    T( *sA )[PXD_NB + 1] = sT[p];
+    for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+        const int r = idx % PXD_NB, cc = idx / PXD_NB;
+        sI[cc][r] = ( r == cc && r < bs ) ? (T)1 : (T)0;
+    }
+    __syncthreads();
+    for ( int cc = 0; cc < bs; cc++ ) {
+        T d = sA[cc][cc];
+        if ( px_abs( d ) < tau ) {
+            d = (T)tau;   // core_zpotf2sp
+            if ( tid == 0 ) {
+                piv++;
+            }
+        }
+        const int m   = bs - cc - 1;
+        const int nsq = m * ( m + 1 ) / 2;
+        const T   rr_ = (T)1 / d;
+        for ( int idx = tid; idx < nsq + m * ( cc + 1 ); idx += PXD_THREADS ) {
+            if ( idx < nsq ) {
+                int i, j;
+                tri_index( idx, i, j );
+                const int rr = cc + 1 + i, c2 = cc + 1 + j;
+                sA[c2][rr] -= sA[cc][rr] * sA[cc][c2] * rr_;
+            }
+            else {
+                const int kk = idx - nsq, rr = cc + 1 + kk % m, j = kk / m;
+                sI[j][rr] -= sA[cc][rr] * rr_ * sI[j][cc];
+            }
+        }
+        if ( tid == 0 ) {
+            sD[cc] = d;
+        }
+        __syncthreads();
+    }
+    for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
+        const int rr = idx % bs, cc = idx / bs;
+        if ( rr > cc ) {
+            sA[cc][rr] = sA[cc][rr] / px_sqrt( sD[cc] );
+        }
+        if ( rr >= cc ) {
+            sI[cc][rr] = sI[cc][rr] / px_sqrt( sD[rr] );
+        }
+    }
+    __syncthreads();
+    if ( tid < bs ) {
+        sA[tid][tid] = px_sqrt( sD[tid] );
+    }
+    __syncthreads();
+    for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
+        const int rr = idx % bs, cc = idx / bs;
+        if ( rr >= cc ) {
+            A[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sA[cc][rr];
+        }
+        LI[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sI[cc][rr];
+    }
+    panel_publish( flag + p, p + 1, tid );
+    if ( tid == 0 && piv > 0 ) {
+        atomicAdd( npiv, piv );
+    }
+    stamp_end( stamps, blockIdx.x, t_stamp );
+}
+
+// ---------------------------------------------------------------------------
 // X = B * S^T for a rows x 32 block B of the diagonal tile, in place, on the
 // matrix cores: wave w takes 16-row blocks w, w+4, ...; row i of B is
 // element (i, k) at base + i * rs + k * ks; S is a 32 x 32 matrix in LDS,

@@ -24,7 +24,7 @@ def _factor_case(nx, mixed=0, kernels=1):
 
 
 @pytest.mark.parametrize("kernels", [1, 0])
-@pytest.mark.parametrize("nx", [3, 5, 8, 12, 20])
+@pytest.mark.parametrize("nx", [3, 5, 8, 12, 20, 28])
 def test_potrf_fp64_tiles_match_oracle(nx, kernels):
     spm, pd, iparm, dparm = _factor_case(nx, kernels=kernels)
     s = px.solver_export(pd)
@@ -60,17 +60,19 @@ def test_solve_fp64_backward_error(nx, nrhs, kernels):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("big_rows,timing", [(1, 0), (10 ** 9, 0), (192, 1)])
-def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing):
+@pytest.mark.parametrize("big_rows,timing,panel_mw", [(1, 0, 1), (10 ** 9, 0, 1), (192, 1, 1), (192, 0, 0)])
+def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing, panel_mw):
     """Every blok update on the tall tiles (big_rows=1), every one on the
-    64 x 64 tiles (big_rows=1e9), and the single-stream timed schedule
-    instead of the two-stream lookahead: the factor matches the oracle."""
+    64 x 64 tiles (big_rows=1e9), the single-stream timed schedule instead
+    of the two-stream lookahead, and the single-workgroup panel instead of
+    the one-workgroup-per-block panel: the factor matches the oracle."""
     nx = 18
     mixed = int(dtype == np.float32)
     spm = make_problem("lap7", nx)
     pd, iparm, dparm = analyzed(spm, nx, mixed=mixed)
     px.set_option(pd, "upd_big_rows", big_rows)
     px.set_option(pd, "timing", timing)
+    px.set_option(pd, "panel_mw", panel_mw)
     px.task_numfact(pd, spm)
     s = px.solver_export(pd)
     L = px.get_coeftab(pd, s["coefnbr"], dtype)
