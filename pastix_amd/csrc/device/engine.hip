@@ -35,6 +35,7 @@ struct level_t {
     int64_t t0, t1;     // trsm tasks (32-row tiles)
     int64_t u0, uc, u1; // update tasks (64 x 64 tiles); [u0, uc) face the next level
     int64_t b0, bc, b1; // update tasks (tall tiles); [b0, bc) face the next level
+    int64_t s0, sc, s1; // small updates (one wavefront each); [s0, sc) face the next level
     int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
     int64_t d0, d1;     // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks)
     int64_t i0, i1;     // block-column inversion tasks
@@ -82,6 +83,7 @@ struct pxd_ctx_s {
     row_task *dtrsm = nullptr;
     upd_task *dupd  = nullptr;
     upd_task *dupdb = nullptr;
+    upd_task *dupds = nullptr;
     row_task *dsolv = nullptr;
     row_task *ddiag = nullptr;
     inv_task *dinvt = nullptr;
@@ -113,6 +115,9 @@ struct pxd_ctx_s {
     int          kernels = 1;
     // blok updates with at least this many stacked rows use the tall tiles
     int          upd_big_rows = 192;
+    // 1: updates of at most 64 rows, PXD_SMALL_N columns and PXD_SMALL_K
+    // source columns take one wavefront each (k_update_small)
+    int          upd_small = 1;
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
 
@@ -241,7 +246,7 @@ build_schedule( pxd_ctx_t *ctx )
     std::vector<pan_task> pan;
     std::vector<row_task> trsm, solv, diag;
     std::vector<inv_task> invt;
-    std::vector<upd_task> upd, updb;
+    std::vector<upd_task> upd, updb, upds;
     ctx->levels.assign( nlev, level_t{} );
     const bool big_ok = ctx->kernels != 0;
 
@@ -257,6 +262,7 @@ build_schedule( pxd_ctx_t *ctx )
         lv.t0   = trsm.size();
         lv.u0   = upd.size();
         lv.b0   = updb.size();
+        lv.s0   = upds.size();
         lv.f0   = solv.size();
         lv.d0   = diag.size();
         lv.i0   = invt.size();
@@ -269,9 +275,9 @@ build_schedule( pxd_ctx_t *ctx )
             lv.maxn = std::max( lv.maxn, nk );
             list.push_back( k );
             {
-                const int32_t f0 = (int32_t)pan.size();
-                for ( int32_t pb = 0; PXD_NB * pb < nk; pb++ ) {
-                    pan.push_back( pan_task{ k, pb, f0, 0 } );
+                const int32_t f0 = (int32_t)pan.size(), nblk = ( nk + PXD_NB - 1 ) / PXD_NB;
+                for ( int32_t pb = 0; pb < nblk; pb++ ) {
+                    pan.push_back( pan_task{ k, pb, f0, nblk } );
                 }
             }
             for ( int32_t r0 = nk; r0 < c.stride; r0 += ctx->trsm_rt ) {
@@ -293,6 +299,7 @@ build_schedule( pxd_ctx_t *ctx )
             if ( pass == 1 ) {
                 lv.uc = upd.size();
                 lv.bc = updb.size();
+                lv.sc = upds.size();
             }
             for ( int32_t k : bylev[L] ) {
                 const dcblk &c  = ctx->hcb[k];
@@ -310,7 +317,16 @@ build_schedule( pxd_ctx_t *ctx )
                             continue;
                         }
                         const int32_t fl = side ? 2 : 0;
-                        if ( big_ok && M >= ctx->upd_big_rows ) {
+                        const int32_t K  = c.lcolnum - c.fcolnum + 1;
+                        if ( big_ok && ctx->upd_small && M <= 64 && N <= PXD_SMALL_N && K <= PXD_SMALL_K ) {
+                            // one wavefront for the whole update
+                            int64_t first = upds.size();
+                            upds.push_back( upd_task{ k, b, 0, 0, fl, 0 } );
+                            ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &upds, first,
+                                               (int64_t)upds.size() } );
+                            ctx->groups.push_back( upd_group{ k, b, M, N, K, 2, side, first, (int64_t)upds.size() } );
+                        }
+                        else if ( big_ok && M >= ctx->upd_big_rows ) {
                             int64_t first = updb.size();
                             // column tiles fastest: the tiles that share a row
                             // panel run together and share its L2 lines
@@ -367,16 +383,17 @@ build_schedule( pxd_ctx_t *ctx )
         lv.t1 = trsm.size();
         lv.u1 = upd.size();
         lv.b1 = updb.size();
+        lv.s1 = upds.size();
         lv.f1 = solv.size();
         lv.d1 = diag.size();
         lv.i1 = invt.size();
         lv.g1 = ctx->groups.size();
     }
-    ctx->nupd  = upd.size() + updb.size();
+    ctx->nupd  = upd.size() + updb.size() + upds.size();
     ctx->ntrsm = trsm.size();
     ctx->nsolv = solv.size();
     ctx->stats.levels       = nlev;
-    ctx->stats.update_tiles = upd.size() + updb.size();
+    ctx->stats.update_tiles = upd.size() + updb.size() + upds.size();
     ctx->hlist = list;
     ctx->hpan  = pan;
     ctx->npan  = (int64_t)pan.size();
@@ -388,7 +405,7 @@ build_schedule( pxd_ctx_t *ctx )
     ctx->htrsm = trsm;
     ctx->hinvt = invt;
     if ( upload( &ctx->dlist, list ) || upload( &ctx->dpan, pan ) || upload( &ctx->dtrsm, trsm ) || upload( &ctx->dupd, upd ) ||
-         upload( &ctx->dupdb, updb ) ||
+         upload( &ctx->dupdb, updb ) || upload( &ctx->dupds, upds ) ||
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
         return 6;
     }
@@ -416,7 +433,7 @@ ev_ms( hipEvent_t a, hipEvent_t b )
 // workgroup's start to its last workgroup's end.
 void
 model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_list, int64_t n_inv,
-                   int64_t n_trsm, int64_t n_upd, double sec_per_tick, bool mw )
+                   int64_t n_trsm, int64_t n_upd, int64_t n_updb, double sec_per_tick, bool mw )
 {
     const bool      lu    = ( ctx->facto == FACT_GETRF );
     const int       nside = lu ? 2 : 1;
@@ -425,6 +442,7 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
     const uint64_t *sm    = si + 4 * n_inv;
     const uint64_t *su    = sm + 4 * n_trsm;
     const uint64_t *sb    = su + 2 * n_upd;
+    const uint64_t *ss    = sb + 2 * n_updb;   // small updates: one task per launch in this mode
     // span of the stamped tasks [a, b) of one buffer; false when none ran
     auto span = [&]( const uint64_t *base, int64_t a, int64_t b, double &t ) {
         uint64_t lo = UINT64_MAX, hi = 0;
@@ -489,7 +507,8 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
         for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
             const upd_group &gr = ctx->groups[g];
             double           t  = 0.;
-            if ( span( gr.big ? sb : su, gr.t0, gr.t1, t ) ) {
+            const bool ok = ( gr.big == 2 ) ? span( ss, gr.t0, gr.t1, t ) : span( gr.big ? sb : su, gr.t0, gr.t1, t );
+            if ( ok ) {
                 out.push_back( pxd_model_entry_t{ PastixKernelGEMMCblk2d2d, gr.M, gr.N, gr.K, t } );
             }
         }
@@ -551,16 +570,17 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     // workgroup (LU: inversions and TRSM of both sides)
     const bool mlog = ctx->model_log && ctx->kernels;
     uint64_t *stamps_all = nullptr, *st_panel = nullptr, *st_inv = nullptr, *st_trsm = nullptr;
-    uint64_t *st_upd = nullptr, *st_updb = nullptr;
+    uint64_t *st_upd = nullptr, *st_updb = nullptr, *st_upds = nullptr;
     const bool    use_mw = ctx->panel_mw && ctx->kernels && !lu && !ldlt;
     const int64_t n_list = use_mw ? (int64_t)ctx->hpan.size() : (int64_t)ctx->hlist.size();
     const int64_t n_inv = ctx->hinvt.size(), n_trsm = ctx->htrsm.size();
-    int64_t n_upd = 0, n_updb = 0;
+    int64_t n_upd = 0, n_updb = 0, n_upds = 0;
     for ( const level_t &lv : ctx->levels ) {
         n_upd  = std::max( n_upd, lv.u1 );
         n_updb = std::max( n_updb, lv.b1 );
+        n_upds = std::max( n_upds, lv.s1 );
     }
-    const int64_t n_st = n_list + 2 * n_inv + 2 * n_trsm + n_upd + n_updb;
+    const int64_t n_st = n_list + 2 * n_inv + 2 * n_trsm + n_upd + n_updb + n_upds;
     if ( mlog ) {
         PXD_CHECK( hipMalloc( (void **)&stamps_all, 2 * sizeof( uint64_t ) * std::max<int64_t>( 1, n_st ) ) );
         PXD_CHECK( hipMemsetAsync( stamps_all, 0, 2 * sizeof( uint64_t ) * std::max<int64_t>( 1, n_st ), s ) );
@@ -569,6 +589,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         st_trsm  = st_inv + 4 * n_inv;
         st_upd   = st_trsm + 4 * n_trsm;
         st_updb  = st_upd + 2 * n_upd;
+        st_upds  = st_updb + 2 * n_updb;
     }
     // stamps of the tasks [i0, ...) of a launch (side 1: the U side's copy)
     auto stp = [&]( uint64_t *base, int64_t i0, int side, int64_t nside ) -> uint64_t * {
@@ -666,9 +687,18 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         launches++;
     };
     // update tiles [u0, u1) of the 64 x 64 list and [b0, b1) of the tall list
-    auto update = [&]( int64_t u0, int64_t u1, int64_t b0, int64_t b1, hipStream_t st, int prio ) {
+    auto update = [&]( int64_t u0, int64_t u1, int64_t b0, int64_t b1, int64_t q0, int64_t q1, hipStream_t st,
+                       int prio ) {
         unsigned nu = (unsigned)( u1 - u0 );
         unsigned nb = (unsigned)( b1 - b0 );
+        unsigned nq = (unsigned)( q1 - q0 );
+        if ( nq ) {
+            constexpr unsigned per = PXD_THREADS / 64;
+            hipLaunchKernelGGL( ( k_update_small<T> ), dim3( ( nq + per - 1 ) / per ), dim3( PXD_THREADS ), 0, st,
+                                ctx->dupds + q0, (int32_t)nq, ctx->dcb, ctx->dbk, L, U, ldlt,
+                                stp( st_upds, q0, 0, 0 ), prio );
+            launches++;
+        }
         if ( nb ) {
             hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
                                 (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
@@ -696,11 +726,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             trsm( lv, s );
             for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
                 const upd_group &gr = ctx->groups[g];
-                if ( gr.big ) {
-                    update( 0, 0, gr.t0, gr.t1, s, 0 );
+                if ( gr.big == 2 ) {
+                    update( 0, 0, 0, 0, gr.t0, gr.t1, s, 0 );
+                }
+                else if ( gr.big ) {
+                    update( 0, 0, gr.t0, gr.t1, 0, 0, s, 0 );
                 }
                 else {
-                    update( gr.t0, gr.t1, 0, 0, s, 0 );
+                    update( gr.t0, gr.t1, 0, 0, 0, 0, s, 0 );
                 }
             }
         }
@@ -719,7 +752,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_trsm += 1e-3 * ev_ms( ea, eb );
             PXD_CHECK( hipEventRecord( ea, s ) );
-            update( lv.u0, lv.u1, lv.b0, lv.b1, s, 0 );
+            update( lv.u0, lv.u1, lv.b0, lv.b1, lv.s0, lv.s1, s, 0 );
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_upd += 1e-3 * ev_ms( ea, eb );
@@ -739,9 +772,9 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             trsm( lv, s2 );
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
             PXD_CHECK( hipStreamWaitEvent( s, paneldone, 0 ) );
-            update( lv.u0, lv.uc, lv.b0, lv.bc, s, ctx->prio );
+            update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, s, ctx->prio );
             PXD_CHECK( hipEventRecord( ready, s ) );
-            update( lv.uc, lv.u1, lv.bc, lv.b1, s, 0 );
+            update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, s, 0 );
         }
     }
     PXD_CHECK( hipGetLastError() );
@@ -761,7 +794,8 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         PXD_CHECK( hipFree( stamps_all ) );
         int rate_khz = 0;
         PXD_CHECK( hipDeviceGetAttribute( &rate_khz, hipDeviceAttributeWallClockRate, ctx->device ) );
-        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, 1. / ( 1e3 * std::max( 1, rate_khz ) ), use_mw );
+        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, n_updb, 1. / ( 1e3 * std::max( 1, rate_khz ) ),
+                           use_mw );
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
@@ -1005,7 +1039,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
     for ( void *p : ptrs ) {
@@ -1052,6 +1086,11 @@ int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
+    if ( k == "upd_small" ) {
+        ctx->upd_small   = ( value != 0. );
+        ctx->tasks_dirty = 1;
+        return 0;
+    }
     if ( k == "panel_mw" ) {
         ctx->panel_mw = ( value != 0. );
         return 0;

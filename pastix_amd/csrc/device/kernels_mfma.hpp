@@ -388,7 +388,7 @@ struct pan_task {
     int32_t cblk;
     int32_t p;
     int32_t flag0;
-    int32_t pad;
+    int32_t nblk;    // blocks of the tile: the last one has no consumer
 };
 #define PXD_PANEL_WAIT_TICKS 200000000ull   // 2 s of the 100 MHz constant clock
 
@@ -468,6 +468,7 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
     T            *A    = L + c.coefoff;
     T            *LI   = Linv + c.dinvoff;
     int          *flag = flags + tk.flag0;
+    const bool    last = ( p == tk.nblk - 1 );   // nobody waits for the last block
     constexpr int NU   = PXD_NB * PXD_NB / PXD_THREADS;
     int           piv  = 0;
     typedef typename px_real<T>::type R;
@@ -522,7 +523,9 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
                 if ( r < bs ) A[( R0 + r ) + (int64_t)( PXD_NB * k + cc ) * n] = x;
             }
         }
-        panel_publish( flag + p, k + 1, tid );
+        if ( !last ) {
+            panel_publish( flag + p, k + 1, tid );
+        }
         __syncthreads();   // sT[k] complete for every wave
         // T(p,j) -= L(p,k) L(j,k)^T, j = k+1..p
         for ( int j = k + 1; j <= p; j++ ) {
@@ -610,7 +613,9 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
         }
         LI[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sI[cc][rr];
     }
-    panel_publish( flag + p, p + 1, tid );
+    if ( !last ) {
+        panel_publish( flag + p, p + 1, tid );
+    }
     if ( tid == 0 && piv > 0 ) {
         atomicAdd( npiv, piv );
     }
@@ -1295,6 +1300,99 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                     const int cc = wc + 16 * j + ( lane & 15 );
                     if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - M::get( acc[i][j], v );
                 }
+    }
+    stamp_end( stamps, blockIdx.x, t_stamp );
+}
+
+// ---------------------------------------------------------------------------
+// Small supernodal updates, one wavefront per blok update (the lower tree
+// has hundreds of thousands of them with a few rows and columns): update
+// {cblk, blok, flags} with at most 64 stacked rows (Mr), PXD_SMALL_N facing
+// columns (N) and PXD_SMALL_K source columns (K).  Lane r owns stacked row
+// r: C[f(r), :] -= A_r B^T, B (the rows of blok b, D-scaled for LDL^T) staged
+// in LDS; plain read-modify-write, or atomic adds when the task is flagged.
+#define PXD_SMALL_N 16
+#define PXD_SMALL_K 32
+template <typename T>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_update_small( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
+                const dblok *__restrict__ bk, T *L, T *U, int ldlt, uint64_t *__restrict__ stamps, int prio )
+{
+    __shared__ T sB[PXD_THREADS / 64][PXD_SMALL_K][PXD_SMALL_N + 1];
+    const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
+    const int     lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int32_t ti   = blockIdx.x * ( PXD_THREADS / 64 ) + w;
+    if ( ti < ntasks ) {
+        const upd_task t  = tasks[ti];
+        const dcblk    c  = cb[t.cblk];
+        const int32_t  K  = c.lcolnum - c.fcolnum + 1;
+        const int32_t  lb = cb[t.cblk + 1].fblok;
+        const dblok    bb = bk[t.blok];
+        const int32_t  N  = bb.lrownum - bb.frownum + 1;
+        const bool     uside = ( t.flags & 2 ) != 0;
+        const T       *LA = uside ? U : L;
+        const T       *LB = uside ? L : U;
+        T             *LC = uside ? U : L;
+        const int32_t  r0 = bb.rowoff + ( uside ? N : 0 );
+        const int32_t  Mr = c.stride - r0;
+        const int32_t  F  = bb.fcblknm;
+        const dcblk    cf = cb[F];
+        const int32_t  Fn = cf.lcolnum - cf.fcolnum + 1;
+        const int32_t  flb = cb[F + 1].fblok;
+        const int32_t  colF0 = bb.frownum - cf.fcolnum;
+        const int64_t  bbase = c.coefoff + (int64_t)bb.rowoff * K;
+        // B[j][k] = LB[bbase + j + k N], times D_k for LDL^T
+        for ( int idx = lane; idx < N * K; idx += 64 ) {
+            const int j = idx % N, kk = idx / N;
+            const T   v = LB[bbase + j + (int64_t)kk * N];
+            sB[w][kk][j] = ldlt ? v * L[c.coefoff + (int64_t)kk * ( K + 1 )] : v;
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        if ( lane < Mr ) {
+            const int32_t s   = r0 + lane;
+            const int32_t bi  = find_blok_by_row( bk, t.blok, lb, s );
+            const dblok   ib  = bk[bi];
+            const int64_t src = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
+            const int32_t sld = ib.lrownum - ib.frownum + 1;
+            const int32_t g   = ib.frownum + ( s - ib.rowoff );
+            const int32_t fi  = find_blok_by_grow( bk, cf.fblok, flb, g );
+            const dblok   fb  = bk[fi];
+            T a[PXD_SMALL_K];
+#pragma unroll
+            for ( int kk = 0; kk < PXD_SMALL_K; kk++ ) a[kk] = ( kk < K ) ? LA[src + (int64_t)kk * sld] : (T)0;
+            T acc[PXD_SMALL_N];
+#pragma unroll
+            for ( int j = 0; j < PXD_SMALL_N; j++ ) acc[j] = (T)0;
+#pragma unroll
+            for ( int kk = 0; kk < PXD_SMALL_K; kk++ ) {
+                if ( kk < K ) {
+#pragma unroll
+                    for ( int j = 0; j < PXD_SMALL_N; j++ ) acc[j] += a[kk] * sB[w][kk][j];
+                }
+            }
+            if ( g >= fb.frownum && g <= fb.lrownum ) {
+                const int32_t dld = fb.lrownum - fb.frownum + 1;
+                T *dst = LC + cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum ) + (int64_t)colF0 * dld;
+                if ( t.flags & 1 ) {
+#pragma unroll
+                    for ( int j = 0; j < PXD_SMALL_N; j++ ) {
+                        if ( j < N ) px_atomic_add( dst + (int64_t)j * dld, -acc[j] );
+                    }
+                }
+                else {
+                    T old[PXD_SMALL_N];
+#pragma unroll
+                    for ( int j = 0; j < PXD_SMALL_N; j++ ) old[j] = ( j < N ) ? dst[(int64_t)j * dld] : (T)0;
+#pragma unroll
+                    for ( int j = 0; j < PXD_SMALL_N; j++ ) {
+                        if ( j < N ) dst[(int64_t)j * dld] = old[j] - acc[j];
+                    }
+                }
+            }
+        }
     }
     stamp_end( stamps, blockIdx.x, t_stamp );
 }

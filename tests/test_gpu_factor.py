@@ -60,12 +60,14 @@ def test_solve_fp64_backward_error(nx, nrhs, kernels):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("big_rows,timing,panel_mw", [(1, 0, 1), (10 ** 9, 0, 1), (192, 1, 1), (192, 0, 0)])
-def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing, panel_mw):
+@pytest.mark.parametrize("big_rows,timing,panel_mw,small", [(1, 0, 1, 0), (10 ** 9, 0, 1, 0), (10 ** 9, 0, 1, 1),
+                                                           (192, 1, 1, 1), (192, 0, 0, 1)])
+def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing, panel_mw, small):
     """Every blok update on the tall tiles (big_rows=1), every one on the
-    64 x 64 tiles (big_rows=1e9), the single-stream timed schedule instead
-    of the two-stream lookahead, and the single-workgroup panel instead of
-    the one-workgroup-per-block panel: the factor matches the oracle."""
+    64 x 64 tiles (big_rows=1e9), the small updates on one wavefront each
+    (small=1) or on tiles, the single-stream timed schedule instead of the
+    two-stream lookahead, and the single-workgroup panel instead of the
+    one-workgroup-per-block panel: the factor matches the oracle."""
     nx = 18
     mixed = int(dtype == np.float32)
     spm = make_problem("lap7", nx)
@@ -73,6 +75,7 @@ def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing, pane
     px.set_option(pd, "upd_big_rows", big_rows)
     px.set_option(pd, "timing", timing)
     px.set_option(pd, "panel_mw", panel_mw)
+    px.set_option(pd, "upd_small", small)
     px.task_numfact(pd, spm)
     s = px.solver_export(pd)
     L = px.get_coeftab(pd, s["coefnbr"], dtype)
