@@ -59,6 +59,7 @@ struct pxd_ctx_s {
     int         device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;    // panels of the next level (lookahead)
+
     hipEvent_t  ev_ready = nullptr, ev_panel = nullptr;
     size_t      lds_max = 65536;
 
@@ -118,6 +119,8 @@ struct pxd_ctx_s {
     // 1: updates of at most 64 rows, PXD_SMALL_N columns and PXD_SMALL_K
     // source columns take one wavefront each (k_update_small)
     int          upd_small = 1;
+    // k values per stage of the tall-tile update: 4 * KV * upd_nh (1 or 2)
+    int          upd_nh = 1;
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
 
@@ -700,8 +703,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             launches++;
         }
         if ( nb ) {
-            hipLaunchKernelGGL( ( k_update_big<T> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
+            if ( ctx->upd_nh == 2 ) {
+                hipLaunchKernelGGL( ( k_update_big<T, 2> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
+                                    (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_update_big<T, 1> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
+                                    (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
+            }
             launches++;
         }
         if ( nu && ctx->kernels ) {
@@ -764,17 +773,18 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         // remaining updates of level L go on in stream s.  Those only write
         // cblks of later levels, so the two streams touch disjoint data.
         hipStream_t s2 = ctx->stream2;
+        hipStream_t su = s;
         hipEvent_t  ready = ctx->ev_ready, paneldone = ctx->ev_panel;
-        PXD_CHECK( hipEventRecord( ready, s ) );
+        PXD_CHECK( hipEventRecord( ready, su ) );
         for ( const level_t &lv : ctx->levels ) {
             PXD_CHECK( hipStreamWaitEvent( s2, ready, 0 ) );
             panel( lv, s2 );
             trsm( lv, s2 );
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
-            PXD_CHECK( hipStreamWaitEvent( s, paneldone, 0 ) );
-            update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, s, ctx->prio );
-            PXD_CHECK( hipEventRecord( ready, s ) );
-            update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, s, 0 );
+            PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
+            update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, su, ctx->prio );
+            PXD_CHECK( hipEventRecord( ready, su ) );
+            update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, su, 0 );
         }
     }
     PXD_CHECK( hipGetLastError() );
@@ -1051,6 +1061,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     (void)hipEventDestroy( ctx->ev_ready );
     (void)hipEventDestroy( ctx->ev_panel );
     (void)hipStreamDestroy( ctx->stream2 );
+
     (void)hipStreamDestroy( ctx->stream );
     delete ctx;
 }
@@ -1086,6 +1097,10 @@ int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
+    if ( k == "upd_nh" ) {
+        ctx->upd_nh = ( value >= 2. ) ? 2 : 1;
+        return 0;
+    }
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;

@@ -1457,8 +1457,8 @@ template <> struct upd_rows<double> { static constexpr int value = 128; };
 template <> struct upd_rows<pxcf>   { static constexpr int value = 128; };
 template <> struct upd_rows<pxcd>   { static constexpr int value = 64; };
 
-template <typename T>
-__global__ void __launch_bounds__( PXD_THREADS, 3 )
+template <typename T, int NH>
+__global__ void __launch_bounds__( PXD_THREADS, ( NH == 1 ) ? 3 : 2 )
 k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
               const dblok *__restrict__ bk, T *L, T *U, int ldlt,
               uint64_t *__restrict__ stamps, int prio )
@@ -1468,7 +1468,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     typedef typename lds_vec<T>::type vec_t;
     constexpr int BM  = upd_rows<T>::value;
     constexpr int KV  = lds_vec<T>::n;    // k values per fragment
-    constexpr int BK  = 4 * KV;           // k values per stage
+    constexpr int BK  = NH * 4 * KV;      // k values per stage (NH MFMA k-groups of 4*KV)
     constexpr int LDK = BK + KV;          // row stride of the staged tiles
     constexpr int TPR = PXD_THREADS / BM; // loader threads per row
     constexpr int KPT = BK / TPR;         // k values loaded per thread
@@ -1553,7 +1553,7 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     const uint32_t bbase  = ( bc < nvalid ) ? (uint32_t)( bb.rowoff * K + t.tn * PXD_UPD_BN + bc ) * ES : PXD_OOB;
     const uint32_t bstep  = ( bc < nvalid ) ? (uint32_t)N * ES : 0;
 
-    T ra[KPT], rb[KV];
+    T ra[KPT], rb[NH * KV];
     auto load = [&]( int32_t k0 ) {
 #pragma unroll
         for ( int q = 0; q < KPT; q++ ) {
@@ -1561,8 +1561,8 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
             ra[q] = bufio<T>::load( rA, ( kg < K ) ? abase + (uint32_t)kg * astep : PXD_OOB );
         }
 #pragma unroll
-        for ( int q = 0; q < KV; q++ ) {
-            const int32_t kg = k0 + bq * KV + q;
+        for ( int q = 0; q < NH * KV; q++ ) {
+            const int32_t kg = k0 + bq * NH * KV + q;
             rb[q] = bufio<T>::load( rB, ( kg < K ) ? bbase + (uint32_t)kg * bstep : PXD_OOB );
         }
     };
@@ -1583,30 +1583,35 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
             for ( int e = 0; e < KV; e++ ) v[e] = ra[q + e];
             *(vec_t *)&sA[ar * LDK + ap * KPT + q] = v;
         }
-        {
+#pragma unroll
+        for ( int h = 0; h < NH; h++ ) {
             vec_t v;
 #pragma unroll
             for ( int e = 0; e < KV; e++ ) {
-                const int32_t kg = k0 + bq * KV + e;
-                v[e] = ( ldlt && kg < K ) ? rb[e] * sDk[kg] : rb[e];
+                const int32_t kg = k0 + bq * NH * KV + h * KV + e;
+                v[e] = ( ldlt && kg < K ) ? rb[h * KV + e] * sDk[kg] : rb[h * KV + e];
             }
-            *(vec_t *)&sB[bc * LDK + bq * KV] = v;
+            *(vec_t *)&sB[bc * LDK + bq * NH * KV + h * KV] = v;
         }
         __syncthreads();
         if ( k0 + BK < K ) {
             load( k0 + BK );
         }
-        vec_t av[RB], bv[4];
 #pragma unroll
-        for ( int i = 0; i < RB; i++ ) av[i] = *(const vec_t *)&sA[( 16 * ( RB * w + i ) + lr ) * LDK + KV * lg];
-#pragma unroll
-        for ( int j = 0; j < 4; j++ ) bv[j] = *(const vec_t *)&sB[( 16 * j + lr ) * LDK + KV * lg];
-#pragma unroll
-        for ( int s = 0; s < KV; s++ )
+        for ( int h = 0; h < NH; h++ ) {
+            vec_t av[RB], bv[4];
 #pragma unroll
             for ( int i = 0; i < RB; i++ )
+                av[i] = *(const vec_t *)&sA[( 16 * ( RB * w + i ) + lr ) * LDK + 4 * KV * h + KV * lg];
 #pragma unroll
-                for ( int j = 0; j < 4; j++ ) acc[i][j] = M::mma( av[i][s], bv[j][s], acc[i][j] );
+            for ( int j = 0; j < 4; j++ ) bv[j] = *(const vec_t *)&sB[( 16 * j + lr ) * LDK + 4 * KV * h + KV * lg];
+#pragma unroll
+            for ( int s = 0; s < KV; s++ )
+#pragma unroll
+                for ( int i = 0; i < RB; i++ )
+#pragma unroll
+                    for ( int j = 0; j < 4; j++ ) acc[i][j] = M::mma( av[i][s], bv[j][s], acc[i][j] );
+        }
         __syncthreads();
     }
 

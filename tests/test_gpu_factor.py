@@ -124,3 +124,20 @@ def test_fp64_gmres_reference_criterion():
     r = b - spm.matmat(x)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
     px.finalize(pd)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_tall_tile_stage_depth_matches_oracle(dtype):
+    """Every blok update on the tall tiles with two MFMA k-groups per LDS
+    stage (upd_nh=2): the factor matches the oracle."""
+    nx = 18
+    spm = make_problem("lap7", nx)
+    pd, iparm, dparm = analyzed(spm, nx, mixed=int(dtype == np.float32))
+    px.set_option(pd, "upd_big_rows", 1)
+    px.set_option(pd, "upd_nh", 2)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    L = px.get_coeftab(pd, s["coefnbr"], dtype)
+    F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype)
+    assert cblk_diff(s, L, F.L, dtype) <= 10.0
+    px.finalize(pd)
