@@ -1013,22 +1013,42 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 // Off-diagonal TRSM on the matrix cores: X = B * Linv^T for a tile of 64
 // off-diagonal rows and all n columns of the cblk, in place (the workgroup
 // owns complete rows, so every read of B precedes every write of X).
-// Wave w computes rows 16w..16w+15 against every 16-column tile; column tiles
-// that only meet the zero upper part of Linv are skipped.
+// Columns go in passes of NTP = 4 * CPW 16-column tiles, the highest pass
+// first: Linv is lower triangular, so a pass reads only B columns up to its
+// own last one and may overwrite its columns before the lower passes run.
+// In a pass every wave covers all 64 rows against CPW column tiles (tiles
+// w, w+4, ...), so each LDS fragment feeds 4 or CPW products; the k values
+// of an MFMA step are permuted (lane group g takes k = KV*g + s) so that
+// every fragment is one 16-byte read.  Stages whose k range lies above a
+// column tile (Linv zero there) are skipped for that tile.
+template <typename T> struct trsm_cpw;
+template <> struct trsm_cpw<float>  { static constexpr int value = 2; };
+template <> struct trsm_cpw<double> { static constexpr int value = 2; };
+template <> struct trsm_cpw<pxcf>   { static constexpr int value = 2; };
+template <> struct trsm_cpw<pxcd>   { static constexpr int value = 1; };
+
 template <typename T>
-__global__ void __launch_bounds__( PXD_THREADS )
+__global__ void __launch_bounds__( PXD_THREADS, 2 )
 k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
              const dblok *__restrict__ bk, T *__restrict__ L, const T *__restrict__ Linv, int ldlt,
              uint64_t *__restrict__ stamps, int prio )
 {
-    typedef mfma16<T>          M;
-    typedef typename M::acc_t  acc_t;
-    constexpr int BK = 16;
-    __shared__ T       sB[BK][64 + 1];
-    __shared__ T       sLi[BK][PXD_TRSM_NT * 16 + 1];
+    typedef mfma16<T>                 M;
+    typedef typename M::acc_t         acc_t;
+    typedef typename lds_vec<T>::type vec_t;
+    constexpr int KV  = lds_vec<T>::n;
+    constexpr int BK  = 4 * KV;                      // k values per stage
+    constexpr int LDK = BK + KV;
+    constexpr int CPW = trsm_cpw<T>::value;          // column tiles per wave
+    constexpr int NTP = 4 * CPW;                     // column tiles per pass
+    constexpr int CPT = NTP * 16;                    // columns per pass
+    constexpr int TPC = PXD_THREADS / CPT;           // Linv loader threads per column
+    constexpr int KPC = BK / TPC;                    // k values per Linv loader thread
+    __shared__ __align__( 16 ) T sB[64 * LDK];       // B rows, k contiguous
+    __shared__ __align__( 16 ) T sLi[CPT * LDK];     // Linv columns of the pass, k contiguous
     __shared__ int64_t sBase[64];
     __shared__ int32_t sLd[64];
-    __shared__ T       sDi[PXD_TRSM_NT * 16];   // LDL^T: 1 / D of the cblk
+    __shared__ T       sDi[PXD_TRSM_NT * 16];        // LDL^T: 1 / D of the cblk
 
     const row_task t    = tasks[blockIdx.x];
     const uint64_t t_stamp = stamp_begin( stamps );
@@ -1038,6 +1058,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     const int32_t  lb   = cb[t.cblk + 1].fblok;
     const int      tid  = threadIdx.x;
     const int      lane = tid & 63, w = tid >> 6;
+    const int      lr   = lane & 15, lg = lane >> 4;
     const T       *LI   = Linv + c.dinvoff;
     const int      nt   = ( n + 15 ) >> 4;
 
@@ -1059,77 +1080,86 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     }
     __syncthreads();
 
-    const int     lr    = tid & 63;          // row this thread loads
-    const int64_t lbase = sBase[lr];
-    const int32_t lld   = sLd[lr];
+    // B loader: row br, k values [bq * KV, bq * KV + KV) of a stage
+    const int     br    = tid & 63, bq = tid >> 6;
+    const int64_t lbase = sBase[br];
+    const int32_t lld   = sLd[br];
+    // Linv loader: column lc of the pass, k values [lq * KPC, lq * KPC + KPC)
+    const int     lc = tid % CPT, lq = tid / CPT;
 
-    // Column passes of NTP 16-column tiles, the highest first: Linv is
-    // lower triangular, so a pass reads only B columns up to its own last
-    // one and may overwrite its columns in place before the lower passes run.
-    constexpr int NTP = ( sizeof( T ) <= 8 ) ? PXD_TRSM_NT / 2 : 6;
-    constexpr int NLI = ( BK * NTP * 16 + PXD_THREADS - 1 ) / PXD_THREADS;
-    const int     npass = ( nt + NTP - 1 ) / NTP;
+    const int npass = ( nt + NTP - 1 ) / NTP;
     for ( int pass = npass - 1; pass >= 0; pass-- ) {
-        const int32_t cb0  = pass * NTP * 16;                  // first column of the pass
-        const int32_t lend = min( n, cb0 + NTP * 16 );         // Linv[c][l] = 0 for l > c
-        acc_t acc[NTP];
+        const int32_t cb0  = pass * CPT;                       // first column of the pass
+        const int32_t lend = min( n, cb0 + CPT );              // Linv[c][l] = 0 for l > c
+        acc_t acc[4][CPW];
 #pragma unroll
-        for ( int j = 0; j < NTP; j++ ) acc[j] = M::zero();
+        for ( int i = 0; i < 4; i++ )
+#pragma unroll
+            for ( int j = 0; j < CPW; j++ ) acc[i][j] = M::zero();
 
         // the next stage's loads are issued before the current stage's products
-        T    rb[BK / 4], rl[NLI];
+        T    rb[KV], rl[KPC];
         auto load = [&]( int32_t l0 ) {
 #pragma unroll
-            for ( int u = 0; u < BK / 4; u++ ) {
-                const int32_t l = l0 + ( tid >> 6 ) + 4 * u;
-                rb[u] = ( lbase >= 0 && l < lend ) ? L[lbase + (int64_t)l * lld] : (T)0;
+            for ( int q = 0; q < KV; q++ ) {
+                const int32_t l = l0 + bq * KV + q;
+                rb[q] = ( lbase >= 0 && l < lend ) ? L[lbase + (int64_t)l * lld] : (T)0;
             }
 #pragma unroll
-            for ( int u = 0; u < NLI; u++ ) {
-                const int     idx = tid + u * PXD_THREADS;
-                const int     ci = idx % ( NTP * 16 ), kk = idx / ( NTP * 16 );
-                const int32_t cc = cb0 + ci, l = l0 + kk;
-                rl[u] = ( kk < BK && cc < n && l < lend && 16 * ( cc >> 4 ) + 15 >= l ) ? LI[cc + (int64_t)l * n]
-                                                                                       : (T)0;
+            for ( int q = 0; q < KPC; q++ ) {
+                const int32_t cc = cb0 + lc, l = l0 + lq * KPC + q;
+                rl[q] = ( cc < n && l <= cc ) ? LI[cc + (int64_t)l * n] : (T)0;
             }
         };
         load( 0 );
         for ( int32_t l0 = 0; l0 < lend; l0 += BK ) {
+            {
+                vec_t v;
 #pragma unroll
-            for ( int u = 0; u < BK / 4; u++ ) sB[( tid >> 6 ) + 4 * u][lr] = rb[u];
+                for ( int e = 0; e < KV; e++ ) v[e] = rb[e];
+                *(vec_t *)&sB[br * LDK + bq * KV] = v;
+            }
 #pragma unroll
-            for ( int u = 0; u < NLI; u++ ) {
-                const int idx = tid + u * PXD_THREADS;
-                if ( idx < BK * NTP * 16 ) sLi[idx / ( NTP * 16 )][idx % ( NTP * 16 )] = rl[u];
+            for ( int q = 0; q < KPC; q += KV ) {
+                vec_t v;
+#pragma unroll
+                for ( int e = 0; e < KV; e++ ) v[e] = rl[q + e];
+                *(vec_t *)&sLi[lc * LDK + lq * KPC + q] = v;
             }
             __syncthreads();
             if ( l0 + BK < lend ) {
                 load( l0 + BK );
             }
+            vec_t av[4], bv[CPW];
 #pragma unroll
-            for ( int k4 = 0; k4 < BK; k4 += 4 ) {
-                const T a = sB[k4 + ( lane >> 4 )][16 * w + ( lane & 15 )];
+            for ( int i = 0; i < 4; i++ ) av[i] = *(const vec_t *)&sB[( 16 * i + lr ) * LDK + KV * lg];
 #pragma unroll
-                for ( int j = 0; j < NTP; j++ ) {
-                    if ( cb0 + 16 * j < n && cb0 + 16 * j + 15 >= l0 + k4 ) {
-                        const T b = sLi[k4 + ( lane >> 4 )][16 * j + ( lane & 15 )];
-                        acc[j] = M::mma( a, b, acc[j] );
-                    }
+            for ( int j = 0; j < CPW; j++ ) bv[j] = *(const vec_t *)&sLi[( 16 * ( w + 4 * j ) + lr ) * LDK + KV * lg];
+#pragma unroll
+            for ( int j = 0; j < CPW; j++ ) {
+                const int32_t c0 = cb0 + 16 * ( w + 4 * j );   // first column of the tile
+                if ( c0 < n && c0 + 15 >= l0 ) {
+#pragma unroll
+                    for ( int e = 0; e < KV; e++ )
+#pragma unroll
+                        for ( int i = 0; i < 4; i++ ) acc[i][j] = M::mma( av[i][e], bv[j][e], acc[i][j] );
                 }
             }
             __syncthreads();
         }
 #pragma unroll
-        for ( int j = 0; j < NTP; j++ ) {
-            const int cc = cb0 + 16 * j + ( lane & 15 );
+        for ( int j = 0; j < CPW; j++ ) {
+            const int cc = cb0 + 16 * ( w + 4 * j ) + lr;
 #pragma unroll
-            for ( int v = 0; v < 4; v++ ) {
-                const int r = 16 * w + M::row( lane, v );
-                if ( sBase[r] >= 0 && cc < n ) {
-                    // LDL^T: L_off = B L_kk^{-T} D^{-1}
-                    L[sBase[r] + (int64_t)cc * sLd[r]] = ldlt ? M::get( acc[j], v ) * sDi[cc] : M::get( acc[j], v );
+            for ( int i = 0; i < 4; i++ )
+#pragma unroll
+                for ( int v = 0; v < 4; v++ ) {
+                    const int r = 16 * i + M::row( lane, v );
+                    if ( sBase[r] >= 0 && cc < n ) {
+                        // LDL^T: L_off = B L_kk^{-T} D^{-1}
+                        L[sBase[r] + (int64_t)cc * sLd[r]] = ldlt ? M::get( acc[i][j], v ) * sDi[cc] : M::get( acc[i][j], v );
+                    }
                 }
-            }
         }
         __syncthreads();   // the pass's stores precede the next pass's loads
     }
