@@ -518,6 +518,9 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
     }
 }
 
+// solve launches with fewer workgroups than this load every column at once
+#define PXD_SOLVE_WIDE_MAX 2048
+
 // ---------------------------------------------------------------------------
 template <typename T>
 int
@@ -858,9 +861,13 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
             hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
                                 ctx->dcb, DI, (const T *)X, ldx, Y, ldy, nrhs, 0 );
-            if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+            if ( nf >= PXD_SOLVE_WIDE_MAX ) {
+                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                     ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
+            }
+            else if ( nf ) {
+                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, solve_wide<T>::value> ), dim3( nf ), dim3( PXD_THREADS ), sh,
+                                    s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
             }
         }
         if ( ctx->facto == FACT_SYTRF ) {
@@ -872,9 +879,14 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
-            if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+            if ( nf >= PXD_SOLVE_WIDE_MAX ) {
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
                                     ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const T *)X, ldx, Y, ldy, nrhs );
+            }
+            else if ( nf ) {
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, solve_wide<T>::value / 16> ), dim3( nf ),
+                                    dim3( PXD_THREADS ), 0, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB,
+                                    (const T *)X, ldx, Y, ldy, nrhs );
             }
             hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
                                 ctx->dcb, DB, (const T *)Y, ldy, X, ldx, nrhs, 1 );
@@ -896,7 +908,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
                             ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
         if ( nf ) {
-            hipLaunchKernelGGL( ( k_solve_fwd_upd<T> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+            hipLaunchKernelGGL( ( k_solve_fwd_upd<T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                 ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)X, ldx );
         }
     }
@@ -929,7 +941,9 @@ set_lds_attributes_mfma( size_t lds )
     }
     (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
         (void)hipFuncSetAttribute( (const void *)k_potrf_mw<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
@@ -945,7 +959,9 @@ set_lds_attributes( size_t lds )
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
     (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );

@@ -52,6 +52,23 @@ struct row_task {
 #define PXD_TRSM_NT 22     // 16-column tiles of the widest cblk (352 columns)
 #define PXD_THREADS 256
 
+// Largest i in [0, n) with a[i] <= key (a ascending, a[0] <= key).
+__device__ inline int32_t
+lds_search_le( const int32_t *a, int32_t n, int32_t key )
+{
+    int32_t lo = 0, hi = n - 1;
+    while ( lo < hi ) {
+        const int32_t mid = ( lo + hi + 1 ) >> 1;
+        if ( a[mid] <= key ) {
+            lo = mid;
+        }
+        else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
 // Blok of cblk k (bloks [fb, lb)) that holds stacked row s.
 __device__ inline int32_t
 find_blok_by_row( const dblok *bk, int32_t fb, int32_t lb, int32_t s )
@@ -84,6 +101,23 @@ find_blok_by_grow( const dblok *bk, int32_t fb, int32_t lb, int32_t g )
         }
     }
     return lo;
+}
+
+// Row offsets of the bloks [fb, lb) of one cblk staged in LDS by one
+// parallel load, for the row searches of a workgroup (lists longer than
+// PXD_ROWKEY_CAP are searched in global memory).
+#define PXD_ROWKEY_CAP 512
+__device__ inline void
+stage_row_keys( int32_t *keys, const dblok *__restrict__ bk, int32_t fb, int32_t lb )
+{
+    if ( lb - fb <= PXD_ROWKEY_CAP ) {
+        for ( int32_t i = threadIdx.x; i < lb - fb; i += blockDim.x ) keys[i] = bk[fb + i].rowoff;
+    }
+}
+__device__ inline int32_t
+search_row_keys( const int32_t *keys, const dblok *__restrict__ bk, int32_t fb, int32_t lb, int32_t s )
+{
+    return ( lb - fb <= PXD_ROWKEY_CAP ) ? fb + lds_search_le( keys, lb - fb, s ) : find_blok_by_row( bk, fb, lb, s );
 }
 
 // ---------------------------------------------------------------------------

@@ -59,23 +59,6 @@ stamp_end( uint64_t *stamps, int64_t i, uint64_t t0 )
     }
 }
 
-// Largest i in [0, n) with a[i] <= key (a ascending, a[0] <= key).
-__device__ inline int32_t
-lds_search_le( const int32_t *a, int32_t n, int32_t key )
-{
-    int32_t lo = 0, hi = n - 1;
-    while ( lo < hi ) {
-        const int32_t mid = ( lo + hi + 1 ) >> 1;
-        if ( a[mid] <= key ) {
-            lo = mid;
-        }
-        else {
-            hi = mid - 1;
-        }
-    }
-    return lo;
-}
-
 // Keys of the row searches of an update tile, staged in LDS with one parallel
 // load: stacked row offsets of the source bloks [b, lb) and first rows of the
 // facing cblk's bloks [ffb, flb).  Lists longer than PXD_SEARCH_CAP are
@@ -1065,10 +1048,13 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     if ( ldlt ) {
         for ( int32_t i = tid; i < n; i += PXD_THREADS ) sDi[i] = (T)1 / L[c.coefoff + (int64_t)i * ( n + 1 )];
     }
+    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
+    stage_row_keys( skeys, bk, c.fblok, lb );
+    __syncthreads();
     if ( tid < 64 ) {
         int32_t s = t.r0 + tid;
         if ( s < c.stride ) {
-            int32_t b  = find_blok_by_row( bk, c.fblok, lb, s );
+            int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
             dblok   bb = bk[b];
             sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
             sLd[tid]   = bb.lrownum - bb.frownum + 1;
@@ -1764,7 +1750,7 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
 // Backward gather of a 64-row tile: x_k -= A(rows,:)^T X[grow].  The tile is
 // staged through LDS by 64-column chunks (coalesced loads), four threads per
 // column reduce 16 rows each, one atomic per column and RHS.
-template <typename T>
+template <typename T, int NCH>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                      const dblok *__restrict__ bk, const T *__restrict__ L, const T *X,
@@ -1781,11 +1767,14 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
     const int32_t  n   = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb  = cb[t.cblk + 1].fblok;
     const int      tid = threadIdx.x;
+    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
+    stage_row_keys( skeys, bk, c.fblok, lb );
+    __syncthreads();
 
     if ( tid < 64 ) {
         int32_t s = t.r0 + tid;
         if ( s < c.stride ) {
-            int32_t b  = find_blok_by_row( bk, c.fblok, lb, s );
+            int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
             dblok   bb = bk[b];
             sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
             sLd[tid]   = bb.lrownum - bb.frownum + 1;
@@ -1800,20 +1789,30 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
     }
     __syncthreads();
     const int col = tid & 63, part = tid >> 6;
+    // NCH 64-column chunks are loaded at once, then staged through LDS one
+    // after the other (NCH = 1 for launches with many workgroups)
+    constexpr int NU  = 64 * 64 / PXD_THREADS;
+    const int     rr0 = tid & 63;
+    const int64_t rb  = sBase[rr0];
+    const int32_t rl  = sLd[rr0];
+    T             v[NCH][NU];
     for ( int32_t c0 = 0; c0 < n; c0 += 64 ) {
-        {
-            constexpr int NU = 64 * 64 / PXD_THREADS;
-            const int     r  = tid & 63;
-            const int64_t rb = sBase[r];
-            const int32_t rl = sLd[r];
-            T             v[NU];
+        const int ch = ( c0 / 64 ) % NCH;
+        if ( ch == 0 ) {
 #pragma unroll
-            for ( int u = 0; u < NU; u++ ) {
-                const int cc = ( tid >> 6 ) + 4 * u;
-                v[u] = ( rb >= 0 && c0 + cc < n ) ? L[rb + (int64_t)( c0 + cc ) * rl] : (T)0;
+            for ( int h = 0; h < NCH; h++ )
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int32_t cc = c0 + 64 * h + ( tid >> 6 ) + 4 * u;
+                    v[h][u] = ( rb >= 0 && cc < n ) ? L[rb + (int64_t)cc * rl] : (T)0;
+                }
+        }
+#pragma unroll
+        for ( int h = 0; h < NCH; h++ ) {
+            if ( h == ch ) {
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) sA[( tid >> 6 ) + 4 * u][rr0] = v[h][u];
             }
-#pragma unroll
-            for ( int u = 0; u < NU; u++ ) sA[( tid >> 6 ) + 4 * u][r] = v[u];
         }
         __syncthreads();
         T acc[PXD_MAXRHS];

@@ -86,7 +86,13 @@ k_solve_diag( const int32_t *__restrict__ list, const dcblk *__restrict__ cb,
 
 // Forward update of a 64-row tile of the off-diagonal rows of a cblk:
 // X[grow] -= A(rows, :) * x_k, four wave-groups splitting the columns.
-template <typename T>
+// B columns per load batch: 16 for launches with many workgroups (the
+// lower tree, where occupancy matters), every column at once (up to 96
+// registers) for launches with few, long workgroups (the upper tree).
+template <typename T> struct solve_wide {
+    static constexpr int value = 96 * 4 / (int)sizeof( T );
+};
+template <typename T, int B>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                  const dblok *__restrict__ bk, const T *__restrict__ L, T *X,
@@ -108,6 +114,8 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
         int rr = idx % n, q = idx / n;
         sx[rr * nrhs + q] = Xs[c.fcolnum + rr + q * lds];
     }
+    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
+    stage_row_keys( skeys, bk, c.fblok, lb );
     __syncthreads();
 
     int32_t s     = t.r0 + r;
@@ -115,7 +123,7 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
     int64_t base  = 0;
     int32_t ld = 0, g = 0;
     if ( valid ) {
-        int32_t b  = find_blok_by_row( bk, c.fblok, lb, s );
+        int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
         dblok   bb = bk[b];
         base = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
         ld   = bb.lrownum - bb.frownum + 1;
@@ -125,8 +133,7 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
 #pragma unroll
     for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
     if ( valid ) {
-        // 16 columns per batch: the loads of a batch are all in flight at once
-        constexpr int B  = 16;
+        // the loads of a batch of B columns are all in flight at once
         const int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
         for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
             T a[B];
