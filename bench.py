@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=60, help="grid edge of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
+                    help="c2 (default): the headline 100^3 Cholesky line; c1, c3, c4, c5: the other "
+                         "configurations, each on its own JSON line")
     ap.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
                     help="engine option for both paths (e.g. prio=0, upd_big_rows=256)")
     ap.add_argument("--profile-stats", action="store_true", help="per-kernel-class event timing pass")
@@ -57,18 +60,20 @@ def log(*a):
 class Run:
     """One analysed problem with the matrix and the RHS resident on the device."""
 
-    def __init__(self, px, spm, n, nrhs, mixed, berr, opts=()):
+    def __init__(self, px, spm, n, nrhs, mixed, berr, opts=(), facto=None, refine=None, dof=1):
         self.px = px
         self.spm = spm
         iparm, dparm = px.initParam()
         iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
-        iparm[px.IPARM_FACTORIZATION] = px.PastixFactPOTRF
+        iparm[px.IPARM_FACTORIZATION] = px.PastixFactPOTRF if facto is None else facto
+        if refine is not None:
+            iparm[px.IPARM_REFINEMENT] = refine
         iparm[px.IPARM_MIXED] = mixed
         iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
         self.iparm, self.dparm = iparm, dparm
         self.pd = px.init(iparm, dparm)
         t0 = time.perf_counter()
-        px.analyze_grid(self.pd, spm, n, n, n)
+        px.analyze_grid(self.pd, spm, n, n, n, dof=dof)
         self.t_analyze = time.perf_counter() - t0
         rc = px.lib.pastix_subtask_spm2bcsc(self.pd, ctypes.byref(spm.s))
         rc |= px.lib.pastix_subtask_bcsc2ctab(self.pd)
@@ -115,7 +120,7 @@ class Run:
         return t1 - t0, t2 - t1, t3 - t2, it.value
 
     def solution(self):
-        xp = np.zeros((self.N, self.nrhs), order="F")
+        xp = np.zeros((self.N, self.nrhs), dtype=self.spm.dtype, order="F")
         self.px.lib.pxd_d2h(self.dev, xp.ctypes.data_as(ctypes.c_void_p), self.dx, self.nbytes)
         return xp[self.perm]
 
@@ -188,6 +193,78 @@ def cpu_baseline(px, nsample, nrhs):
                 fact_seconds=F.seconds, solve_seconds=tsolve, thflops=fl)
 
 
+# The other configurations of the benchmark suite (the headline is c2):
+# generator, grid edge, unknowns per node, factorization, refinement, RHS,
+# backward-error target, paths (label, IPARM_MIXED).
+CONFIGS = {
+    "c1": dict(desc="20^3 7-point Laplacian, FP64 Cholesky, 1 RHS", gen=("lap7",), nx=20, dof=1,
+               facto="PastixFactPOTRF", refine=None, nrhs=1, berr=1e-13, paths=(("fp64", 0),)),
+    "c2": dict(desc="100^3 7-point Laplacian, POTRF, 4 RHS, grid nested dissection", gen=("lap7",), nx=100,
+               dof=1, facto="PastixFactPOTRF", refine=None, nrhs=4, berr=1e-12,
+               paths=(("fp32_ir", 1), ("fp64", 0))),
+    "c3": dict(desc="64^3 27-point convection-diffusion, LU with static pivoting, FP32 LU + FP64 GMRES vs FP64, "
+                    "8 RHS", gen=("convdiff27",), nx=64, dof=1, facto="PastixFactGETRF", refine="PastixRefineGMRES",
+               nrhs=8, berr=1e-12, paths=(("fp32_ir", 1), ("fp64", 0))),
+    "c4": dict(desc="80^3 Helmholtz A = L7 - (0.5 + 0.1i) I, complex-symmetric LDL^T, complex64 + complex128 "
+                    "BiCGStab vs complex128, 16 RHS", gen=("helmholtz7",), nx=80, dof=1,
+               facto="PastixFactLDLT", refine="PastixRefineBiCGSTAB", nrhs=16, berr=1e-12,
+               paths=(("c64_ir", 1), ("c128", 0))),
+    "c5": dict(desc="60^3 nodes x 3 dof vector Laplacian (lognormal coefficients), FP32 Cholesky + FP64 GMRES vs "
+                    "FP64, 32 RHS", gen=("vlap7",), nx=60, dof=3, facto="PastixFactPOTRF",
+               refine="PastixRefineGMRES", nrhs=32, berr=1e-12, paths=(("fp32_ir", 1), ("fp64", 0))),
+}
+
+
+def make_config_matrix(px, cfg):
+    nx = cfg["nx"]
+    kind = cfg["gen"][0]
+    if kind == "lap7":
+        return px.Spm.laplacian7(nx, nx, nx)
+    if kind == "convdiff27":
+        return px.Spm.convdiff27(nx, nx, nx, 1.0, 0.5, 0)
+    if kind == "helmholtz7":
+        return px.Spm.helmholtz7(nx, nx, nx, 0.5, 0.1)
+    return px.Spm.vlaplacian7(nx, nx, nx, cfg["dof"], 1.0, 0)
+
+
+def run_other_config(px, args):
+    """One JSON line for a configuration other than the headline one."""
+    cfg = CONFIGS[args.config]
+    log("step: generate %s" % cfg["desc"])
+    spm = make_config_matrix(px, cfg)
+    facto = getattr(px, cfg["facto"])
+    refine = getattr(px, cfg["refine"]) if cfg["refine"] else None
+    results = {}
+    for label, mixed in cfg["paths"]:
+        log("step: analyse + upload (%s)" % label)
+        run = Run(px, spm, cfg["nx"], cfg["nrhs"], mixed, cfg["berr"], args.opt, facto=facto, refine=refine,
+                  dof=cfg["dof"])
+        r = measure(run, args.steps, args.warmup)
+        r["thflops"] = run.dparm[px.DPARM_FACT_THFLOPS]
+        r["t_analyze"] = run.t_analyze
+        r["static_pivots"] = int(run.iparm[px.IPARM_STATIC_PIVOTING])
+        del r["x"]
+        results[label] = r
+        log("%s: fact %.3f s solve %.3f s refine %.3f s (%d it) tts %.3f s berr %.2e"
+            % (label, r["fact"], r["solve"], r["refine"], r["iters"], r["tts"], r["berr"]))
+        run.close()
+    head = results[cfg["paths"][0][0]]
+    out = {"metric": "fact+solve s at bwd err<=%g: %s" % (cfg["berr"], cfg["desc"]), "value": head["tts"],
+           "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": head["tts"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "data": "synthetic (%s)" % cfg["gen"][0],
+           "config": {"workload": args.config, "unknowns": int(spm.n), "nrhs": cfg["nrhs"],
+                      "berr_target": cfg["berr"]},
+           "gflops_fact": head["thflops"] / head["fact"] / 1e9}
+    out.update(results)
+    if len(cfg["paths"]) == 2:
+        out["speedup_mixed_vs_full"] = results[cfg["paths"][1][0]]["tts"] / head["tts"]
+    ok = all(r["berr"] <= cfg["berr"] for r in results.values())
+    out["berr_met"] = bool(ok)
+    print(json.dumps(out))
+    return 0 if ok else 1
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -198,6 +275,8 @@ def main():
     if px.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(3)
+    if args.config != "c2":
+        sys.exit(run_other_config(px, args))
     n = args.n
     log("step: generate %d^3 Laplacian" % n)
     spm = px.Spm.laplacian7(n, n, n)

@@ -187,6 +187,18 @@ nblocks( int64_t n, int bs )
     return (unsigned)( ( n + bs - 1 ) / bs );
 }
 
+// Dynamic LDS that a launch of kernel fn may take on a CU with lds_max bytes
+// of LDS: what the kernel's static LDS leaves.
+size_t
+lds_room( const void *fn, size_t lds_max )
+{
+    hipFuncAttributes a;
+    if ( hipFuncGetAttributes( &a, fn ) != hipSuccess || a.sharedSizeBytes >= lds_max ) {
+        return 0;
+    }
+    return lds_max - a.sharedSizeBytes;
+}
+
 // LDS bytes of the scalar TRSM tile: rt rows x n columns
 template <typename T>
 size_t
@@ -597,6 +609,23 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         st_updb  = st_upd + 2 * n_upd;
         st_upds  = st_updb + 2 * n_updb;
     }
+    // rows of the panel's L21 strip and of the inverse's R block kept in
+    // LDS (PXD_NB + 1 values each); the kernels keep the rest in HBM
+    const size_t  lds_row = ( PXD_NB + 1 ) * sizeof( T );
+    const void   *potrf2  = (const void *)k_potrf2<T, true>;
+    if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
+        if ( !ldlt ) {
+            potrf2 = (const void *)k_potrf2<T, false>;
+        }
+    }
+    const int32_t pcap = (int32_t)( lds_room( potrf2, ctx->lds_max ) / lds_row );
+    const int32_t rcap = (int32_t)( lds_room( (const void *)k_trtri<T>, ctx->lds_max ) / lds_row );
+    if ( ctx->kernels && ( rcap < 1 || ( !lu && pcap < 1 ) ) ) {
+        if ( stamps_all ) {
+            (void)hipFree( stamps_all );
+        }
+        return 3;
+    }
     // stamps of the tasks [i0, ...) of a launch (side 1: the U side's copy)
     auto stp = [&]( uint64_t *base, int64_t i0, int side, int64_t nside ) -> uint64_t * {
         return base ? base + 2 * ( i0 + side * nside ) : nullptr;
@@ -615,24 +644,24 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
         unsigned ni = (unsigned)( lv.i1 - lv.i0 );
         // rows below a block column of the widest cblk of the level
-        size_t trtri_lds = (size_t)std::max( 1, lv.maxn - PXD_NB ) * ( PXD_NB + 1 ) * sizeof( T );
+        const size_t trtri_lds = (size_t)std::max( 1, std::min( lv.maxn - PXD_NB, rcap ) ) * lds_row;
         if ( lu ) {
             hipLaunchKernelGGL( ( k_getrf2<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, U, DI, DU, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio, rcap );
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU, stp( st_inv, lv.i0, 1, n_inv ), ctx->prio );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)U, DU, stp( st_inv, lv.i0, 1, n_inv ), ctx->prio, rcap );
                 launches += 2;
             }
         }
         else if ( ctx->kernels ) {
-            const size_t sh = (size_t)std::max( 1, lv.maxn ) * ( PXD_NB + 1 ) * sizeof( T );
+            const size_t sh = (size_t)std::max( 1, std::min( lv.maxn, pcap ) ) * lds_row;
             if ( ldlt ) {
                 hipLaunchKernelGGL( ( k_potrf2<T, true> ), dim3( np ), dim3( PXD_THREADS ), sh, st, ctx->dlist + lv.c0,
-                                    ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
+                                    ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
                 if ( ctx->panel_mw ) {
@@ -647,13 +676,13 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                 else {
                     hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
                                         ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
-                                        stp( st_panel, lv.c0, 0, 0 ), ctx->prio );
+                                        stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
                 }
             }
             launches++;
             if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
-                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio );
+                                    ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio, rcap );
                 launches++;
             }
         }
@@ -854,6 +883,14 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         }
         T            *Y   = (T *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
+        // the widest cblk's right-hand-side block must fit next to each
+        // kernel's static LDS
+        const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( T );
+        if ( shmax > lds_room( (const void *)k_solve_diag_chunk<T>, ctx->lds_max ) ||
+             shmax > lds_room( (const void *)k_solve_fwd_upd<T, 16>, ctx->lds_max ) ||
+             shmax > lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, ctx->lds_max ) ) {
+            return 3;
+        }
         for ( int li = 0; li < nl; li++ ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
@@ -929,43 +966,39 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
     return 0;
 }
 
-// dynamic-LDS limits of the matrix-core kernels (all four arithmetics)
+// dynamic-LDS limits: each kernel may take what its static LDS leaves of
+// the CU's lds bytes
+inline void
+set_lds_room( const void *fn, size_t lds )
+{
+    (void)hipFuncSetAttribute( fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_room( fn, lds ) );
+}
+
+// the matrix-core kernels (all four arithmetics)
 template <typename T>
 void
 set_lds_attributes_mfma( size_t lds )
 {
-    lds -= 16384;   // room for the kernels' static LDS
     if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
-        (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds );
+        set_lds_room( (const void *)k_potrf2<T, false>, lds );
+        set_lds_room( (const void *)k_potrf_mw<T>, lds );
     }
-    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
-        (void)hipFuncSetAttribute( (const void *)k_potrf_mw<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    }
+    set_lds_room( (const void *)k_potrf2<T, true>, lds );
+    set_lds_room( (const void *)k_trtri<T>, lds );
+    set_lds_room( (const void *)k_solve_fwd_upd<T, 16>, lds );
+    set_lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, lds );
+    set_lds_room( (const void *)k_solve_diag_chunk<T>, lds );
 }
 
+// and the scalar kernels (real arithmetics)
 template <typename T>
 void
 set_lds_attributes( size_t lds )
 {
     set_lds_attributes_mfma<T>( lds );
-    lds -= 16384;   // room for the kernels' static LDS
-    (void)hipFuncSetAttribute( (const void *)k_trsm<T, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_trsm<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_diag<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_solve_diag_chunk<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_potrf2<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
-    (void)hipFuncSetAttribute( (const void *)k_trtri<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds );
+    set_lds_room( (const void *)k_trsm<T, 32>, lds );
+    set_lds_room( (const void *)k_trsm<T, 16>, lds );
+    set_lds_room( (const void *)k_solve_diag<T>, lds );
 }
 
 // Solve of nrhs vectors of type TV (double or complex double) with a factor

@@ -97,12 +97,15 @@ search_grow( const search_keys &sk, const dblok *__restrict__ bk, int32_t ffb, i
 // Panel: Cholesky of the diagonal tile with static pivoting, right looking by
 // PXD_NB columns; the unblocked step updates with unscaled columns so that it
 // needs one barrier per column.  The inverses of the PXD_NB x PXD_NB diagonal
-// sub-blocks go to the diagonal blocks of Linv.
+// sub-blocks go to the diagonal blocks of Linv.  The first pcap rows of the
+// current block's L21 are kept in dynamic LDS; rows beyond pcap (wide cblks in
+// complex double, whose strip would not fit the CU's LDS) are read back from
+// the copy just written to A.
 template <typename T, bool LDLT>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__restrict__ L,
           T *__restrict__ Linv, double thresh, int *__restrict__ npiv,
-          uint64_t *__restrict__ stamps, int prio )
+          uint64_t *__restrict__ stamps, int prio, int32_t pcap )
 {
     __shared__ T sA[PXD_NB][PXD_NB + 1];   // sA[col][row]
     __shared__ T sI[PXD_NB][PXD_NB + 1];   // inverse, sI[col][row]
@@ -253,8 +256,10 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                     // LDL^T: the product is L21 D; L21 = product / D goes below
                     // the block and the product itself, transposed, to the
                     // block's rows right of it (D L^T, core_zsytrfsp.c:176-186)
-                    sP[rr][c0] = M::get( acc0, v );   // zero beyond bs: sI is zero there
-                    sP[rr][c1] = M::get( acc1, v );
+                    if ( rr < pcap ) {
+                        sP[rr][c0] = M::get( acc0, v );   // zero beyond bs: sI is zero there
+                        sP[rr][c1] = M::get( acc1, v );
+                    }
                     if ( c0 < bs ) A[( m0 + rr ) + (int64_t)( kb + c0 ) * n] = LDLT ? M::get( acc0, v ) * sDi[c0] : M::get( acc0, v );
                     if ( c1 < bs ) A[( m0 + rr ) + (int64_t)( kb + c1 ) * n] = LDLT ? M::get( acc1, v ) * sDi[c1] : M::get( acc1, v );
                     if constexpr ( LDLT ) {
@@ -268,7 +273,18 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 
         // trailing update A22 -= L21 L21^T (lower part) on the matrix cores
         // from sP: 32 x 32 tiles, wave w takes every fourth tile, two at a
-        // time, with the tiles' loads of A22 issued before their products
+        // time, with the tiles' loads of A22 issued before their products.
+        // Row ri of the product (L21, times D for LDL^T) beyond pcap comes
+        // from A: the D L^T strip for LDL^T, the L21 column block otherwise.
+        auto prow = [&]( int32_t ri, int l ) -> T {
+            if ( ri < pcap ) {
+                return sP[ri][l];
+            }
+            if ( l >= bs ) {
+                return (T)0;
+            }
+            return LDLT ? A[( kb + l ) + (int64_t)( m0 + ri ) * n] : A[( m0 + ri ) + (int64_t)( kb + l ) * n];
+        };
         {
             const int32_t nt = ( m + 31 ) / 32;
             const int32_t ntile = nt * ( nt + 1 ) / 2;
@@ -316,8 +332,8 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
                             const int32_t ri = 32 * ti[g] + 16 * a + ( lane & 15 );
                             const int32_t rj = 32 * tj[g] + 16 * a + ( lane & 15 );
                             // LDL^T: A22 -= (L21 D) D^{-1} (L21 D)^T
-                            af[a] = ( ri < m ) ? ( LDLT ? sP[ri][l] * sDi[l] : sP[ri][l] ) : (T)0;
-                            bf[a] = ( rj < m ) ? sP[rj][l] : (T)0;
+                            af[a] = ( ri < m ) ? ( LDLT ? prow( ri, l ) * sDi[l] : prow( ri, l ) ) : (T)0;
+                            bf[a] = ( rj < m ) ? prow( rj, l ) : (T)0;
                         }
 #pragma unroll
                         for ( int a = 0; a < 2; a++ )
@@ -877,12 +893,14 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 // Inverse of the lower-triangular L_kk by block columns, one workgroup per
 // (cblk, block column J).  The diagonal blocks D_K = L[K,K]^{-1} were written
 // to Linv by the panel.  Column sweep: R = -L[J+1:, J] D_J, then for K = J+1,
-// J+2, ...: X[K,J] = D_K R[K], R[K+1:] -= L[K+1:, K] X[K,J].  R stays in LDS;
-// both products run on the matrix cores.
+// J+2, ...: X[K,J] = D_K R[K], R[K+1:] -= L[K+1:, K] X[K,J].  The first rcap
+// rows of R stay in LDS, the rest (wide cblks in complex double) in place in
+// Linv, where X[K,J] later overwrites them; both products run on the matrix
+// cores.
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ L,
-         T *__restrict__ Linv, uint64_t *__restrict__ stamps, int prio )
+         T *__restrict__ Linv, uint64_t *__restrict__ stamps, int prio, int32_t rcap )
 {
     typedef mfma16<T>         M;
     typedef typename M::acc_t acc_t;
@@ -904,6 +922,19 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
     const int32_t  r0   = J + PXD_NB;          // first row of R
     const int32_t  nr   = n - r0;
     const int32_t  bj   = min( PXD_NB, n - J );
+    // R(row, col): rows below rcap in LDS, the others at their place in Linv
+    // (nr > 0 implies bj == PXD_NB, so every column is inside the tile)
+    auto rget = [&]( int32_t rr, int cc ) -> T {
+        return ( rr < rcap ) ? sR[rr][cc] : LI[( r0 + rr ) + (int64_t)( J + cc ) * n];
+    };
+    auto rset = [&]( int32_t rr, int cc, T v ) {
+        if ( rr < rcap ) {
+            sR[rr][cc] = v;
+        }
+        else {
+            LI[( r0 + rr ) + (int64_t)( J + cc ) * n] = v;
+        }
+    };
 
     // load a 32 x 32 block of Linv (rows R0, cols C0) into dst[row][col]
     auto load_block = [&]( T ( *dst )[PXD_NB + 1], int32_t R0, int32_t C0, int32_t nrow, int32_t ncol ) {
@@ -934,8 +965,8 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 #pragma unroll
             for ( int v = 0; v < 4; v++ ) {
                 const int32_t rr = rb + M::row( lane, v );
-                M::set( acc0, v, ( rr < nr ) ? sR[rr][lane & 15] : (T)0 );
-                M::set( acc1, v, ( rr < nr ) ? sR[rr][16 + ( lane & 15 )] : (T)0 );
+                M::set( acc0, v, ( rr < nr ) ? rget( rr, lane & 15 ) : (T)0 );
+                M::set( acc1, v, ( rr < nr ) ? rget( rr, 16 + ( lane & 15 ) ) : (T)0 );
             }
 #pragma unroll
             for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
@@ -947,15 +978,18 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
             for ( int v = 0; v < 4; v++ ) {
                 const int32_t rr = rb + M::row( lane, v );
                 if ( rr < nr ) {
-                    sR[rr][lane & 15]        = M::get( acc0, v );
-                    sR[rr][16 + ( lane & 15 )] = M::get( acc1, v );
+                    rset( rr, lane & 15, M::get( acc0, v ) );
+                    rset( rr, 16 + ( lane & 15 ), M::get( acc1, v ) );
                 }
             }
         }
     };
 
-    for ( int idx = tid; idx < nr * ( PXD_NB + 1 ); idx += PXD_THREADS ) {
+    for ( int idx = tid; idx < min( nr, rcap ) * ( PXD_NB + 1 ); idx += PXD_THREADS ) {
         ( &sR[0][0] )[idx] = (T)0;
+    }
+    for ( int idx = tid; idx < ( nr - rcap ) * PXD_NB; idx += PXD_THREADS ) {
+        rset( rcap + idx % ( nr - rcap ), idx / ( nr - rcap ), (T)0 );
     }
     load_block( sX, J, J, bj, bj );   // D_J
     __syncthreads();
@@ -973,7 +1007,7 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
             for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
                 const int l = 4 * s4 + ( lane >> 4 );
                 const T   a = sD[16 * bi + ( lane & 15 )][l];
-                const T   b = ( K - r0 + l < nr ) ? sR[K - r0 + l][16 * bc + ( lane & 15 )] : (T)0;
+                const T   b = ( K - r0 + l < nr ) ? rget( K - r0 + l, 16 * bc + ( lane & 15 ) ) : (T)0;
                 acc = M::mma( a, b, acc );
             }
             __syncthreads();   // every wave has read sX (previous step) and sR[K block]

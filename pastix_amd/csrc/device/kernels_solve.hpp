@@ -327,6 +327,13 @@ k_nrm2sq( int64_t n, const T *__restrict__ x, double *__restrict__ res )
     }
 }
 
+// max that keeps a NaN of either operand (fmax drops it)
+__device__ inline double
+max_nan( double a, double b )
+{
+    return ( a > b || a != a ) ? a : b;
+}
+
 template <typename T>
 __global__ void __launch_bounds__( 256 )
 k_maxabs( int64_t n, const T *__restrict__ x, unsigned long long *__restrict__ res )
@@ -337,16 +344,17 @@ k_maxabs( int64_t n, const T *__restrict__ x, unsigned long long *__restrict__ r
     double    s  = 0.;
     for ( int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
           i += (int64_t)gridDim.x * blockDim.x ) {
-        s = fmax( s, (double)px_abs( xx[i] ) );
+        s = max_nan( s, (double)px_abs( xx[i] ) );
     }
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
 #pragma unroll
-    for ( int off = 32; off > 0; off >>= 1 ) s = fmax( s, __shfl_xor( s, off, 64 ) );
+    for ( int off = 32; off > 0; off >>= 1 ) s = max_nan( s, __shfl_xor( s, off, 64 ) );
     if ( lane == 0 ) sh[wave] = s;
     __syncthreads();
     if ( threadIdx.x == 0 ) {
-        double m = fmax( fmax( sh[0], sh[1] ), fmax( sh[2], sh[3] ) );
-        // non-negative doubles order like their bit patterns
+        double m = max_nan( max_nan( sh[0], sh[1] ), max_nan( sh[2], sh[3] ) );
+        // non-negative doubles order like their bit patterns, and a positive
+        // NaN orders above +inf, so a NaN entry makes the norm NaN
         atomicMax( res + q, (unsigned long long)__double_as_longlong( m ) );
     }
 }

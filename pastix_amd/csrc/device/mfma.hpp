@@ -117,12 +117,14 @@ template <> struct bufio<float> {
         __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned, v ), r, off, 0, 0 );
     }
 };
+// The 64- and 128-bit loads are bit-cast whole: taking 32-bit elements of the
+// builtin's vector result compiles (ROCm 7.2 clang, gfx950) to a single
+// dword load whose value is copied into every element.
 template <> struct bufio<pxcf> {
     typedef unsigned u2 __attribute__( ( ext_vector_type( 2 ) ) );
     __device__ static inline pxcf load( __amdgpu_buffer_rsrc_t r, uint32_t off )
     {
-        u2 v = __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 );
-        return pxcf( __builtin_bit_cast( float, v.x ), __builtin_bit_cast( float, v.y ) );
+        return __builtin_bit_cast( pxcf, __builtin_amdgcn_raw_buffer_load_b64( r, off, 0, 0 ) );
     }
     __device__ static inline void store( pxcf v, __amdgpu_buffer_rsrc_t r, uint32_t off )
     {
@@ -137,9 +139,7 @@ template <> struct bufio<pxcd> {
     typedef unsigned u2 __attribute__( ( ext_vector_type( 2 ) ) );
     __device__ static inline pxcd load( __amdgpu_buffer_rsrc_t r, uint32_t off )
     {
-        u4 v = __builtin_amdgcn_raw_buffer_load_b128( r, off, 0, 0 );
-        u2 a = { v.x, v.y }, b = { v.z, v.w };
-        return pxcd( __builtin_bit_cast( double, a ), __builtin_bit_cast( double, b ) );
+        return __builtin_bit_cast( pxcd, __builtin_amdgcn_raw_buffer_load_b128( r, off, 0, 0 ) );
     }
     __device__ static inline void store( pxcd v, __amdgpu_buffer_rsrc_t r, uint32_t off )
     {

@@ -330,6 +330,13 @@ spmGenRHS( spm_rhstype_t type, spm_int_t nrhs, const spmatrix_t *spm,
     return spmGenRHSSeed( type, nrhs, spm, 0, x, ldx, b, ldb );
 }
 
+/* max that keeps a NaN of either operand (fmax drops it) */
+static inline double
+max_nan( double a, double b )
+{
+    return ( a > b || a != a ) ? a : b;
+}
+
 double
 spmBackwardError( spm_int_t nrhs, const spmatrix_t *spm, const void *b, spm_int_t ldb,
                   const void *x, spm_int_t ldx, double *berr_each )
@@ -355,21 +362,21 @@ spmBackwardError( spm_int_t nrhs, const spmatrix_t *spm, const void *b, spm_int_
         double nr = 0., nx = 0., nb = 0.;
         for ( spm_int_t i = 0; i < n; i++ ) {
             if ( cplx ) {
-                nr = fmax( nr, cabs( ( (double complex *)r )[i + c * n] ) );
-                nx = fmax( nx, cabs( ( (const double complex *)x )[i + c * ldx] ) );
-                nb = fmax( nb, cabs( ( (const double complex *)b )[i + c * ldb] ) );
+                nr = max_nan( nr, cabs( ( (double complex *)r )[i + c * n] ) );
+                nx = max_nan( nx, cabs( ( (const double complex *)x )[i + c * ldx] ) );
+                nb = max_nan( nb, cabs( ( (const double complex *)b )[i + c * ldb] ) );
             }
             else {
-                nr = fmax( nr, fabs( ( (double *)r )[i + c * n] ) );
-                nx = fmax( nx, fabs( ( (const double *)x )[i + c * ldx] ) );
-                nb = fmax( nb, fabs( ( (const double *)b )[i + c * ldb] ) );
+                nr = max_nan( nr, fabs( ( (double *)r )[i + c * n] ) );
+                nx = max_nan( nx, fabs( ( (const double *)x )[i + c * ldx] ) );
+                nb = max_nan( nb, fabs( ( (const double *)b )[i + c * ldb] ) );
             }
         }
         double be = nr / ( normA * nx + nb );
         if ( berr_each ) {
             berr_each[c] = be;
         }
-        worst = fmax( worst, be );
+        worst = max_nan( worst, be );
     }
     free( r );
     return worst;

@@ -99,6 +99,20 @@ def test_oracle_spmv_is_exact_on_the_integer_laplacian():
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("complex_", [False, True])
+def test_backward_error_reports_nan(complex_):
+    """A NaN anywhere in x makes its column's backward error NaN (a max that
+    drops NaN would report the column as solved)."""
+    spm = px.Spm.laplacian7(4, 4, 4, complex_=complex_)
+    x0, b = spm.gen_rhs(2, seed=0)
+    worst, each = spm.backward_error(b, x0)
+    assert worst <= 1e-15
+    x = np.array(x0, copy=True)
+    x[5, 1] = np.nan
+    worst, each = spm.backward_error(b, x)
+    assert np.isnan(worst) and each[0] <= 1e-15 and np.isnan(each[1])
+
+
 @pytest.mark.parametrize("nrhs", [1, 3])
 def test_oracle_solve_backward_error(nrhs):
     spm, pd, iparm, dparm, s, bc = _setup(12)
