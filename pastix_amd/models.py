@@ -16,6 +16,13 @@ Mirrors ``common/models.[ch]`` of the reference (``modelsGetCost{1,2,3}Param``,
 ``python -m pastix_amd.models --nx 100`` factorizes the 7-point Laplacian with
 the timing log on, fits both models on a random 80 % of the records and
 prints the error of each on the other 20 % (needs the GPU).
+
+``python -m pastix_amd.models --transfer`` logs the factorizations of the
+20^3 and 100^3 Laplacians and of the 64^3 convection-diffusion LU, fits both
+models on those records, and scores them, next to the reference's built-in
+defaults (the fallback of ``common/models.c`` when no model file is given),
+on the log of the 60^3 x 3-dof vector Laplacian, a problem none of the
+training runs contains.
 """
 import argparse
 import ctypes
@@ -301,15 +308,102 @@ def compare(records, arith, steps=2000, lr=1e-2, seed=7, frac=0.8):
     return out, poly, learned
 
 
+# Problems of the transfer experiment: (label, generator, grid edge, dof,
+# factorization, factor precisions logged)
+TRAIN_RUNS = [("20^3 Laplacian POTRF", "lap7", 20, 1, "POTRF", (1,)),
+              ("100^3 Laplacian POTRF", "lap7", 100, 1, "POTRF", (0, 1)),
+              ("64^3 convection-diffusion GETRF", "convdiff27", 64, 1, "GETRF", (0, 1))]
+TEST_RUN = ("60^3 x 3 dof vector Laplacian POTRF", "vlap7", 60, 3, "POTRF", (0, 1))
+
+
+def _logged_factorization(px, kind, nx, dof, facto, mixed):
+    """Timing-log records of one factorization (model_log = 1)."""
+    if kind == "lap7":
+        spm = px.Spm.laplacian7(nx, nx, nx, 6.0, 1.0)
+    elif kind == "convdiff27":
+        spm = px.Spm.convdiff27(nx, nx, nx, 1.0, 0.5, 0)
+    else:
+        spm = px.Spm.vlaplacian7(nx, nx, nx, dof, 1.0, 0)
+    iparm, dparm = px.initParam()
+    iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
+    iparm[px.IPARM_FACTORIZATION] = px.PastixFactGETRF if facto == "GETRF" else px.PastixFactPOTRF
+    iparm[px.IPARM_MIXED] = mixed
+    iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
+    pd = px.init(iparm, dparm)
+    px.analyze_grid(pd, spm, nx, nx, nx, dof=dof)
+    px.set_option(pd, "model_log", 1)
+    px.task_numfact(pd, spm)
+    rec = timing_log(pd)
+    px.finalize(pd)
+    return rec[rec["time"] > 0]
+
+
+def _families(rec):
+    fams = dict(panel=rec["ktype"] <= KTYPE["SYTRF"],
+                trsm=(rec["ktype"] >= KTYPE["TRSMCblk1d"]) & (rec["ktype"] < KTYPE["GEMMCblk1d1d"]),
+                update=rec["ktype"] >= KTYPE["GEMMCblk1d1d"])
+    fams["all"] = np.ones(len(rec), dtype=bool)
+    return fams
+
+
+def transfer(steps=3000, lr=1e-2, seed=7, log=print):
+    """Fit on the training problems' logs, score on the test problem's log;
+    per factor precision (arith 0: FP32, 1: FP64) and kernel family."""
+    import pastix_amd as px
+    logs = {0: [], 1: []}
+    runs = []
+    for label, kind, nx, dof, facto, ariths in TRAIN_RUNS:
+        for arith in ariths:
+            rec = _logged_factorization(px, kind, nx, dof, facto, mixed=int(arith == 0))
+            logs[arith].append(rec)
+            runs.append(dict(problem=label, arith=arith, records=int(len(rec)),
+                             logged_seconds=float(rec["time"].sum())))
+            log("logged %s arith %d: %d records" % (label, arith, len(rec)))
+    label, kind, nx, dof, facto, ariths = TEST_RUN
+    out = dict(train=runs, test_problem=label, results={})
+    cpu, gpu = Model.default_cpu(), Model.default_gpu()
+    for arith in ariths:
+        test = _logged_factorization(px, kind, nx, dof, facto, mixed=int(arith == 0))
+        log("logged %s arith %d: %d records" % (label, arith, len(test)))
+        train = np.concatenate(logs[arith])
+        poly = Model().fit(train, arith, name="fit on the training problems")
+        learned = LearnedModel(train, arith, steps=steps, lr=lr, seed=seed)
+        preds = dict(polynomial=poly.costs(arith, test), learned=learned.predict(test),
+                     default_gpu=gpu.costs(arith, test), default_cpu=cpu.costs(arith, test))
+        res = dict(train_records=int(len(train)), test_records=int(len(test)),
+                   test_logged_seconds=float(test["time"].sum()), mlp_train_loss=learned.loss)
+        for fam, sel in _families(test).items():
+            if not sel.any():
+                continue
+            res[fam] = {}
+            for name, p in preds.items():
+                # a model without coefficients for a kernel returns -1 there
+                has = sel & ( p >= 0 )
+                if has.any():
+                    res[fam][name] = error_summary(p[has], test["time"][has])
+                    res[fam][name]["coverage"] = float(has.sum() / sel.sum())
+                else:
+                    res[fam][name] = dict(coverage=0.0)
+        out["results"]["fp32" if arith == 0 else "fp64"] = res
+    return out
+
+
 def main(argv=None):
     import pastix_amd as px
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--transfer", action="store_true",
+                    help="fit on the 20^3 / 100^3 Laplacian and 64^3 LU logs, test on the 60^3 x 3 dof log")
     ap.add_argument("--nx", type=int, default=100)
     ap.add_argument("--mixed", type=int, default=1, help="1: FP32 factor, 0: FP64")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--csv", default=None, help="write the timing log as model.csv here")
     ap.add_argument("--model-out", default=None, help="write the fitted polynomial model here")
     a = ap.parse_args(argv)
+    if a.transfer:
+        import sys
+        out = transfer(steps=a.steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(out, indent=1))
+        return
     spm = px.Spm.laplacian7(a.nx, a.nx, a.nx, 6.0, 1.0)
     iparm, dparm = px.initParam()
     iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
