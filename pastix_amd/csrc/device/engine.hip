@@ -23,6 +23,7 @@
 #include <vector>
 #include "kernels_solve.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_sfsolve.hpp"
 #include "pastix/kernels_enums.h"
 
 namespace {
@@ -132,6 +133,19 @@ struct pxd_ctx_s {
     std::vector<inv_task>  hinvt;
     std::vector<upd_group> groups;
 
+    // one-launch solve sweeps (k_sf_forward / k_sf_backward): task lists,
+    // the cblks each tile's rows face, the tiles facing each cblk, and the
+    // dependency counters (forward cnt, rdy; backward cnt, rdy), which grow
+    // by one pass's worth per pass (sf_epoch)
+    int       solve_sf = 1;
+    int32_t   sf_from_level = -1;   // option: first level of the one-launch part (-1: automatic)
+    int32_t   sf_lc = 0;            // the cut in use
+    sf_task  *dsf_fw = nullptr, *dsf_bw = nullptr;
+    int32_t  *dsf_facing = nullptr, *dsf_need = nullptr;
+    uint32_t *dsf_cnt = nullptr;
+    int64_t   nsf_fw = 0, nsf_bw = 0;
+    uint32_t  sf_epoch = 0;
+
     // options / stats
     int          timing = 0;
     // 1: panels, inverses, TRSM and the updates into the next level raise
@@ -224,6 +238,134 @@ compute_levels( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
         level[k]  = l;
         ctx->nlev = std::max( ctx->nlev, l + 1 );
     }
+}
+
+// Task lists of the one-launch solve sweeps, for the cblks of dependency
+// level sf_lc and above (the levels below run level by level).  The cut is
+// the lowest level from which every level has at most PXD_SF_AUTO_TASKS
+// tasks (the latency-bound top of the tree), unless the option sets it.
+// A cblk with one diagonal chunk and at most one off-diagonal tile is one
+// SF_LEAF task; others are SF_DIAG chunks and SF_TILE tiles.  Forward: level
+// by level from sf_lc up, the chunks (and leaves) of a level before its
+// tiles; backward: from the top level down, a level's tiles (and leaves)
+// before its chunks.  So workgroups are dispatched as the level launches
+// would run, without the barriers.  A tile lists the distinct cblks its
+// rows face (the bloks of a cblk face rows in order, so equal faces are
+// adjacent); need[f] counts the tiles of the list that update f.
+#define PXD_SF_AUTO_TASKS 1024
+int
+build_sf_schedule( pxd_ctx_t *ctx )
+{
+    const int64_t nc   = ctx->cblknbr;
+    const int32_t nlev = ctx->nlev;
+    // per cblk: chunk count, tiles (r0, facing range) -- facing built once
+    std::vector<int32_t> facing, need( std::max<int64_t>( 1, nc ), 0 );
+    std::vector<int32_t> tile0( nc + 1, 0 );
+    std::vector<sf_task> tiles;
+    for ( int64_t k = 0; k < nc; k++ ) {
+        const dcblk  &c  = ctx->hcb[k];
+        const int32_t n  = c.lcolnum - c.fcolnum + 1;
+        const int32_t lb = ctx->hcb[k + 1].fblok;
+        int32_t       b  = c.fblok + 1;
+        tile0[k]         = (int32_t)tiles.size();
+        for ( int32_t r0 = n; r0 < c.stride; r0 += PXD_SF_TILE ) {
+            const int32_t r1 = std::min( r0 + PXD_SF_TILE, c.stride );
+            while ( b < lb && ctx->hbk[b].rowoff + ( ctx->hbk[b].lrownum - ctx->hbk[b].frownum + 1 ) <= r0 ) {
+                b++;
+            }
+            const int32_t f0 = (int32_t)facing.size();
+            for ( int32_t bb = b; bb < lb && ctx->hbk[bb].rowoff < r1; bb++ ) {
+                const int32_t f = ctx->hbk[bb].fcblknm;
+                if ( facing.size() == (size_t)f0 || facing.back() != f ) {
+                    facing.push_back( f );
+                }
+            }
+            tiles.push_back( sf_task{ SF_TILE, (int32_t)k, r0, f0, (int32_t)facing.size(), 0 } );
+        }
+    }
+    tile0[nc] = (int32_t)tiles.size();
+    auto nchunk = [&]( int64_t k ) {
+        return ( ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1 + PXD_SF_ROWS - 1 ) / PXD_SF_ROWS;
+    };
+    auto leaf = [&]( int64_t k ) { return nchunk( k ) == 1 && tile0[k + 1] - tile0[k] <= 1; };
+    std::vector<std::vector<int32_t>> bylev( std::max( 1, nlev ) );
+    std::vector<int64_t>              ntask( std::max( 1, nlev ), 0 );
+    for ( int64_t k = 0; k < nc; k++ ) {
+        bylev[ctx->level[k]].push_back( (int32_t)k );
+        ntask[ctx->level[k]] += leaf( k ) ? 1 : nchunk( k ) + ( tile0[k + 1] - tile0[k] );
+    }
+    int32_t lc = ctx->sf_from_level;
+    if ( lc < 0 ) {
+        lc = nlev;
+        while ( lc > 0 && ntask[lc - 1] <= PXD_SF_AUTO_TASKS ) {
+            lc--;
+        }
+    }
+    lc         = std::min( std::max( lc, 0 ), nlev );
+    ctx->sf_lc = lc;
+
+    std::vector<sf_task> fw, bw;
+    for ( int32_t L = lc; L < nlev; L++ ) {
+        for ( int32_t k : bylev[L] ) {
+            if ( leaf( k ) ) {
+                sf_task t{ SF_LEAF, k, 0, 0, 0, 0 };
+                if ( tile0[k + 1] > tile0[k] ) {
+                    t.f0 = tiles[tile0[k]].f0;
+                    t.f1 = tiles[tile0[k]].f1;
+                }
+                fw.push_back( t );
+            }
+            else {
+                for ( int32_t R0 = 0; R0 < ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1; R0 += PXD_SF_ROWS ) {
+                    fw.push_back( sf_task{ SF_DIAG, k, R0, 0, 0, 0 } );
+                }
+            }
+        }
+        for ( int32_t k : bylev[L] ) {
+            if ( !leaf( k ) ) {
+                fw.insert( fw.end(), tiles.begin() + tile0[k], tiles.begin() + tile0[k + 1] );
+            }
+            for ( int32_t t = tile0[k]; t < tile0[k + 1]; t++ ) {
+                for ( int32_t i = tiles[t].f0; i < tiles[t].f1; i++ ) need[facing[i]]++;
+            }
+        }
+    }
+    for ( int32_t L = nlev - 1; L >= lc; L-- ) {
+        for ( int32_t k : bylev[L] ) {
+            if ( leaf( k ) ) {
+                sf_task t{ SF_LEAF, k, 0, 0, 0, 0 };
+                if ( tile0[k + 1] > tile0[k] ) {
+                    t.f0 = tiles[tile0[k]].f0;
+                    t.f1 = tiles[tile0[k]].f1;
+                }
+                bw.push_back( t );
+            }
+            else {
+                bw.insert( bw.end(), tiles.begin() + tile0[k], tiles.begin() + tile0[k + 1] );
+            }
+        }
+        for ( int32_t k : bylev[L] ) {
+            if ( !leaf( k ) ) {
+                for ( int32_t R0 = 0; R0 < ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1; R0 += PXD_SF_ROWS ) {
+                    bw.push_back( sf_task{ SF_DIAG, k, R0, 0, 0, 0 } );
+                }
+            }
+        }
+    }
+    if ( upload( &ctx->dsf_fw, fw ) || upload( &ctx->dsf_bw, bw ) || upload( &ctx->dsf_facing, facing ) ||
+         upload( &ctx->dsf_need, need ) ) {
+        return 6;
+    }
+    ctx->nsf_fw = (int64_t)fw.size();
+    ctx->nsf_bw = (int64_t)bw.size();
+    if ( ctx->dsf_cnt ) {
+        PXD_CHECK( hipFree( ctx->dsf_cnt ) );
+    }
+    PXD_CHECK( hipMalloc( (void **)&ctx->dsf_cnt, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
+    PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
+    PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
+    ctx->sf_epoch = 0;
+    return 0;
 }
 
 // Build the task lists of the factorization and the solve, level by level.
@@ -424,7 +566,7 @@ build_schedule( pxd_ctx_t *ctx )
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
         return 6;
     }
-    return 0;
+    return build_sf_schedule( ctx );
 }
 
 // Elapsed milliseconds between two events (the stream must be synchronized).
@@ -855,6 +997,87 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
 }
 
 // ---------------------------------------------------------------------------
+// Right-hand sides per pass of the one-launch sweeps (the accumulators and
+// LDS of a tile grow with it).
+template <typename T> struct sf_nr_max { static constexpr int value = 16; };
+template <> struct sf_nr_max<pxcd> { static constexpr int value = 8; };
+
+// The one-launch part of a sweep (the levels from sf_lc up) for nrhs <= NR
+// columns of X (factor precision, leading dimension ldx), Y the second
+// vector (ld n).  The forward launch opens a pass (epoch).
+template <typename T, int NR>
+int
+sf_launch( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
+{
+    constexpr int B  = 128 / (int)sizeof( T );   // columns per load batch of a forward tile
+    const bool    lu = ( ctx->facto == FACT_GETRF );
+    const T      *L  = (const T *)ctx->dL;
+    const T      *DI = (const T *)ctx->ddinv;
+    const T      *LB = lu ? (const T *)ctx->dU : L;
+    const T      *DB = lu ? (const T *)ctx->dudinv : DI;
+    const size_t  sh = sf_lds_bytes<T, NR>( ctx->colmax );
+    if ( ctx->colmax > PXD_TRSM_NT * 16 || sh > lds_room( (const void *)k_sf_forward<T, NR, B>, ctx->lds_max ) ||
+         sh > lds_room( (const void *)k_sf_backward<T, NR>, ctx->lds_max ) ) {
+        return 3;
+    }
+    if ( forward ) {
+        ctx->sf_epoch++;
+    }
+    if ( ( forward ? ctx->nsf_fw : ctx->nsf_bw ) == 0 ) {
+        return 0;
+    }
+    const int64_t nc  = ctx->cblknbr;
+    uint32_t     *cnt = ctx->dsf_cnt;
+    int          *err = ctx->dscratch + 8;
+    if ( forward ) {
+        hipLaunchKernelGGL( ( k_sf_forward<T, NR, B> ), dim3( (unsigned)ctx->nsf_fw ), dim3( PXD_THREADS ), sh,
+                            ctx->stream, ctx->dsf_fw, ctx->dsf_facing, ctx->dsf_need, ctx->dcb, ctx->dbk, L, DI, X,
+                            ldx, Y, ctx->n, nrhs, ctx->sf_epoch, cnt, cnt + nc, err );
+    }
+    else {
+        hipLaunchKernelGGL( ( k_sf_backward<T, NR> ), dim3( (unsigned)ctx->nsf_bw ), dim3( PXD_THREADS ), sh,
+                            ctx->stream, ctx->dsf_bw, ctx->dsf_facing, ctx->dcb, ctx->dbk, LB, DB, X, ldx, Y, ctx->n,
+                            nrhs, ctx->sf_epoch, cnt + 2 * nc, cnt + 3 * nc, err );
+    }
+    PXD_CHECK( hipGetLastError() );
+    return 0;
+}
+
+template <typename T>
+int
+sf_sweep( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
+{
+    return ( nrhs <= 4 ) ? sf_launch<T, 4>( ctx, forward, X, ldx, nrhs, Y )
+                         : sf_launch<T, sf_nr_max<T>::value>( ctx, forward, X, ldx, nrhs, Y );
+}
+
+// Right-hand sides per solve pass: the level kernels take PXD_MAXRHS, the
+// one-launch kernels sf_nr_max when they cover every level.
+template <typename T>
+int
+solve_pass_rhs( const pxd_ctx_t *ctx )
+{
+    return ( ctx->kernels && ctx->solve_sf && ctx->sf_lc == 0 ) ? sf_nr_max<T>::value : PXD_MAXRHS;
+}
+
+// After the passes of a solve: a timed-out wait leaves the counters
+// inconsistent, so they are cleared and the solve reports the error.
+int
+sf_check( pxd_ctx_t *ctx )
+{
+    int e = 0;
+    PXD_CHECK( hipMemcpyAsync( &e, ctx->dscratch + 8, sizeof( int ), hipMemcpyDeviceToHost, ctx->stream ) );
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    if ( e != 0 ) {
+        PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, ctx->cblknbr ) ) );
+        PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
+        ctx->sf_epoch = 0;
+        fprintf( stderr, "pastix_amd device: a solve task timed out waiting for its dependencies\n" );
+        return 6;
+    }
+    return 0;
+}
+
 template <typename T>
 int
 solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
@@ -875,7 +1098,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         //             update reads x_k from Y and subtracts into X;
         // backward -- the gather reads the solved rows from X and subtracts
         //             into Y; the diagonal reads Y and writes X.
-        size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( T );
+        size_t need = (size_t)ctx->n * std::max( PXD_MAXRHS, sf_nr_max<T>::value ) * sizeof( T );
         if ( ctx->dsolvy_bytes < need ) {
             if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
             PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
@@ -883,15 +1106,17 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         }
         T            *Y   = (T *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
+        // levels [0, nlv) run level by level, the others in one launch per sweep
+        const int     nlv = ctx->solve_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
         const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( T );
-        if ( shmax > lds_room( (const void *)k_solve_diag_chunk<T>, ctx->lds_max ) ||
-             shmax > lds_room( (const void *)k_solve_fwd_upd<T, 16>, ctx->lds_max ) ||
-             shmax > lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, ctx->lds_max ) ) {
+        if ( nlv > 0 && ( nrhs > PXD_MAXRHS || shmax > lds_room( (const void *)k_solve_diag_chunk<T>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, 16>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, ctx->lds_max ) ) ) {
             return 3;
         }
-        for ( int li = 0; li < nl; li++ ) {
+        for ( int li = 0; li < nlv; li++ ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
@@ -907,11 +1132,18 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
                                     s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
             }
         }
+        int rc = 0;
+        if ( ctx->solve_sf && ( rc = sf_sweep<T>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
+            return rc;
+        }
         if ( ctx->facto == FACT_SYTRF ) {
             hipLaunchKernelGGL( ( k_diag_scale<T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
                                 ctx->dcol2cblk, ctx->dcb, L, Y, ldy, nrhs );
         }
-        for ( int li = nl - 1; li >= 0; li-- ) {
+        if ( ctx->solve_sf && ( rc = sf_sweep<T>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
+            return rc;
+        }
+        for ( int li = nlv - 1; li >= 0; li-- ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
@@ -988,6 +1220,10 @@ set_lds_attributes_mfma( size_t lds )
     set_lds_room( (const void *)k_solve_fwd_upd<T, 16>, lds );
     set_lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, lds );
     set_lds_room( (const void *)k_solve_diag_chunk<T>, lds );
+    set_lds_room( (const void *)k_sf_forward<T, 4, 128 / sizeof( T )>, lds );
+    set_lds_room( (const void *)k_sf_backward<T, 4>, lds );
+    set_lds_room( (const void *)k_sf_forward<T, sf_nr_max<T>::value, 128 / sizeof( T )>, lds );
+    set_lds_room( (const void *)k_sf_backward<T, sf_nr_max<T>::value>, lds );
 }
 
 // and the scalar kernels (real arithmetics)
@@ -1008,23 +1244,24 @@ template <typename TV, typename T>
 int
 solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
 {
-    int rc = 0;
+    int       rc   = 0;
+    const int pass = solve_pass_rhs<T>( ctx );
     if constexpr ( std::is_same<TV, T>::value ) {
-        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
-            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
+            int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
             rc = solve_sweeps<T>( ctx, (T *)dx + q0 * ldx, ldx, nq );
         }
     }
     else {
-        size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, PXD_MAXRHS ) * sizeof( T );
+        size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, pass ) * sizeof( T );
         if ( ctx->dvecs_bytes < need ) {
             if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
             PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
             ctx->dvecs_bytes = need;
         }
         T *xs = (T *)ctx->dvecs;
-        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += PXD_MAXRHS ) {
-            int nq = (int)std::min<int64_t>( PXD_MAXRHS, nrhs - q0 );
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
+            int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
@@ -1100,7 +1337,8 @@ pxd_destroy( pxd_ctx_t *ctx )
     (void)hipStreamSynchronize( ctx->stream );
     void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
-                     ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred };
+                     ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred,
+                     ctx->dsf_fw, ctx->dsf_bw, ctx->dsf_facing, ctx->dsf_need, ctx->dsf_cnt };
     for ( void *p : ptrs ) {
         if ( p ) {
             (void)hipFree( p );
@@ -1153,6 +1391,16 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "solve_sf" ) {
+        ctx->solve_sf = ( value != 0. );
+        return 0;
+    }
+    if ( k == "sf_from_level" ) {
+        // takes effect at the next factorization
+        ctx->sf_from_level = ( value < 0. ) ? -1 : (int32_t)std::min( value, 1e9 );
+        ctx->tasks_dirty   = 1;
         return 0;
     }
     if ( k == "panel_mw" ) {
@@ -1465,6 +1713,9 @@ pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
     ctx->stats.solve_seconds = 1e-3 * ev_ms( e0, e1 );
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
+    if ( rc == 0 && ctx->kernels && ctx->solve_sf ) {
+        rc = sf_check( ctx );
+    }
     return rc;
 }
 
