@@ -79,7 +79,19 @@ int  pxd_create ( pxd_ctx_t **ctx, int device );
 void pxd_destroy( pxd_ctx_t *ctx );
 int  pxd_sync   ( pxd_ctx_t *ctx );
 int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
-/* Engine knobs: key names are documented in engine.hip (e.g. "timing"). */
+/*
+ * Engine knobs (returns 7 for an unknown key):
+ *   "timing"        1: one stream, per-phase kernel times in the stats
+ *   "model_log"     1: kernel timing log of the next factorization
+ *   "kernels"       1: matrix-core kernels (default); 0: scalar kernels
+ *   "panel_mw"      1: multi-workgroup Cholesky panels (default)
+ *   "upd_big_rows"  stacked rows from which updates use the tall tiles
+ *   "upd_small"     1: one-wavefront small updates (default)
+ *   "upd_nh"        k-groups per stage of the tall-tile update (1 or 2)
+ *   "prio"          1: raised wave priority on the critical path (default)
+ *   "solve_sf"      1: one-launch solve sweeps above the cut level (default)
+ *   "sf_from_level" first level of the one-launch part (-1: automatic)
+ */
 int  pxd_set_option( pxd_ctx_t *ctx, const char *key, double value );
 
 int  pxd_malloc ( pxd_ctx_t *ctx, size_t bytes, void **dptr );
