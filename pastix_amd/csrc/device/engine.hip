@@ -56,6 +56,9 @@ struct upd_group {
 
 }  // namespace
 
+// solve launches with fewer workgroups than this load every column at once
+#define PXD_SOLVE_WIDE_MAX 2048
+
 struct pxd_ctx_s {
     int         device = 0;
     hipStream_t stream = nullptr;
@@ -138,6 +141,9 @@ struct pxd_ctx_s {
     // dependency counters (forward cnt, rdy; backward cnt, rdy), which grow
     // by one pass's worth per pass (sf_epoch)
     int       solve_sf = 1;
+    // level solve launches with at least this many tiles load 16 columns
+    // per batch (higher occupancy); smaller ones load every column at once
+    int64_t   solve_wide_max = PXD_SOLVE_WIDE_MAX;
     int32_t   sf_from_level = -1;   // option: first level of the one-launch part (-1: automatic)
     int32_t   sf_lc = 0;            // the cut in use
     sf_task  *dsf_fw = nullptr, *dsf_bw = nullptr;
@@ -544,6 +550,17 @@ build_schedule( pxd_ctx_t *ctx )
         lv.f1 = solv.size();
         lv.d1 = diag.size();
         lv.i1 = invt.size();
+        // solve tasks of the level widest cblk first: the long workgroups
+        // start first and the launch's tail is short (stable: equal widths
+        // keep cblk order)
+        {
+            auto wider = [&]( const row_task &a, const row_task &b ) {
+                return ( ctx->hcb[a.cblk].lcolnum - ctx->hcb[a.cblk].fcolnum ) >
+                       ( ctx->hcb[b.cblk].lcolnum - ctx->hcb[b.cblk].fcolnum );
+            };
+            std::stable_sort( solv.begin() + lv.f0, solv.end(), wider );
+            std::stable_sort( diag.begin() + lv.d0, diag.end(), wider );
+        }
         lv.g1 = ctx->groups.size();
     }
     ctx->nupd  = upd.size() + updb.size() + upds.size();
@@ -671,9 +688,6 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
         }
     }
 }
-
-// solve launches with fewer workgroups than this load every column at once
-#define PXD_SOLVE_WIDE_MAX 2048
 
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1123,7 +1137,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
             hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
                                 ctx->dcb, DI, (const T *)X, ldx, Y, ldy, nrhs, 0 );
-            if ( nf >= PXD_SOLVE_WIDE_MAX ) {
+            if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
                 hipLaunchKernelGGL( ( k_solve_fwd_upd<T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                     ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
             }
@@ -1148,7 +1162,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
-            if ( nf >= PXD_SOLVE_WIDE_MAX ) {
+            if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
                 hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
                                     ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const T *)X, ldx, Y, ldy, nrhs );
             }
@@ -1395,6 +1409,10 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     }
     if ( k == "solve_sf" ) {
         ctx->solve_sf = ( value != 0. );
+        return 0;
+    }
+    if ( k == "solve_wide_max" ) {
+        ctx->solve_wide_max = (int64_t)std::max( 0., std::min( value, 1e15 ) );
         return 0;
     }
     if ( k == "sf_from_level" ) {
