@@ -85,6 +85,9 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "model_log"     1: kernel timing log of the next factorization
  *   "kernels"       1: matrix-core kernels (default); 0: scalar kernels
  *   "panel_mw"      1: multi-workgroup Cholesky panels (default)
+ *   "panel_reg"     single precision: levels with cblks this wide or wider
+ *                   factor each diagonal tile in one workgroup's registers
+ *                   (0: never, the default)
  *   "upd_big_rows"  stacked rows from which updates use the tall tiles
  *   "upd_small"     1: one-wavefront small updates (default)
  *   "upd_nh"        k-groups per stage of the tall-tile update (1 or 2)

@@ -160,6 +160,11 @@ struct pxd_ctx_s {
     // 1: Cholesky diagonal tiles factored by one workgroup per 32-row block
     // (k_potrf_mw); 0: one workgroup per cblk (k_potrf2)
     int          panel_mw = 1;
+    // single-precision Cholesky levels whose widest cblk has at least this
+    // many columns factor each diagonal tile in one workgroup's registers
+    // (k_potrf_reg); 0: never (default: a 330-column tile takes ~245 us on
+    // one CU against ~240 us over eleven workgroups)
+    int          panel_reg = 0;
     // 1: the factorization runs one stream, launches every blok update on
     // its own and records per-workgroup times into model_entries
     int          model_log = 0;
@@ -820,7 +825,17 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                                     ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
-                if ( ctx->panel_mw ) {
+                bool reg = false;
+                if constexpr ( std::is_same<T, float>::value ) {
+                    // single precision: the whole diagonal tile in one workgroup's registers
+                    reg = !mlog && ctx->panel_reg > 0 && lv.maxn >= ctx->panel_reg && lv.maxn <= PXD_REG_NMAX;
+                    if ( reg ) {
+                        hipLaunchKernelGGL( k_potrf_reg, dim3( np ), dim3( 64 * PXD_REG_WAVES ), 0, st,
+                                            ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
+                                            (uint64_t *)nullptr, ctx->prio );
+                    }
+                }
+                if ( !reg && ctx->panel_mw ) {
                     // one workgroup per 32-row block of each diagonal tile
                     const unsigned nq  = (unsigned)( lv.q1 - lv.q0 );
                     const size_t   shq = (size_t)( ( std::max( 1, lv.maxn ) + PXD_NB - 1 ) / PXD_NB ) * PXD_NB *
@@ -829,7 +844,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                                         ctx->dpan + lv.q0, ctx->dcb, L, DI, thresh, ctx->dscratch, ctx->dflags,
                                         ctx->dscratch + 2, stp( st_panel, lv.q0, 0, 0 ), ctx->prio );
                 }
-                else {
+                else if ( !reg ) {
                     hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
                                         ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
                                         stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
@@ -1419,6 +1434,10 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
         // takes effect at the next factorization
         ctx->sf_from_level = ( value < 0. ) ? -1 : (int32_t)std::min( value, 1e9 );
         ctx->tasks_dirty   = 1;
+        return 0;
+    }
+    if ( k == "panel_reg" ) {
+        ctx->panel_reg = (int)std::max( 0., std::min( value, 1e9 ) );
         return 0;
     }
     if ( k == "panel_mw" ) {

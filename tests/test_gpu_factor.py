@@ -46,6 +46,30 @@ def test_potrf_fp32_tiles_match_oracle(nx, kernels):
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("panel_reg", [1, 96, 0])
+@pytest.mark.parametrize("nx,kind", [(9, "lap7"), (17, "lap7"), (43, "lap7"), (13, "vlap7")])
+def test_register_panel_matches_oracle(nx, kind, panel_reg):
+    """Single-precision Cholesky with the diagonal tiles factored in one
+    workgroup's registers (k_potrf_reg) on every level (panel_reg = 1), on
+    the levels with cblks of 96 columns or more, or never (the default);
+    43^3 has a 331-column cblk (11 block columns, a partial last one), and
+    the 3-dof vector Laplacian has lognormal coefficients."""
+    spm = make_problem(kind, nx)
+    pd, iparm, dparm = analyzed(spm, nx, mixed=1, dof=3 if kind == "vlap7" else 1)
+    px.set_option(pd, "panel_reg", panel_reg)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    cbk = s["cblk"].reshape(-1, 6)[:s["cblknbr"]]
+    L = px.get_coeftab(pd, s["coefnbr"], np.float32)
+    F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float32), np.float32,
+                         nthreads=8)
+    assert iparm[px.IPARM_STATIC_PIVOTING] == F.npiv
+    assert cblk_diff(s, L, F.L, np.float32) <= 10.0
+    if nx == 43:
+        assert (cbk[:, 1] - cbk[:, 0] + 1).max() > 320
+    px.finalize(pd)
+
+
 @pytest.mark.parametrize("kernels", [1, 0])
 @pytest.mark.parametrize("nx,nrhs", [(8, 1), (20, 1), (20, 3), (12, 9), (30, 2)])
 def test_solve_fp64_backward_error(nx, nrhs, kernels):
