@@ -1783,7 +1783,9 @@ pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy, i
 static unsigned
 red_blocks( int64_t n )
 {
-    return (unsigned)std::min<int64_t>( 1024, std::max<int64_t>( 1, ( n + 255 ) / 256 ) );
+    // each workgroup ends with one atomic on its column's result, and the
+    // atomics on one address serialize: few workgroups, many values each
+    return (unsigned)std::min<int64_t>( 64, std::max<int64_t>( 1, ( n + 4095 ) / 4096 ) );
 }
 
 int

@@ -220,28 +220,40 @@ k_solve_bwd_gather( const row_task *__restrict__ tasks, const dcblk *__restrict_
 // ---------------------------------------------------------------------------
 // SpMV on the full-pattern CSC: column j of the stored matrix gives row j of
 // the operator (symmetric: A itself; general: the transposed CSC is passed).
-// y = alpha * A x + beta * y for nrhs columns (real alpha / beta).
+// y = alpha * A x + beta * y for nrhs columns (real alpha / beta).  One
+// thread per column of the stored CSC (a row of A); the column's entries
+// are read once for up to PXD_SPMV_NR right-hand sides.
+#define PXD_SPMV_NR 8
 template <typename T, typename TV>
-__global__ void
+__global__ void __launch_bounds__( 256 )
 k_spmv( int64_t n, const int64_t *__restrict__ colptr, const int64_t *__restrict__ rowtab,
         const T *__restrict__ val, const TV *__restrict__ x, TV *__restrict__ y, int64_t ld,
         int nrhs, double alpha, double beta )
 {
+    typedef typename px_real<TV>::type RV;
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if ( j >= n ) {
         return;
     }
-    for ( int q = 0; q < nrhs; q++ ) {
-        TV s = px_zero( TV() );
-        for ( int64_t e = colptr[j]; e < colptr[j + 1]; e++ ) {
-            s = px_fma( val[e], x[rowtab[e] + q * ld], s );
+    const int64_t e0 = colptr[j], e1 = colptr[j + 1];
+    for ( int q0 = 0; q0 < nrhs; q0 += PXD_SPMV_NR ) {
+        TV s[PXD_SPMV_NR];
+#pragma unroll
+        for ( int q = 0; q < PXD_SPMV_NR; q++ ) s[q] = px_zero( TV() );
+        for ( int64_t e = e0; e < e1; e++ ) {
+            const T       v = val[e];
+            const int64_t r = rowtab[e];
+#pragma unroll
+            for ( int q = 0; q < PXD_SPMV_NR; q++ ) {
+                if ( q0 + q < nrhs ) s[q] = px_fma( v, x[r + ( q0 + q ) * ld], s[q] );
+            }
         }
-        TV *yy = y + j + q * ld;
-        if ( beta == 0. ) {
-            *yy = s * (typename px_real<TV>::type)alpha;
-        }
-        else {
-            *yy = s * (typename px_real<TV>::type)alpha + *yy * (typename px_real<TV>::type)beta;
+#pragma unroll
+        for ( int q = 0; q < PXD_SPMV_NR; q++ ) {
+            if ( q0 + q < nrhs ) {
+                TV *yy = y + j + ( q0 + q ) * ld;
+                *yy = ( beta == 0. ) ? s[q] * (RV)alpha : s[q] * (RV)alpha + *yy * (RV)beta;
+            }
         }
     }
 }
@@ -292,8 +304,14 @@ k_dot( int64_t n, const T *__restrict__ x, const T *__restrict__ y, T *__restric
     const int q = blockIdx.y;
     const T  *xx = x + q * n, *yy = y + q * n;
     T         s  = px_zero( T() );
-    for ( int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-          i += (int64_t)gridDim.x * blockDim.x ) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t       i  = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for ( ; i + 3 * st < n; i += 4 * st ) {
+        const T a0 = px_conj_mul( xx[i], yy[i] ), a1 = px_conj_mul( xx[i + st], yy[i + st] );
+        const T a2 = px_conj_mul( xx[i + 2 * st], yy[i + 2 * st] ), a3 = px_conj_mul( xx[i + 3 * st], yy[i + 3 * st] );
+        s = s + ( ( a0 + a1 ) + ( a2 + a3 ) );
+    }
+    for ( ; i < n; i += st ) {
         s = s + px_conj_mul( xx[i], yy[i] );
     }
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -314,8 +332,12 @@ k_nrm2sq( int64_t n, const T *__restrict__ x, double *__restrict__ res )
     const int q  = blockIdx.y;
     const T  *xx = x + q * n;
     double    s  = 0.;
-    for ( int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-          i += (int64_t)gridDim.x * blockDim.x ) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t       i  = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for ( ; i + 3 * st < n; i += 4 * st ) {
+        s += ( px_re2( xx[i] ) + px_re2( xx[i + st] ) ) + ( px_re2( xx[i + 2 * st] ) + px_re2( xx[i + 3 * st] ) );
+    }
+    for ( ; i < n; i += st ) {
         s += px_re2( xx[i] );
     }
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -342,8 +364,14 @@ k_maxabs( int64_t n, const T *__restrict__ x, unsigned long long *__restrict__ r
     const int q  = blockIdx.y;
     const T  *xx = x + q * n;
     double    s  = 0.;
-    for ( int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-          i += (int64_t)gridDim.x * blockDim.x ) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t       i  = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for ( ; i + 3 * st < n; i += 4 * st ) {
+        const double a = max_nan( (double)px_abs( xx[i] ), (double)px_abs( xx[i + st] ) );
+        const double b = max_nan( (double)px_abs( xx[i + 2 * st] ), (double)px_abs( xx[i + 3 * st] ) );
+        s              = max_nan( s, max_nan( a, b ) );
+    }
+    for ( ; i < n; i += st ) {
         s = max_nan( s, (double)px_abs( xx[i] ) );
     }
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;

@@ -184,6 +184,11 @@ pastixFinalize( pastix_data_t **pastix_data )
         free( pd->bcsc );
     }
     if ( pd->dev ) {
+        for ( int i = 0; i < PX_REF_CACHE; i++ ) {
+            if ( pd->ref_cache[i] ) {
+                pxd_free( pd->dev, pd->ref_cache[i] );
+            }
+        }
         pxd_destroy( pd->dev );
     }
     pastixModelsFree( pd->cpu_models );

@@ -9,6 +9,7 @@
  * the cblk buffer (2D tile layout, blend/solver_matrix_gen_utils.h:90).
  */
 #ifndef PX_INTERNAL_H
+#define PX_REF_CACHE 16   /* refinement device buffers kept per instance */
 #define PX_INTERNAL_H
 
 #include <stdint.h>
@@ -192,6 +193,8 @@ struct pastix_data_s {
     char            *dir_global;
     double           berr_target;        /* >0: refinement stops on backward error */
     int              refine_nrhs_batch;   /* 1: reference behaviour (one RHS at a time) */
+    void            *ref_cache[PX_REF_CACHE];       /* device buffers kept for the refinement */
+    size_t           ref_cache_bytes[PX_REF_CACHE];
     pastix_model_t  *cpu_models;          /* kernel cost models (common/pastixdata.h:106-107) */
     pastix_model_t  *gpu_models;
 };

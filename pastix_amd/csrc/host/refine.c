@@ -34,6 +34,48 @@ typedef struct ref_ctx_s {
     double        *dnrm;      /* device reals (nrhs) */
 } ref_ctx_t;
 
+/*
+ * Device buffers of the refinement are kept by the instance between calls
+ * (PX_REF_CACHE slots, freed by pastixFinalize): a call takes the smallest
+ * cached buffer that is large enough, else allocates, and hands its buffers
+ * back when it returns.  Repeated refinements then allocate nothing.
+ */
+static int
+ref_alloc( ref_ctx_t *r, size_t bytes, void **p )
+{
+    pastix_data_t *pd   = r->pd;
+    int            best = -1;
+    for ( int i = 0; i < PX_REF_CACHE; i++ ) {
+        if ( pd->ref_cache[i] && pd->ref_cache_bytes[i] >= bytes &&
+             ( best < 0 || pd->ref_cache_bytes[i] < pd->ref_cache_bytes[best] ) ) {
+            best = i;
+        }
+    }
+    if ( best >= 0 ) {
+        *p                  = pd->ref_cache[best];
+        pd->ref_cache[best] = NULL;
+        return 0;
+    }
+    return pxd_malloc( r->dev, bytes, p );
+}
+
+static void
+ref_release( ref_ctx_t *r, void *p, size_t bytes )
+{
+    pastix_data_t *pd = r->pd;
+    if ( p == NULL ) {
+        return;
+    }
+    for ( int i = 0; i < PX_REF_CACHE; i++ ) {
+        if ( pd->ref_cache[i] == NULL ) {
+            pd->ref_cache[i]       = p;
+            pd->ref_cache_bytes[i] = bytes;
+            return;
+        }
+    }
+    pxd_free( r->dev, p );
+}
+
 static void
 put_scalars( ref_ctx_t *r, const zc *h, void *dptr, pastix_int_t cnt )
 {
@@ -139,8 +181,9 @@ gmres_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int
     pastix_int_t *mcol, iters = 0;
     int          rc = PASTIX_SUCCESS;
 
-    if ( pxd_malloc( r->dev, blk * im1, &V ) || pxd_malloc( r->dev, blk * im, &W ) ||
-         pxd_malloc( r->dev, im1 * nr * r->es, &dh ) || pxd_malloc( r->dev, im * nr * r->es, &dy ) ) {
+    V = W = dh = dy = NULL;
+    if ( ref_alloc( r, blk * im1, &V ) || ref_alloc( r, blk * im, &W ) ||
+         ref_alloc( r, im1 * nr * r->es, &dh ) || ref_alloc( r, im * nr * r->es, &dy ) ) {
         return PASTIX_ERR_OUTOFMEMORY;
     }
     H      = px_calloc( (size_t)nr * im1 * im, sizeof( zc ) );
@@ -267,10 +310,10 @@ gmres_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int
 done:
     pxd_sync( r->dev );
     *iters_out = iters;
-    pxd_free( r->dev, V );
-    pxd_free( r->dev, W );
-    pxd_free( r->dev, dh );
-    pxd_free( r->dev, dy );
+    ref_release( r, V, blk * im1 );
+    ref_release( r, W, blk * im );
+    ref_release( r, dh, im1 * nr * r->es );
+    ref_release( r, dy, im * nr * r->es );
     free( H ); free( cs ); free( sn ); free( g ); free( h ); free( y ); free( alpha );
     free( normb ); free( target ); free( res ); free( conv ); free( active ); free( mcol );
     return rc;
@@ -290,7 +333,7 @@ sr_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     double       normA = 0.;
     int          rc = PASTIX_SUCCESS;
 
-    if ( pxd_malloc( r->dev, blk, &R ) ) {
+    if ( ref_alloc( r, blk, &R ) ) {
         return PASTIX_ERR_OUTOFMEMORY;
     }
     pxd_matrix_norm_inf( r->dev, &normA );
@@ -315,7 +358,7 @@ sr_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     }
     pxd_sync( r->dev );
     *iters_out = iters;
-    pxd_free( r->dev, R );
+    ref_release( r, R, blk );
     free( normb ); free( target ); free( conv ); free( one );
     return rc;
 }
@@ -364,7 +407,7 @@ bicgstab_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_
     int          rc = PASTIX_SUCCESS;
 
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
-        if ( pxd_malloc( r->dev, blk, vec[i] ) ) {
+        if ( ref_alloc( r, blk, vec[i] ) ) {
             return PASTIX_ERR_OUTOFMEMORY;
         }
     }
@@ -448,7 +491,7 @@ bicgstab_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_
     pxd_sync( r->dev );
     *iters_out = iters;
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
-        pxd_free( r->dev, *vec[i] );
+        ref_release( r, *vec[i], blk );
     }
     free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( beta );
     free( om ); free( t1 ); free( t2 );
@@ -479,7 +522,7 @@ cg_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     int          rc = PASTIX_SUCCESS;
 
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
-        if ( pxd_malloc( r->dev, blk, vec[i] ) ) {
+        if ( ref_alloc( r, blk, vec[i] ) ) {
             return PASTIX_ERR_OUTOFMEMORY;
         }
     }
@@ -536,7 +579,7 @@ cg_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     pxd_sync( r->dev );
     *iters_out = iters;
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
-        pxd_free( r->dev, *vec[i] );
+        ref_release( r, *vec[i], blk );
     }
     free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( rz ); free( t1 );
     return rc;
@@ -559,7 +602,7 @@ px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_
     r.n    = pd->bcsc->n;
     r.cplx = ( pd->bcsc->flttype == SpmComplex64 );
     r.es   = r.cplx ? 16 : 8;
-    if ( pxd_malloc( r.dev, 64 * 16, &r.dscal ) || pxd_malloc( r.dev, 64 * 8, (void **)&r.dnrm ) ) {
+    if ( ref_alloc( &r, 64 * 16, &r.dscal ) || ref_alloc( &r, 64 * 8, (void **)&r.dnrm ) ) {
         return PASTIX_ERR_OUTOFMEMORY;
     }
 
@@ -591,8 +634,8 @@ px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_
             itmax = px_imax( itmax, it );
         }
     }
-    pxd_free( r.dev, r.dscal );
-    pxd_free( r.dev, r.dnrm );
+    ref_release( &r, r.dscal, 64 * 16 );
+    ref_release( &r, r.dnrm, 64 * 8 );
     *iters = itmax;
     return rc;
 }
