@@ -555,17 +555,6 @@ build_schedule( pxd_ctx_t *ctx )
         lv.f1 = solv.size();
         lv.d1 = diag.size();
         lv.i1 = invt.size();
-        // solve tasks of the level widest cblk first: the long workgroups
-        // start first and the launch's tail is short (stable: equal widths
-        // keep cblk order)
-        {
-            auto wider = [&]( const row_task &a, const row_task &b ) {
-                return ( ctx->hcb[a.cblk].lcolnum - ctx->hcb[a.cblk].fcolnum ) >
-                       ( ctx->hcb[b.cblk].lcolnum - ctx->hcb[b.cblk].fcolnum );
-            };
-            std::stable_sort( solv.begin() + lv.f0, solv.end(), wider );
-            std::stable_sort( diag.begin() + lv.d0, diag.end(), wider );
-        }
         lv.g1 = ctx->groups.size();
     }
     ctx->nupd  = upd.size() + updb.size() + upds.size();

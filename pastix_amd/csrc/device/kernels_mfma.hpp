@@ -1283,7 +1283,8 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ int32_t sLd[64];
     __shared__ T       sDi[PXD_TRSM_NT * 16];        // LDL^T: 1 / D of the cblk
 
-    const row_task t    = tasks[blockIdx.x];
+    const int32_t  ti   = xcd_task( blockIdx.x, gridDim.x );
+    const row_task t    = tasks[ti];
     const uint64_t t_stamp = stamp_begin( stamps );
     wave_priority( prio );
     const dcblk    c    = cb[t.cblk];
@@ -1399,7 +1400,7 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
         }
         __syncthreads();   // the pass's stores precede the next pass's loads
     }
-    stamp_end( stamps, blockIdx.x, t_stamp );
+    stamp_end( stamps, ti, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -1425,7 +1426,8 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     __shared__ search_keys sk;
     __shared__ T           sDk[PXD_TRSM_NT * 16];   // LDL^T: D of the source cblk
 
-    const upd_task t    = tasks[blockIdx.x];
+    const int32_t  ti   = xcd_task( blockIdx.x, gridDim.x );
+    const upd_task t    = tasks[ti];
     const uint64_t t_stamp = stamp_begin( stamps );
     wave_priority( prio );
     const int      tid  = threadIdx.x;
@@ -1567,7 +1569,7 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
                     if ( pv[i][j][v] ) p[i][v][(int64_t)cc * sDstLd[r]] = old[i][j][v] - M::get( acc[i][j], v );
                 }
     }
-    stamp_end( stamps, blockIdx.x, t_stamp );
+    stamp_end( stamps, ti, t_stamp );
 }
 
 // ---------------------------------------------------------------------------
@@ -1948,7 +1950,7 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     T *sb = reinterpret_cast<T *>( smem_raw );      // [n][nrhs]
     __shared__ T sred[NG][CR][PXD_MAXRHS + 1];
 
-    const row_task tk  = tasks[blockIdx.x];
+    const row_task tk  = tasks[xcd_task( blockIdx.x, gridDim.x )];
     const dcblk    c   = cb[tk.cblk];
     const int32_t  n   = c.lcolnum - c.fcolnum + 1;
     const int32_t  R0  = tk.r0;
@@ -2012,7 +2014,7 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
     __shared__ int64_t sBase[64];
     __shared__ int32_t sLd[64];
 
-    const row_task t   = tasks[blockIdx.x];
+    const row_task t   = tasks[xcd_task( blockIdx.x, gridDim.x )];
     const dcblk    c   = cb[t.cblk];
     const int32_t  n   = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb  = cb[t.cblk + 1].fblok;

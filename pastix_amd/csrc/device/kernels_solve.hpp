@@ -10,6 +10,7 @@
 // launch), so the factor is streamed once per sweep for all of them.
 #pragma once
 #include "kernels_fact.hpp"
+#include "mfma.hpp"
 
 #define PXD_MAXRHS 8
 
@@ -102,7 +103,7 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
     T *sx = reinterpret_cast<T *>( smem_raw );    // [n][nrhs]
     __shared__ T sred[4][64][PXD_MAXRHS + 1];
 
-    const row_task t   = tasks[blockIdx.x];
+    const row_task t   = tasks[xcd_task( blockIdx.x, gridDim.x )];
     const dcblk    c   = cb[t.cblk];
     const int32_t  n   = c.lcolnum - c.fcolnum + 1;
     const int32_t  lb  = cb[t.cblk + 1].fblok;
