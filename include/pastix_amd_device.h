@@ -94,6 +94,10 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "prio"          1: raised wave priority on the critical path (default)
  *   "solve_sf"      1: one-launch solve sweeps above the cut level (default)
  *   "sf_from_level" first level of the one-launch part (-1: automatic)
+ *   "solve_wide_max" level-solve launches with at least this many tiles load
+ *                   16 columns per batch
+ *   "solve_vec_double" 1: a single factor solves double vectors directly;
+ *                   0: vectors rounded to single and back (default)
  */
 int  pxd_set_option( pxd_ctx_t *ctx, const char *key, double value );
 

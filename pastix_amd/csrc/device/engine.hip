@@ -141,6 +141,12 @@ struct pxd_ctx_s {
     // dependency counters (forward cnt, rdy; backward cnt, rdy), which grow
     // by one pass's worth per pass (sf_epoch)
     int       solve_sf = 1;
+    // 1: a single factor solves double vectors directly (entries widened as
+    // they are read); 0 (default): the vectors are converted to single and
+    // back, as in the reference.  On the 100^3 Laplacian the double vectors
+    // lower the first solve's backward error from 4.5e-7 to 2.7e-7 but the
+    // refinement still takes two iterations, and the solve is twice as slow.
+    int       solve_vec_double = 0;
     // level solve launches with at least this many tiles load 16 columns
     // per batch (higher occupancy); smaller ones load every column at once
     int64_t   solve_wide_max = PXD_SOLVE_WIDE_MAX;
@@ -1021,11 +1027,11 @@ template <typename T> struct sf_nr_max { static constexpr int value = 16; };
 template <> struct sf_nr_max<pxcd> { static constexpr int value = 8; };
 
 // The one-launch part of a sweep (the levels from sf_lc up) for nrhs <= NR
-// columns of X (factor precision, leading dimension ldx), Y the second
-// vector (ld n).  The forward launch opens a pass (epoch).
-template <typename T, int NR>
+// columns of X (vector precision V, leading dimension ldx), Y the second
+// vector (ld n); factor entries T.  The forward launch opens a pass (epoch).
+template <typename T, typename V, int NR>
 int
-sf_launch( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
+sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
 {
     constexpr int B  = 128 / (int)sizeof( T );   // columns per load batch of a forward tile
     const bool    lu = ( ctx->facto == FACT_GETRF );
@@ -1033,9 +1039,9 @@ sf_launch( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
     const T      *DI = (const T *)ctx->ddinv;
     const T      *LB = lu ? (const T *)ctx->dU : L;
     const T      *DB = lu ? (const T *)ctx->dudinv : DI;
-    const size_t  sh = sf_lds_bytes<T, NR>( ctx->colmax );
-    if ( ctx->colmax > PXD_TRSM_NT * 16 || sh > lds_room( (const void *)k_sf_forward<T, NR, B>, ctx->lds_max ) ||
-         sh > lds_room( (const void *)k_sf_backward<T, NR>, ctx->lds_max ) ) {
+    const size_t  sh = sf_lds_bytes<V, NR>( ctx->colmax );
+    if ( ctx->colmax > PXD_TRSM_NT * 16 || sh > lds_room( (const void *)k_sf_forward<T, V, NR, B>, ctx->lds_max ) ||
+         sh > lds_room( (const void *)k_sf_backward<T, V, NR>, ctx->lds_max ) ) {
         return 3;
     }
     if ( forward ) {
@@ -1048,12 +1054,12 @@ sf_launch( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
     uint32_t     *cnt = ctx->dsf_cnt;
     int          *err = ctx->dscratch + 8;
     if ( forward ) {
-        hipLaunchKernelGGL( ( k_sf_forward<T, NR, B> ), dim3( (unsigned)ctx->nsf_fw ), dim3( PXD_THREADS ), sh,
+        hipLaunchKernelGGL( ( k_sf_forward<T, V, NR, B> ), dim3( (unsigned)ctx->nsf_fw ), dim3( PXD_THREADS ), sh,
                             ctx->stream, ctx->dsf_fw, ctx->dsf_facing, ctx->dsf_need, ctx->dcb, ctx->dbk, L, DI, X,
                             ldx, Y, ctx->n, nrhs, ctx->sf_epoch, cnt, cnt + nc, err );
     }
     else {
-        hipLaunchKernelGGL( ( k_sf_backward<T, NR> ), dim3( (unsigned)ctx->nsf_bw ), dim3( PXD_THREADS ), sh,
+        hipLaunchKernelGGL( ( k_sf_backward<T, V, NR> ), dim3( (unsigned)ctx->nsf_bw ), dim3( PXD_THREADS ), sh,
                             ctx->stream, ctx->dsf_bw, ctx->dsf_facing, ctx->dcb, ctx->dbk, LB, DB, X, ldx, Y, ctx->n,
                             nrhs, ctx->sf_epoch, cnt + 2 * nc, cnt + 3 * nc, err );
     }
@@ -1061,21 +1067,22 @@ sf_launch( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
     return 0;
 }
 
-template <typename T>
+template <typename T, typename V>
 int
-sf_sweep( pxd_ctx_t *ctx, bool forward, T *X, int64_t ldx, int nrhs, T *Y )
+sf_sweep( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
 {
-    return ( nrhs <= 4 ) ? sf_launch<T, 4>( ctx, forward, X, ldx, nrhs, Y )
-                         : sf_launch<T, sf_nr_max<T>::value>( ctx, forward, X, ldx, nrhs, Y );
+    return ( nrhs <= 4 ) ? sf_launch<T, V, 4>( ctx, forward, X, ldx, nrhs, Y )
+                         : sf_launch<T, V, sf_nr_max<V>::value>( ctx, forward, X, ldx, nrhs, Y );
 }
 
-// Right-hand sides per solve pass: the level kernels take PXD_MAXRHS, the
-// one-launch kernels sf_nr_max when they cover every level.
-template <typename T>
+// Right-hand sides per solve pass (vector precision V): the level kernels
+// take PXD_MAXRHS, the one-launch kernels sf_nr_max when they cover every
+// level.
+template <typename V>
 int
 solve_pass_rhs( const pxd_ctx_t *ctx )
 {
-    return ( ctx->kernels && ctx->solve_sf && ctx->sf_lc == 0 ) ? sf_nr_max<T>::value : PXD_MAXRHS;
+    return ( ctx->kernels && ctx->solve_sf && ctx->sf_lc == 0 ) ? sf_nr_max<V>::value : PXD_MAXRHS;
 }
 
 // After the passes of a solve: a timed-out wait leaves the counters
@@ -1096,9 +1103,13 @@ sf_check( pxd_ctx_t *ctx )
     return 0;
 }
 
-template <typename T>
+// Forward and backward sweeps for nrhs columns of X (vector precision V,
+// leading dimension ldx) with the factor of type T: V is T, or its double
+// precision counterpart for a single factor (option solve_vec_double; the
+// matrix-core kernels only).
+template <typename T, typename V>
 int
-solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
+solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
 {
     const T    *L  = (const T *)ctx->dL;
     const T    *DI = (const T *)ctx->ddinv;
@@ -1116,74 +1127,75 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         //             update reads x_k from Y and subtracts into X;
         // backward -- the gather reads the solved rows from X and subtracts
         //             into Y; the diagonal reads Y and writes X.
-        size_t need = (size_t)ctx->n * std::max( PXD_MAXRHS, sf_nr_max<T>::value ) * sizeof( T );
+        size_t need = (size_t)ctx->n * std::max( PXD_MAXRHS, sf_nr_max<V>::value ) * sizeof( V );
         if ( ctx->dsolvy_bytes < need ) {
             if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
             PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
             ctx->dsolvy_bytes = need;
         }
-        T            *Y   = (T *)ctx->dsolvy;
+        V            *Y   = (V *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
         // levels [0, nlv) run level by level, the others in one launch per sweep
         const int     nlv = ctx->solve_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
-        const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( T );
-        if ( nlv > 0 && ( nrhs > PXD_MAXRHS || shmax > lds_room( (const void *)k_solve_diag_chunk<T>, ctx->lds_max ) ||
-                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, 16>, ctx->lds_max ) ||
-                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, ctx->lds_max ) ) ) {
+        const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( V );
+        if ( nlv > 0 && ( nrhs > PXD_MAXRHS || shmax > lds_room( (const void *)k_solve_diag_chunk<T, V>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, V, 16>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, ctx->lds_max ) ) ) {
             return 3;
         }
         for ( int li = 0; li < nlv; li++ ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
-            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
-            hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
-                                ctx->dcb, DI, (const T *)X, ldx, Y, ldy, nrhs, 0 );
+            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
+            hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
+                                ctx->ddiag + lv.d0, ctx->dcb, DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
             if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
-                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
+                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, V, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const V *)Y, ldy );
             }
             else if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, solve_wide<T>::value> ), dim3( nf ), dim3( PXD_THREADS ), sh,
-                                    s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)Y, ldy );
+                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, V, solve_wide<T>::value> ), dim3( nf ), dim3( PXD_THREADS ),
+                                    sh, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const V *)Y,
+                                    ldy );
             }
         }
         int rc = 0;
-        if ( ctx->solve_sf && ( rc = sf_sweep<T>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( ctx->solve_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
             return rc;
         }
         if ( ctx->facto == FACT_SYTRF ) {
-            hipLaunchKernelGGL( ( k_diag_scale<T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
+            hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
                                 ctx->dcol2cblk, ctx->dcb, L, Y, ldy, nrhs );
         }
-        if ( ctx->solve_sf && ( rc = sf_sweep<T>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( ctx->solve_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
             return rc;
         }
         for ( int li = nlv - 1; li >= 0; li-- ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
-            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( T );
+            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
             if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const T *)X, ldx, Y, ldy, nrhs );
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, V, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const V *)X, ldx, Y, ldy, nrhs );
             }
             else if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, solve_wide<T>::value / 16> ), dim3( nf ),
+                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, V, solve_wide<T>::value / 16> ), dim3( nf ),
                                     dim3( PXD_THREADS ), 0, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB,
-                                    (const T *)X, ldx, Y, ldy, nrhs );
+                                    (const V *)X, ldx, Y, ldy, nrhs );
             }
-            hipLaunchKernelGGL( ( k_solve_diag_chunk<T> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->ddiag + lv.d0,
-                                ctx->dcb, DB, (const T *)Y, ldy, X, ldx, nrhs, 1 );
+            hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
+                                ctx->ddiag + lv.d0, ctx->dcb, DB, (const V *)Y, ldy, X, ldx, nrhs, 1 );
         }
         PXD_CHECK( hipGetLastError() );
         return 0;
     }
 
-    // scalar kernels (real factors), in place
-    if constexpr ( !std::is_same<T, typename px_real<T>::type>::value ) {
+    // scalar kernels (real factors, vectors in the factor precision), in place
+    if constexpr ( !std::is_same<T, typename px_real<T>::type>::value || !std::is_same<T, V>::value ) {
         return 3;
     }
     else {
@@ -1195,7 +1207,7 @@ solve_sweeps( pxd_ctx_t *ctx, T *X, int64_t ldx, int nrhs )
         hipLaunchKernelGGL( ( k_solve_diag<T> ), dim3( np ), dim3( PXD_THREADS ), sh, s,
                             ctx->dlist + lv.c0, ctx->dcb, L, DI, X, ldx, nrhs, 0, 0 );
         if ( nf ) {
-            hipLaunchKernelGGL( ( k_solve_fwd_upd<T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+            hipLaunchKernelGGL( ( k_solve_fwd_upd<T, T, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                 ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const T *)X, ldx );
         }
     }
@@ -1224,6 +1236,20 @@ set_lds_room( const void *fn, size_t lds )
     (void)hipFuncSetAttribute( fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_room( fn, lds ) );
 }
 
+// the solve kernels for factor entries T and vectors V
+template <typename T, typename V>
+void
+set_lds_attributes_solve( size_t lds )
+{
+    set_lds_room( (const void *)k_solve_fwd_upd<T, V, 16>, lds );
+    set_lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, lds );
+    set_lds_room( (const void *)k_solve_diag_chunk<T, V>, lds );
+    set_lds_room( (const void *)k_sf_forward<T, V, 4, 128 / sizeof( T )>, lds );
+    set_lds_room( (const void *)k_sf_backward<T, V, 4>, lds );
+    set_lds_room( (const void *)k_sf_forward<T, V, sf_nr_max<V>::value, 128 / sizeof( T )>, lds );
+    set_lds_room( (const void *)k_sf_backward<T, V, sf_nr_max<V>::value>, lds );
+}
+
 // the matrix-core kernels (all four arithmetics)
 template <typename T>
 void
@@ -1235,13 +1261,13 @@ set_lds_attributes_mfma( size_t lds )
     }
     set_lds_room( (const void *)k_potrf2<T, true>, lds );
     set_lds_room( (const void *)k_trtri<T>, lds );
-    set_lds_room( (const void *)k_solve_fwd_upd<T, 16>, lds );
-    set_lds_room( (const void *)k_solve_fwd_upd<T, solve_wide<T>::value>, lds );
-    set_lds_room( (const void *)k_solve_diag_chunk<T>, lds );
-    set_lds_room( (const void *)k_sf_forward<T, 4, 128 / sizeof( T )>, lds );
-    set_lds_room( (const void *)k_sf_backward<T, 4>, lds );
-    set_lds_room( (const void *)k_sf_forward<T, sf_nr_max<T>::value, 128 / sizeof( T )>, lds );
-    set_lds_room( (const void *)k_sf_backward<T, sf_nr_max<T>::value>, lds );
+    set_lds_attributes_solve<T, T>( lds );
+    if constexpr ( std::is_same<T, float>::value ) {
+        set_lds_attributes_solve<float, double>( lds );
+    }
+    if constexpr ( std::is_same<T, pxcf>::value ) {
+        set_lds_attributes_solve<pxcf, pxcd>( lds );
+    }
 }
 
 // and the scalar kernels (real arithmetics)
@@ -1256,21 +1282,35 @@ set_lds_attributes( size_t lds )
 }
 
 // Solve of nrhs vectors of type TV (double or complex double) with a factor
-// of type T: directly when the types agree, else through vectors converted
-// to the factor precision and back, PXD_MAXRHS columns at a time.
+// of type T: directly when the types agree or, for a single factor on the
+// matrix-core kernels with option solve_vec_double, with the factor entries
+// widened as they are read (the preconditioner of the refinement is then
+// the factor applied without rounding of the vectors); otherwise through
+// vectors converted to the factor precision and back, as the reference's
+// mixed solve does.
 template <typename TV, typename T>
 int
 solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
 {
-    int       rc   = 0;
-    const int pass = solve_pass_rhs<T>( ctx );
+    int rc = 0;
     if constexpr ( std::is_same<TV, T>::value ) {
+        const int pass = solve_pass_rhs<T>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T>( ctx, (T *)dx + q0 * ldx, ldx, nq );
+            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq );
         }
+        return rc;
+    }
+    else if ( ctx->kernels && ctx->solve_vec_double ) {
+        const int pass = solve_pass_rhs<TV>( ctx );
+        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
+            int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
+            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq );
+        }
+        return rc;
     }
     else {
+        const int pass = solve_pass_rhs<T>( ctx );
         size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, pass ) * sizeof( T );
         if ( ctx->dvecs_bytes < need ) {
             if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
@@ -1284,7 +1324,7 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
                 hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
             }
-            rc = solve_sweeps<T>( ctx, xs, ctx->n, nq );
+            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<TV, T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, xs + q * ctx->n, (TV *)dx + ( q0 + q ) * ldx );
@@ -1413,6 +1453,10 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     }
     if ( k == "solve_sf" ) {
         ctx->solve_sf = ( value != 0. );
+        return 0;
+    }
+    if ( k == "solve_vec_double" ) {
+        ctx->solve_vec_double = ( value != 0. );
         return 0;
     }
     if ( k == "solve_wide_max" ) {

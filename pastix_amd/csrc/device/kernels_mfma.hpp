@@ -1938,17 +1938,17 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
 // lanes of a 16-thread group read 16 consecutive elements (rows of Linv
 // forward, columns of Linv, i.e. rows of Linv^T, backward).
 #define PXD_DIAG_ROWS 16
-template <typename T>
+template <typename T, typename V>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ Linv,
-                    const T *__restrict__ Xin, int64_t ldx, T *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
+                    const V *__restrict__ Xin, int64_t ldx, V *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
 {
     constexpr int CR   = PXD_DIAG_ROWS;
     constexpr int NG   = PXD_THREADS / CR;          // column phases
     constexpr int MAXU = ( PXD_TRSM_NT * 16 + NG - 1 ) / NG;
     extern __shared__ unsigned char smem_raw[];
-    T *sb = reinterpret_cast<T *>( smem_raw );      // [n][nrhs]
-    __shared__ T sred[NG][CR][PXD_MAXRHS + 1];
+    V *sb = reinterpret_cast<V *>( smem_raw );      // [n][nrhs]
+    __shared__ V sred[NG][CR][PXD_MAXRHS + 1];
 
     const row_task tk  = tasks[xcd_task( blockIdx.x, gridDim.x )];
     const dcblk    c   = cb[tk.cblk];
@@ -1956,7 +1956,7 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     const int32_t  R0  = tk.r0;
     const int      tid = threadIdx.x;
     const T       *LI  = Linv + c.dinvoff;
-    const T       *x   = Xin + c.fcolnum;
+    const V       *x   = Xin + c.fcolnum;
     const int      r   = trans ? tid / NG : tid % CR;
     const int      g   = trans ? tid % NG : tid / CR;
     const int32_t  i   = R0 + r;
@@ -1974,16 +1974,17 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     }
     __syncthreads();
 
-    T acc[PXD_MAXRHS];
+    V acc[PXD_MAXRHS];
 #pragma unroll
-    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
+    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (V)0;
 #pragma unroll
     for ( int u = 0; u < MAXU; u++ ) {
         const int32_t j = g + NG * u;
         if ( j < n ) {
+            const V av = px_cvt<V>( a[u] );
 #pragma unroll
             for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                if ( q < nrhs ) acc[q] += a[u] * sb[j * nrhs + q];
+                if ( q < nrhs ) acc[q] += av * sb[j * nrhs + q];
             }
         }
     }
@@ -1992,7 +1993,7 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
     __syncthreads();
     if ( tid < CR * nrhs ) {
         const int r2 = tid % CR, q = tid / CR;
-        T         v  = (T)0;
+        V         v  = (V)0;
 #pragma unroll
         for ( int p2 = 0; p2 < NG; p2++ ) v += sred[p2][r2][q];
         if ( R0 + r2 < n ) Yout[c.fcolnum + R0 + r2 + q * ldy] = v;
@@ -2002,15 +2003,15 @@ k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict_
 // Backward gather of a 64-row tile: x_k -= A(rows,:)^T X[grow].  The tile is
 // staged through LDS by 64-column chunks (coalesced loads), four threads per
 // column reduce 16 rows each, one atomic per column and RHS.
-template <typename T, int NCH>
+template <typename T, typename V, int NCH>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-                     const dblok *__restrict__ bk, const T *__restrict__ L, const T *X,
-                     int64_t ldx, T *Yt, int64_t ldt, int nrhs )
+                     const dblok *__restrict__ bk, const T *__restrict__ L, const V *X,
+                     int64_t ldx, V *Yt, int64_t ldt, int nrhs )
 {
     __shared__ T       sA[64][64 + 1];           // sA[col][row]
-    __shared__ T       sx[64][PXD_MAXRHS];
-    __shared__ T       sred[4][64][PXD_MAXRHS + 1];
+    __shared__ V       sx[64][PXD_MAXRHS];
+    __shared__ V       sred[4][64][PXD_MAXRHS + 1];
     __shared__ int64_t sBase[64];
     __shared__ int32_t sLd[64];
 
@@ -2031,12 +2032,12 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
             sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
             sLd[tid]   = bb.lrownum - bb.frownum + 1;
             int32_t g  = bb.frownum + ( s - bb.rowoff );
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = ( q < nrhs ) ? X[g + q * ldx] : (T)0;
+            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = ( q < nrhs ) ? X[g + q * ldx] : (V)0;
         }
         else {
             sBase[tid] = -1;
             sLd[tid]   = 0;
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = (T)0;
+            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = (V)0;
         }
     }
     __syncthreads();
@@ -2067,12 +2068,12 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
             }
         }
         __syncthreads();
-        T acc[PXD_MAXRHS];
+        V acc[PXD_MAXRHS];
 #pragma unroll
-        for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
+        for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (V)0;
 #pragma unroll
         for ( int r = 16 * part; r < 16 * part + 16; r++ ) {
-            T a = sA[col][r];
+            const V a = px_cvt<V>( sA[col][r] );
 #pragma unroll
             for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] += a * sx[r][q];
         }
@@ -2081,7 +2082,7 @@ k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict
         __syncthreads();
         if ( part == 0 && c0 + col < n ) {
             for ( int q = 0; q < nrhs; q++ ) {
-                T v = sred[0][col][q] + sred[1][col][q] + sred[2][col][q] + sred[3][col][q];
+                V v = sred[0][col][q] + sred[1][col][q] + sred[2][col][q] + sred[3][col][q];
                 px_atomic_add( Yt + c.fcolnum + c0 + col + q * ldt, -v );
             }
         }

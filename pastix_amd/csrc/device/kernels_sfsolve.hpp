@@ -122,10 +122,10 @@ sf_stage( T *s, const T *v, int64_t ld, int32_t j0, int32_t j1, int nrhs, const 
 // LI of the diagonal block, ld n.  Thread (r, g) sums columns g, g + 16, ...
 // of row R0 + r; the rows go to out (ld ldo) and, if keep is given, to
 // keep[(R0 + r) * NR + q] in LDS.  sred: 4 x 16 x NR values of LDS.
-template <typename T, int NR>
+template <typename T, typename V, int NR>
 __device__ inline void
-sf_diag_rows( const T *__restrict__ LI, int32_t n, int32_t R0, bool trans, const T *sb, T *sred, T *out,
-              int64_t ldo, int nrhs, T *keep, int tid )
+sf_diag_rows( const T *__restrict__ LI, int32_t n, int32_t R0, bool trans, const V *sb, V *sred, V *out,
+              int64_t ldo, int nrhs, V *keep, int tid )
 {
     constexpr int CR   = PXD_SF_ROWS;
     constexpr int NG   = PXD_THREADS / CR;
@@ -141,15 +141,16 @@ sf_diag_rows( const T *__restrict__ LI, int32_t n, int32_t R0, bool trans, const
         const bool    ok = i < n && j < n && ( trans ? j >= i : j <= i );
         a[u] = ok ? LI[trans ? j + (int64_t)i * n : i + (int64_t)j * n] : (T)0;
     }
-    T acc[NR];
+    V acc[NR];
 #pragma unroll
-    for ( int q = 0; q < NR; q++ ) acc[q] = (T)0;
+    for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
 #pragma unroll
     for ( int u = 0; u < MAXU; u++ ) {
         const int32_t j = g + NG * u;
         if ( j < n && ( trans ? j >= i : j <= i ) ) {
+            const V av = px_cvt<V>( a[u] );
 #pragma unroll
-            for ( int q = 0; q < NR; q++ ) acc[q] += a[u] * sb[j * NR + q];
+            for ( int q = 0; q < NR; q++ ) acc[q] += av * sb[j * NR + q];
         }
     }
     if ( trans ) {
@@ -181,7 +182,7 @@ sf_diag_rows( const T *__restrict__ LI, int32_t n, int32_t R0, bool trans, const
         __syncthreads();
         for ( int idx = tid; idx < CR * NR; idx += PXD_THREADS ) {
             const int r2 = idx % CR, q = idx / CR;
-            const T   v  = sred[r2 * NR + q] + sred[( CR + r2 ) * NR + q] + sred[( 2 * CR + r2 ) * NR + q] +
+            const V   v  = sred[r2 * NR + q] + sred[( CR + r2 ) * NR + q] + sred[( 2 * CR + r2 ) * NR + q] +
                         sred[( 3 * CR + r2 ) * NR + q];
             if ( R0 + r2 < n ) {
                 if ( q < nrhs ) out[R0 + r2 + q * ldo] = v;
@@ -217,10 +218,10 @@ sf_row( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__restrict_
 // takes row r0 + r; wave w the columns [n w / 4, n (w + 1) / 4), loaded B
 // at a time.  red: 4 x 64 x NR values of LDS (may alias sx: it is written
 // after a barrier that follows the last read of sx).
-template <typename T, int NR, int B>
+template <typename T, typename V, int NR, int B>
 __device__ inline void
 sf_fwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__restrict__ cb,
-             const dblok *__restrict__ bk, const T *__restrict__ L, int32_t r0, const T *sx, T *red, T *X,
+             const dblok *__restrict__ bk, const T *__restrict__ L, int32_t r0, const V *sx, V *red, V *X,
              int64_t ldx, int nrhs, int tid )
 {
     const int32_t n    = c.lcolnum - c.fcolnum + 1;
@@ -228,9 +229,9 @@ sf_fwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
     int64_t       base;
     int32_t       ld, g;
     sf_row<T>( keys, c, k, cb, bk, n, r0 + r, base, ld, g );
-    T acc[NR];
+    V acc[NR];
 #pragma unroll
-    for ( int q = 0; q < NR; q++ ) acc[q] = (T)0;
+    for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
     if ( base >= 0 ) {
         const int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
         for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
@@ -240,8 +241,9 @@ sf_fwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
 #pragma unroll
             for ( int u = 0; u < B; u++ ) {
                 if ( cb0 + u < c1 ) {
+                    const V av = px_cvt<V>( a[u] );
 #pragma unroll
-                    for ( int q = 0; q < NR; q++ ) acc[q] += a[u] * sx[( cb0 + u ) * NR + q];
+                    for ( int q = 0; q < NR; q++ ) acc[q] += av * sx[( cb0 + u ) * NR + q];
                 }
             }
         }
@@ -254,7 +256,7 @@ sf_fwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
 #pragma unroll
         for ( int q = 0; q < NR; q++ ) {
             if ( q < nrhs ) {
-                const T v = red[r * NR + q] + red[( 64 + r ) * NR + q] + red[( 128 + r ) * NR + q] +
+                const V v = red[r * NR + q] + red[( 64 + r ) * NR + q] + red[( 128 + r ) * NR + q] +
                             red[( 192 + r ) * NR + q];
                 px_atomic_add( X + g + q * ldx, -v );
             }
@@ -268,11 +270,11 @@ sf_fwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
 // reduces 16 rows; the four parts are summed through sA after the chunk's
 // products.  The result is added atomically to Y (out == nullptr) or, for a
 // one-task cblk, written to gk[j * NR + q] in LDS (n <= 64).
-template <typename T, int NR>
+template <typename T, typename V, int NR>
 __device__ inline void
 sf_bwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__restrict__ cb,
-             const dblok *__restrict__ bk, const T *__restrict__ L, int32_t r0, const T *X, int64_t ldx, T *Y,
-             int64_t ldy, int nrhs, T *sA, T *sx, int64_t *sBase, int32_t *sLd, T *gk, int tid )
+             const dblok *__restrict__ bk, const T *__restrict__ L, int32_t r0, const V *X, int64_t ldx, V *Y,
+             int64_t ldy, int nrhs, V *sA, V *sx, int64_t *sBase, int32_t *sLd, V *gk, int tid )
 {
     const int32_t n = c.lcolnum - c.fcolnum + 1;
     if ( tid < 64 ) {
@@ -282,7 +284,7 @@ sf_bwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
         sBase[tid] = base;
         sLd[tid]   = ld;
 #pragma unroll
-        for ( int q = 0; q < NR; q++ ) sx[tid * NR + q] = ( base >= 0 && q < nrhs ) ? X[g + q * ldx] : (T)0;
+        for ( int q = 0; q < NR; q++ ) sx[tid * NR + q] = ( base >= 0 && q < nrhs ) ? X[g + q * ldx] : (V)0;
     }
     __syncthreads();
     constexpr int NU   = 64 * 64 / PXD_THREADS;
@@ -297,14 +299,14 @@ sf_bwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
             v[u] = ( rb >= 0 && cc < n ) ? L[rb + (int64_t)cc * rl] : (T)0;
         }
 #pragma unroll
-        for ( int u = 0; u < NU; u++ ) sA[( part + 4 * u ) * 65 + col] = v[u];
+        for ( int u = 0; u < NU; u++ ) sA[( part + 4 * u ) * 65 + col] = px_cvt<V>( v[u] );
         __syncthreads();
-        T acc[NR];
+        V acc[NR];
 #pragma unroll
-        for ( int q = 0; q < NR; q++ ) acc[q] = (T)0;
+        for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
 #pragma unroll
         for ( int rr = 16 * part; rr < 16 * part + 16; rr++ ) {
-            const T a = sA[col * 65 + rr];
+            const V a = sA[col * 65 + rr];
 #pragma unroll
             for ( int q = 0; q < NR; q++ ) acc[q] += a * sx[rr * NR + q];
         }
@@ -315,7 +317,7 @@ sf_bwd_tile( const int32_t *keys, const dcblk &c, int32_t k, const dcblk *__rest
         if ( part == 0 && c0 + col < n ) {
 #pragma unroll
             for ( int q = 0; q < NR; q++ ) {
-                const T s = sA[col * NR + q] + sA[( 64 + col ) * NR + q] + sA[( 128 + col ) * NR + q] +
+                const V s = sA[col * NR + q] + sA[( 64 + col ) * NR + q] + sA[( 128 + col ) * NR + q] +
                             sA[( 192 + col ) * NR + q];
                 if ( gk ) {
                     gk[( c0 + col ) * NR + q] = s;
@@ -344,18 +346,18 @@ sf_lds_bytes( int32_t nmax )
 
 // Forward sweep of one pass (NR right-hand sides at most): X holds b on
 // entry and is consumed; Y receives y = L^{-1} b.
-template <typename T, int NR, int B>
+template <typename T, typename V, int NR, int B>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sf_forward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ facing,
               const int32_t *__restrict__ need, const dcblk *__restrict__ cb, const dblok *__restrict__ bk,
-              const T *__restrict__ L, const T *__restrict__ DI, T *X, int64_t ldx, T *Y, int64_t ldy, int nrhs,
+              const T *__restrict__ L, const T *__restrict__ DI, V *X, int64_t ldx, V *Y, int64_t ldy, int nrhs,
               uint32_t epoch, uint32_t *cnt, uint32_t *rdy, int *err )
 {
     extern __shared__ unsigned char smem_raw[];
-    T *sbuf = reinterpret_cast<T *>( smem_raw );
+    V *sbuf = reinterpret_cast<V *>( smem_raw );
     __shared__ int32_t keys[PXD_ROWKEY_CAP];
-    __shared__ T       sred[4 * PXD_SF_ROWS * NR];
-    __shared__ T       sy[PXD_SF_ROWS * NR];
+    __shared__ V       sred[4 * PXD_SF_ROWS * NR];
+    __shared__ V       sy[PXD_SF_ROWS * NR];
 
     const sf_task t   = tasks[blockIdx.x];
     const int     tid = threadIdx.x;
@@ -369,10 +371,10 @@ k_sf_forward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fac
         }
         sf_acquire( tid );
         const int32_t R0 = ( t.kind == SF_DIAG ) ? t.r0 : 0;
-        sf_stage<T, NR>( sbuf, X + c.fcolnum, ldx, 0, min( n, R0 + PXD_SF_ROWS ), nrhs, nullptr, tid );
+        sf_stage<V, NR>( sbuf, X + c.fcolnum, ldx, 0, min( n, R0 + PXD_SF_ROWS ), nrhs, nullptr, tid );
         __syncthreads();
-        sf_diag_rows<T, NR>( DI + c.dinvoff, n, R0, false, sbuf, sred, Y + c.fcolnum, ldy, nrhs,
-                             ( t.kind == SF_LEAF ) ? sy : nullptr, tid );
+        sf_diag_rows<T, V, NR>( DI + c.dinvoff, n, R0, false, sbuf, sred, Y + c.fcolnum, ldy, nrhs,
+                                ( t.kind == SF_LEAF ) ? sy : nullptr, tid );
         if ( t.kind == SF_DIAG ) {
             sf_release( tid );
             if ( tid == 0 ) {
@@ -384,7 +386,7 @@ k_sf_forward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fac
         if ( t.f1 > t.f0 ) {
             stage_row_keys( keys, bk, c.fblok, cb[k + 1].fblok );
             __syncthreads();
-            sf_fwd_tile<T, NR, B>( keys, c, k, cb, bk, L, n, sy, sbuf, X, ldx, nrhs, tid );
+            sf_fwd_tile<T, V, NR, B>( keys, c, k, cb, bk, L, n, sy, sbuf, X, ldx, nrhs, tid );
         }
     }
     else {
@@ -392,10 +394,10 @@ k_sf_forward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fac
             sf_spin( rdy + k, epoch * (uint32_t)sf_nchunk( n ), err );
         }
         sf_acquire( tid );
-        sf_stage<T, NR>( sbuf, Y + c.fcolnum, ldy, 0, n, nrhs, nullptr, tid );
+        sf_stage<V, NR>( sbuf, Y + c.fcolnum, ldy, 0, n, nrhs, nullptr, tid );
         stage_row_keys( keys, bk, c.fblok, cb[k + 1].fblok );
         __syncthreads();
-        sf_fwd_tile<T, NR, B>( keys, c, k, cb, bk, L, t.r0, sbuf, sbuf, X, ldx, nrhs, tid );
+        sf_fwd_tile<T, V, NR, B>( keys, c, k, cb, bk, L, t.r0, sbuf, sbuf, X, ldx, nrhs, tid );
     }
     sf_release( tid );
     if ( tid == 0 ) {
@@ -406,18 +408,18 @@ k_sf_forward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fac
 // Backward sweep of one pass: Y holds y on entry (and is consumed), X
 // receives x = L^{-T} y.  LB, DB: the tiles and diagonal inverses of the
 // backward factor (L for Cholesky and LDL^T, U^T for LU).
-template <typename T, int NR>
+template <typename T, typename V, int NR>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sf_backward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ facing, const dcblk *__restrict__ cb,
-               const dblok *__restrict__ bk, const T *__restrict__ LB, const T *__restrict__ DB, T *X, int64_t ldx,
-               T *Y, int64_t ldy, int nrhs, uint32_t epoch, uint32_t *cnt, uint32_t *rdy, int *err )
+               const dblok *__restrict__ bk, const T *__restrict__ LB, const T *__restrict__ DB, V *X, int64_t ldx,
+               V *Y, int64_t ldy, int nrhs, uint32_t epoch, uint32_t *cnt, uint32_t *rdy, int *err )
 {
     extern __shared__ unsigned char smem_raw[];
-    T *sbuf = reinterpret_cast<T *>( smem_raw );
+    V *sbuf = reinterpret_cast<V *>( smem_raw );
     __shared__ int32_t keys[PXD_ROWKEY_CAP];
-    __shared__ T       sred[4 * PXD_SF_ROWS * NR];
-    __shared__ T       sx[64 * NR];
-    __shared__ T       gk[PXD_SF_ROWS * NR];
+    __shared__ V       sred[4 * PXD_SF_ROWS * NR];
+    __shared__ V       sx[64 * NR];
+    __shared__ V       gk[PXD_SF_ROWS * NR];
     __shared__ int64_t sBase[64];
     __shared__ int32_t sLd[64];
 
@@ -439,11 +441,11 @@ k_sf_backward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fa
         if ( t.f1 > t.f0 ) {
             stage_row_keys( keys, bk, c.fblok, cb[k + 1].fblok );
             __syncthreads();
-            sf_bwd_tile<T, NR>( keys, c, k, cb, bk, LB, ( t.kind == SF_LEAF ) ? n : t.r0, X, ldx, Y, ldy, nrhs,
-                                sbuf, sx, sBase, sLd, ( t.kind == SF_LEAF ) ? gk : nullptr, tid );
+            sf_bwd_tile<T, V, NR>( keys, c, k, cb, bk, LB, ( t.kind == SF_LEAF ) ? n : t.r0, X, ldx, Y, ldy, nrhs,
+                                   sbuf, sx, sBase, sLd, ( t.kind == SF_LEAF ) ? gk : nullptr, tid );
         }
         else if ( t.kind == SF_LEAF ) {
-            for ( int idx = tid; idx < PXD_SF_ROWS * NR; idx += PXD_THREADS ) gk[idx] = (T)0;
+            for ( int idx = tid; idx < PXD_SF_ROWS * NR; idx += PXD_THREADS ) gk[idx] = (V)0;
         }
         if ( t.kind == SF_TILE ) {
             sf_release( tid );
@@ -453,18 +455,18 @@ k_sf_backward( const sf_task *__restrict__ tasks, const int32_t *__restrict__ fa
             return;
         }
         __syncthreads();
-        sf_stage<T, NR>( sbuf, Y + c.fcolnum, ldy, 0, n, nrhs, gk, tid );
+        sf_stage<V, NR>( sbuf, Y + c.fcolnum, ldy, 0, n, nrhs, gk, tid );
     }
     else {
         if ( tid == 0 && c.stride > n ) {
             sf_spin( cnt + k, epoch * (uint32_t)sf_ntile( n, c.stride ), err );
         }
         sf_acquire( tid );
-        sf_stage<T, NR>( sbuf, Y + c.fcolnum, ldy, t.r0, n, nrhs, nullptr, tid );
+        sf_stage<V, NR>( sbuf, Y + c.fcolnum, ldy, t.r0, n, nrhs, nullptr, tid );
     }
     __syncthreads();
-    sf_diag_rows<T, NR>( DB + c.dinvoff, n, ( t.kind == SF_DIAG ) ? t.r0 : 0, true, sbuf, sred, X + c.fcolnum, ldx,
-                         nrhs, nullptr, tid );
+    sf_diag_rows<T, V, NR>( DB + c.dinvoff, n, ( t.kind == SF_DIAG ) ? t.r0 : 0, true, sbuf, sred, X + c.fcolnum,
+                            ldx, nrhs, nullptr, tid );
     sf_release( tid );
     if ( tid == 0 ) {
         sf_signal( rdy + k );

@@ -93,15 +93,15 @@ k_solve_diag( const int32_t *__restrict__ list, const dcblk *__restrict__ cb,
 template <typename T> struct solve_wide {
     static constexpr int value = 96 * 4 / (int)sizeof( T );
 };
-template <typename T, int B>
+template <typename T, typename V, int B>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-                 const dblok *__restrict__ bk, const T *__restrict__ L, T *X,
-                 int64_t ldx, int nrhs, const T *Xs, int64_t lds )
+                 const dblok *__restrict__ bk, const T *__restrict__ L, V *X,
+                 int64_t ldx, int nrhs, const V *Xs, int64_t lds )
 {
     extern __shared__ unsigned char smem_raw[];
-    T *sx = reinterpret_cast<T *>( smem_raw );    // [n][nrhs]
-    __shared__ T sred[4][64][PXD_MAXRHS + 1];
+    V *sx = reinterpret_cast<V *>( smem_raw );    // [n][nrhs]
+    __shared__ V sred[4][64][PXD_MAXRHS + 1];
 
     const row_task t   = tasks[xcd_task( blockIdx.x, gridDim.x )];
     const dcblk    c   = cb[t.cblk];
@@ -130,9 +130,9 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
         ld   = bb.lrownum - bb.frownum + 1;
         g    = bb.frownum + ( s - bb.rowoff );
     }
-    T acc[PXD_MAXRHS];
+    V acc[PXD_MAXRHS];
 #pragma unroll
-    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (T)0;
+    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (V)0;
     if ( valid ) {
         // the loads of a batch of B columns are all in flight at once
         const int32_t c0 = ( n * part ) / 4, c1 = ( n * ( part + 1 ) ) / 4;
@@ -143,9 +143,10 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
 #pragma unroll
             for ( int u = 0; u < B; u++ ) {
                 if ( cb0 + u < c1 ) {
+                    const V av = px_cvt<V>( a[u] );
 #pragma unroll
                     for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                        if ( q < nrhs ) acc[q] += a[u] * sx[( cb0 + u ) * nrhs + q];
+                        if ( q < nrhs ) acc[q] += av * sx[( cb0 + u ) * nrhs + q];
                     }
                 }
             }
@@ -158,7 +159,7 @@ k_solve_fwd_upd( const row_task *__restrict__ tasks, const dcblk *__restrict__ c
     __syncthreads();
     if ( part == 0 && valid ) {
         for ( int q = 0; q < nrhs; q++ ) {
-            T v = sred[0][r][q] + sred[1][r][q] + sred[2][r][q] + sred[3][r][q];
+            V v = sred[0][r][q] + sred[1][r][q] + sred[2][r][q] + sred[3][r][q];
             px_atomic_add( X + g + q * ldx, -v );
         }
     }
@@ -453,16 +454,17 @@ k_convert( int64_t n, const TI *__restrict__ in, TO *__restrict__ out )
 }
 
 // LDL^T solve, between the sweeps: y_i <- y_i / D_i, D read from the
-// diagonal of the cblk's diagonal tile (solve_cblk_zdiag, kernels/solve_ztrsmsp.c:629-679)
-template <typename T>
+// diagonal of the cblk's diagonal tile (solve_cblk_zdiag, kernels/solve_ztrsmsp.c:629-679);
+// factor entries T, vectors V (T or its double-precision counterpart)
+template <typename T, typename V>
 __global__ void
 k_diag_scale( int64_t n, const int32_t *__restrict__ col2cblk, const dcblk *__restrict__ cb,
-              const T *__restrict__ L, T *__restrict__ Y, int64_t ldy, int nrhs )
+              const T *__restrict__ L, V *__restrict__ Y, int64_t ldy, int nrhs )
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if ( i >= n ) return;
     const dcblk   c  = cb[col2cblk[i]];
     const int32_t nk = c.lcolnum - c.fcolnum + 1;
-    const T       d  = L[c.coefoff + (int64_t)( i - c.fcolnum ) * ( nk + 1 )];
+    const V       d  = px_cvt<V>( L[c.coefoff + (int64_t)( i - c.fcolnum ) * ( nk + 1 )] );
     for ( int q = 0; q < nrhs; q++ ) Y[i + q * ldy] = px_div( Y[i + q * ldy], d );
 }

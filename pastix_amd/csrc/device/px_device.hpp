@@ -95,6 +95,25 @@ __host__ __device__ inline double px_div( double a, double b ) { return a / b; }
 __host__ __device__ inline float  px_div( float a, float b ) { return a / b; }
 
 // fused multiply-add c + a*b
+// Widening of a factor entry to the vector precision of a solve (single
+// factor, double vectors) or identity.
+template <typename V, typename T> struct px_cvt_s;
+template <typename T> struct px_cvt_s<T, T> {
+    __device__ static inline T f( T x ) { return x; }
+};
+template <> struct px_cvt_s<double, float> {
+    __device__ static inline double f( float x ) { return (double)x; }
+};
+template <> struct px_cvt_s<pxcd, pxcf> {
+    __device__ static inline pxcd f( pxcf x ) { return pxcd( (double)x.re, (double)x.im ); }
+};
+template <typename V, typename T>
+__device__ inline V
+px_cvt( T x )
+{
+    return px_cvt_s<V, T>::f( x );
+}
+
 __device__ inline double px_fma( double a, double b, double c ) { return fma( a, b, c ); }
 __device__ inline float  px_fma( float a, float b, float c ) { return fmaf( a, b, c ); }
 __device__ inline pxcd   px_fma( pxcd a, pxcd b, pxcd c ) { return c + a * b; }
