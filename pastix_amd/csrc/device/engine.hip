@@ -1021,10 +1021,12 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
 }
 
 // ---------------------------------------------------------------------------
-// Right-hand sides per pass of the one-launch sweeps (the accumulators and
-// LDS of a tile grow with it).
-template <typename T> struct sf_nr_max { static constexpr int value = 16; };
-template <> struct sf_nr_max<pxcd> { static constexpr int value = 8; };
+// Right-hand sides of a pass that the one-launch sweeps take: with more, the
+// accumulators and LDS of a task lower the occupancy below what the level
+// kernels reach (measured: a complex64 solve of 16 RHS in passes of 8 took
+// 26 ms one-launch against 19 ms level by level), so larger passes run
+// level by level throughout.
+#define PXD_SF_NR 4
 
 // The one-launch part of a sweep (the levels from sf_lc up) for nrhs <= NR
 // columns of X (vector precision V, leading dimension ldx), Y the second
@@ -1071,18 +1073,15 @@ template <typename T, typename V>
 int
 sf_sweep( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
 {
-    return ( nrhs <= 4 ) ? sf_launch<T, V, 4>( ctx, forward, X, ldx, nrhs, Y )
-                         : sf_launch<T, V, sf_nr_max<V>::value>( ctx, forward, X, ldx, nrhs, Y );
+    return sf_launch<T, V, PXD_SF_NR>( ctx, forward, X, ldx, nrhs, Y );
 }
 
-// Right-hand sides per solve pass (vector precision V): the level kernels
-// take PXD_MAXRHS, the one-launch kernels sf_nr_max when they cover every
-// level.
+// Right-hand sides per solve pass.
 template <typename V>
 int
-solve_pass_rhs( const pxd_ctx_t *ctx )
+solve_pass_rhs( const pxd_ctx_t * )
 {
-    return ( ctx->kernels && ctx->solve_sf && ctx->sf_lc == 0 ) ? sf_nr_max<V>::value : PXD_MAXRHS;
+    return PXD_MAXRHS;
 }
 
 // After the passes of a solve: a timed-out wait leaves the counters
@@ -1127,7 +1126,7 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
         //             update reads x_k from Y and subtracts into X;
         // backward -- the gather reads the solved rows from X and subtracts
         //             into Y; the diagonal reads Y and writes X.
-        size_t need = (size_t)ctx->n * std::max( PXD_MAXRHS, sf_nr_max<V>::value ) * sizeof( V );
+        size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( V );
         if ( ctx->dsolvy_bytes < need ) {
             if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
             PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
@@ -1135,8 +1134,10 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
         }
         V            *Y   = (V *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
-        // levels [0, nlv) run level by level, the others in one launch per sweep
-        const int     nlv = ctx->solve_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
+        // levels [0, nlv) run level by level, the others in one launch per
+        // sweep (passes of at most PXD_SF_NR right-hand sides)
+        const bool    use_sf = ctx->solve_sf && nrhs <= PXD_SF_NR;
+        const int     nlv    = use_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
         const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( V );
@@ -1163,14 +1164,14 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
             }
         }
         int rc = 0;
-        if ( ctx->solve_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
             return rc;
         }
         if ( ctx->facto == FACT_SYTRF ) {
             hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
                                 ctx->dcol2cblk, ctx->dcb, L, Y, ldy, nrhs );
         }
-        if ( ctx->solve_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
             return rc;
         }
         for ( int li = nlv - 1; li >= 0; li-- ) {
@@ -1244,10 +1245,8 @@ set_lds_attributes_solve( size_t lds )
     set_lds_room( (const void *)k_solve_fwd_upd<T, V, 16>, lds );
     set_lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, lds );
     set_lds_room( (const void *)k_solve_diag_chunk<T, V>, lds );
-    set_lds_room( (const void *)k_sf_forward<T, V, 4, 128 / sizeof( T )>, lds );
-    set_lds_room( (const void *)k_sf_backward<T, V, 4>, lds );
-    set_lds_room( (const void *)k_sf_forward<T, V, sf_nr_max<V>::value, 128 / sizeof( T )>, lds );
-    set_lds_room( (const void *)k_sf_backward<T, V, sf_nr_max<V>::value>, lds );
+    set_lds_room( (const void *)k_sf_forward<T, V, PXD_SF_NR, 128 / sizeof( T )>, lds );
+    set_lds_room( (const void *)k_sf_backward<T, V, PXD_SF_NR>, lds );
 }
 
 // the matrix-core kernels (all four arithmetics)

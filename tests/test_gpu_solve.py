@@ -1,7 +1,8 @@
-"""The one-launch solve sweeps (engine option solve_sf = 1, the default)
-against the level-by-level sweeps (solve_sf = 0) and the oracle's solve with
-the same factor, for every factorization and arithmetic, with right-hand
-side counts that take one and several passes.
+"""The one-launch solve sweeps (engine option solve_sf = 1, the default;
+passes of at most 4 right-hand sides) against the level-by-level sweeps
+(solve_sf = 0) and the oracle's solve with the same factor, for every
+factorization and arithmetic, with right-hand side counts that take one
+pass and several (20 RHS: passes of 8, level by level).
 
 Both device solves compute the same sums in different orders (atomic
 accumulation), so they agree to rounding of the factor precision; the
@@ -38,7 +39,7 @@ def _problem(kind, nx):
 
 
 @pytest.mark.parametrize("cut", [0, 3, -1])
-@pytest.mark.parametrize("nrhs", [1, 5, 20])
+@pytest.mark.parametrize("nrhs", [1, 4, 20])
 @pytest.mark.parametrize("kind,facto,mixed,nx", CASES)
 def test_one_launch_solve_matches_level_solve(kind, facto, mixed, nx, nrhs, cut):
     """cut: the first level of the one-launch part (0: every level, 3: the
