@@ -39,6 +39,153 @@ tri_index( int idx, int &i, int &j )
     j = idx - i * ( i + 1 ) / 2;
 }
 
+// Unblocked eliminations of a 32 x 32 diagonal block in LDS (s[col][row],
+// zero beyond bs rows and columns), one barrier per elimination step.  Each
+// of the 256 threads keeps four entries of the block and of each inverse in
+// registers: row r = tid & 31, columns g, g + 8, g + 16, g + 24 (g = tid >>
+// 5).  The column and row that a step reads go through a double-buffered
+// exchange area x[2 * NX][32] in LDS, written by their owners before the
+// step's barrier.  On return the block and the inverses are back in LDS,
+// sD holds the pivots, and the result is the number of replaced pivots
+// (counted by thread 0 only; the other threads return 0).
+static_assert( PXD_THREADS == 8 * PXD_NB, "the elimination maps 256 threads on a 32 x 32 block" );
+
+template <typename T>
+__device__ inline T
+pick4( T v0, T v1, T v2, T v3, int u )
+{
+    return ( u == 0 ) ? v0 : ( u == 1 ) ? v1 : ( u == 2 ) ? v2 : v3;
+}
+
+// Symmetric: A = U D U^T, U unit lower with multipliers u_rc = a_rc / d_c,
+// on the lower triangle of sA; the same row operations on W = I (sI) leave
+// W = U^{-1}.  A pivot below tau in modulus becomes tau (Cholesky,
+// core_zpotf2sp) or sign(re d) tau (LDL^T, core_zsytf2sp).
+template <typename T, bool LDLT>
+__device__ inline int
+elim_sym32( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB], int bs,
+            typename px_real<T>::type tau, int tid )
+{
+    typedef typename px_real<T>::type R;
+    const int r = tid & ( PXD_NB - 1 ), g = tid >> 5;
+    T a[4], w[4];
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        a[u] = sA[g + 8 * u][r];
+        w[u] = sI[g + 8 * u][r];
+    }
+    int piv = 0;
+    for ( int cc = 0; cc < bs; cc++ ) {
+        T *colA = x[2 * ( cc & 1 )], *rowW = x[2 * ( cc & 1 ) + 1];
+        if ( g == ( cc & 7 ) ) {
+            colA[r] = pick4( a[0], a[1], a[2], a[3], cc >> 3 );   // column cc of A
+        }
+        if ( r == cc ) {
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                rowW[g + 8 * u] = w[u];      // row cc of W
+            }
+        }
+        __syncthreads();
+        T d = colA[cc];
+        if ( px_abs( d ) < tau ) {
+            d = ( LDLT && px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
+            piv += ( tid == 0 );
+        }
+        const T rinv = (T)1 / d;
+        if ( r > cc && r < bs ) {
+            const T ar = colA[r];
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                const int c = g + 8 * u;
+                if ( c > cc && c <= r ) {
+                    a[u] -= ar * colA[c] * rinv;
+                }
+                else if ( c <= cc ) {
+                    w[u] -= ar * rinv * rowW[c];
+                }
+            }
+        }
+        if ( tid == 0 ) {
+            sD[cc] = d;
+        }
+    }
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        sA[g + 8 * u][r] = a[u];
+        sI[g + 8 * u][r] = w[u];
+    }
+    __syncthreads();
+    return piv;
+}
+
+// LU without pivoting: multipliers a_r,cc / d left unscaled in sA; the row
+// operations of L on W_L = I (sIL) leave L11^{-1}, those of U11^T = U~ D on
+// W_U = I (sIU) leave U~^{-1}.  A pivot below tau becomes sign(re d) tau.
+template <typename T>
+__device__ inline int
+elim_lu32( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB],
+           int bs, typename px_real<T>::type tau, int tid )
+{
+    typedef typename px_real<T>::type R;
+    const int r = tid & ( PXD_NB - 1 ), g = tid >> 5;
+    T a[4], wl[4], wu[4];
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        a[u]  = sA[g + 8 * u][r];
+        wl[u] = sIL[g + 8 * u][r];
+        wu[u] = sIU[g + 8 * u][r];
+    }
+    int piv = 0;
+    for ( int cc = 0; cc < bs; cc++ ) {
+        T *colA = x[4 * ( cc & 1 )], *rowA = x[4 * ( cc & 1 ) + 1];
+        T *rowL = x[4 * ( cc & 1 ) + 2], *rowU = x[4 * ( cc & 1 ) + 3];
+        if ( g == ( cc & 7 ) ) {
+            colA[r] = pick4( a[0], a[1], a[2], a[3], cc >> 3 );
+        }
+        if ( r == cc ) {
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                rowA[g + 8 * u] = a[u];
+                rowL[g + 8 * u] = wl[u];
+                rowU[g + 8 * u] = wu[u];
+            }
+        }
+        __syncthreads();
+        T d = colA[cc];
+        if ( px_abs( d ) < tau ) {
+            d = ( px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
+            piv += ( tid == 0 );
+        }
+        const T rinv = (T)1 / d;
+        if ( r > cc && r < bs ) {
+            const T cr = colA[r], rr = rowA[r];
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                const int c = g + 8 * u;
+                if ( c > cc && c < bs ) {
+                    a[u] -= cr * rinv * rowA[c];
+                }
+                else if ( c <= cc ) {
+                    wl[u] -= cr * rinv * rowL[c];
+                    wu[u] -= rr * rinv * rowU[c];
+                }
+            }
+        }
+        if ( tid == 0 ) {
+            sD[cc] = d;
+        }
+    }
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        sA[g + 8 * u][r]  = a[u];
+        sIL[g + 8 * u][r] = wl[u];
+        sIU[g + 8 * u][r] = wu[u];
+    }
+    __syncthreads();
+    return piv;
+}
+
 // Per-workgroup timing for the kernel timing log: when stamps is not null,
 // task i of the launch writes its start and end times (constant-rate wall
 // clock ticks) to stamps[2i] and stamps[2i+1].
@@ -111,6 +258,7 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     __shared__ T sI[PXD_NB][PXD_NB + 1];   // inverse, sI[col][row]
     __shared__ T sD[PXD_NB];
     __shared__ T sDi[PXD_NB];               // LDL^T: 1 / D of the block, 0 beyond it
+    __shared__ T sX[4][PXD_NB];             // exchange area of the elimination
     // L21 of the current block, sP[row][k]: dynamic LDS for the widest cblk of the launch
     extern __shared__ unsigned char smem_raw[];
     T ( *sP )[PXD_NB + 1] = reinterpret_cast<T ( * )[PXD_NB + 1]>( smem_raw );
@@ -147,45 +295,15 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         }
         __syncthreads();
 
-        // unblocked elimination A = U D U^T (U unit lower, multipliers
-        // u_rc = a_rc / d_c).  The same row operations applied to W = I leave
-        // W = U^{-1} (sI[col][row]): step cc subtracts u_r,cc * W[cc][:] from
-        // every row r > cc, in the same barrier step as the trailing update.
-        for ( int cc = 0; cc < bs; cc++ ) {
-            T d = sA[cc][cc];
-            if ( px_abs( d ) < tau ) {
-                // Cholesky: tau (core_zpotf2sp); LDL^T: sign(re d) tau (core_zsytf2sp)
-                d = ( LDLT && px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
-                if ( tid == 0 ) {
-                    piv++;
-                }
-            }
-            const int m   = bs - cc - 1;
-            const int nsq = m * ( m + 1 ) / 2;   // lower triangle of the trailing block
-            const T   r   = (T)1 / d;
-            for ( int idx = tid; idx < nsq + m * ( cc + 1 ); idx += PXD_THREADS ) {
-                if ( idx < nsq ) {
-                    int i, j;
-                    tri_index( idx, i, j );
-                    const int rr = cc + 1 + i, c2 = cc + 1 + j;
-                    sA[c2][rr] -= sA[cc][rr] * sA[cc][c2] * r;
-                }
-                else {
-                    const int k = idx - nsq, rr = cc + 1 + k % m, j = k / m;
-                    sI[j][rr] -= sA[cc][rr] * r * sI[j][cc];
-                }
-            }
-            if ( tid == 0 ) {
-                sD[cc] = d;
-            }
-            __syncthreads();
-        }
+        // unblocked elimination A = U D U^T, W = U^{-1} alongside
+        piv += elim_sym32<T, LDLT>( sA, sI, sD, sX, bs, tau, tid );
         // Cholesky: L = U D^{1/2}, the columns scaled by 1/sqrt(pivot), and
         // L^{-1} = D^{-1/2} U^{-1}, the rows of W scaled likewise.  LDL^T:
         // L = U (column / d), L^{-1} = W, the diagonal holds D and the upper
         // part D L^T (the unscaled columns).
-        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-            int rr = idx % bs, cc = idx / bs;
+        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+            const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+            if ( rr >= bs || cc >= bs ) continue;
             if ( rr > cc ) {
                 const T a = sA[cc][rr];
                 if constexpr ( LDLT ) {
@@ -216,8 +334,9 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         }
         __syncthreads();
 
-        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-            int rr = idx % bs, cc = idx / bs;
+        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+            const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+            if ( rr >= bs || cc >= bs ) continue;
             if ( LDLT || rr >= cc ) {
                 A[( kb + rr ) + (int64_t)( kb + cc ) * n] = sA[cc][rr];
             }
@@ -556,43 +675,18 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
         }
     }
 
-    // T(p,p): elimination with the inverse built alongside (as in k_potrf2)
+    // T(p,p): elimination with the inverse built alongside (as in k_potrf2);
+    // sO is free now and serves as the exchange area
     T( *sA )[PXD_NB + 1] = sT[p];
     for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
         const int r = idx % PXD_NB, cc = idx / PXD_NB;
         sI[cc][r] = ( r == cc && r < bs ) ? (T)1 : (T)0;
     }
     __syncthreads();
-    for ( int cc = 0; cc < bs; cc++ ) {
-        T d = sA[cc][cc];
-        if ( px_abs( d ) < tau ) {
-            d = (T)tau;   // core_zpotf2sp
-            if ( tid == 0 ) {
-                piv++;
-            }
-        }
-        const int m   = bs - cc - 1;
-        const int nsq = m * ( m + 1 ) / 2;
-        const T   rr_ = (T)1 / d;
-        for ( int idx = tid; idx < nsq + m * ( cc + 1 ); idx += PXD_THREADS ) {
-            if ( idx < nsq ) {
-                int i, j;
-                tri_index( idx, i, j );
-                const int rr = cc + 1 + i, c2 = cc + 1 + j;
-                sA[c2][rr] -= sA[cc][rr] * sA[cc][c2] * rr_;
-            }
-            else {
-                const int kk = idx - nsq, rr = cc + 1 + kk % m, j = kk / m;
-                sI[j][rr] -= sA[cc][rr] * rr_ * sI[j][cc];
-            }
-        }
-        if ( tid == 0 ) {
-            sD[cc] = d;
-        }
-        __syncthreads();
-    }
-    for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-        const int rr = idx % bs, cc = idx / bs;
+    piv += elim_sym32<T, false>( sA, sI, sD, reinterpret_cast<T ( * )[PXD_NB]>( &sO[0][0] ), bs, tau, tid );
+    for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+        const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+        if ( rr >= bs || cc >= bs ) continue;
         if ( rr > cc ) {
             sA[cc][rr] = sA[cc][rr] / px_sqrt( sD[cc] );
         }
@@ -605,8 +699,9 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
         sA[tid][tid] = px_sqrt( sD[tid] );
     }
     __syncthreads();
-    for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-        const int rr = idx % bs, cc = idx / bs;
+    for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+        const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+        if ( rr >= bs || cc >= bs ) continue;
         if ( rr >= cc ) {
             A[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sA[cc][rr];
         }
@@ -679,6 +774,7 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
     __shared__ T sIL[PXD_NB][PXD_NB + 1];   // L11^{-1}, [col][row]
     __shared__ T sIU[PXD_NB][PXD_NB + 1];   // (U11^T)^{-1}, [col][row]
     __shared__ T sD[PXD_NB];
+    __shared__ T sX[8][PXD_NB];              // exchange area of the elimination
     __shared__ T sPi[PXD_NB][64 + 1];
     __shared__ T sPj[PXD_NB][64 + 1];
 
@@ -738,42 +834,12 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         __syncthreads();
 
         // unblocked right-looking LU, the multipliers left unscaled until the
-        // end.  The row operations of L (multipliers a_r,cc / d) applied to
-        // W_L = I leave W_L = L11^{-1}; those of U11^T = U~ D (multipliers
-        // a_cc,j / d) applied to W_U = I leave W_U = U~^{-1}, so that
-        // (U11^T)^{-1} = D^{-1} W_U.  Both run in the step's barrier interval.
-        for ( int cc = 0; cc < bs; cc++ ) {
-            T d = sA[cc][cc];
-            if ( px_abs( d ) < tau ) {
-                d = ( px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
-                if ( tid == 0 ) {
-                    piv++;
-                }
-            }
-            const int m   = bs - cc - 1;
-            const int nsq = m * m, nw = m * ( cc + 1 );
-            const T   r   = (T)1 / d;
-            for ( int idx = tid; idx < nsq + 2 * nw; idx += PXD_THREADS ) {
-                if ( idx < nsq ) {
-                    const int rr = cc + 1 + idx % m, c2 = cc + 1 + idx / m;
-                    sA[c2][rr] -= sA[cc][rr] * r * sA[c2][cc];
-                }
-                else if ( idx < nsq + nw ) {
-                    const int k = idx - nsq, rr = cc + 1 + k % m, j = k / m;
-                    sIL[j][rr] -= sA[cc][rr] * r * sIL[j][cc];
-                }
-                else {
-                    const int k = idx - nsq - nw, rr = cc + 1 + k % m, j = k / m;
-                    sIU[j][rr] -= sA[rr][cc] * r * sIU[j][cc];
-                }
-            }
-            if ( tid == 0 ) {
-                sD[cc] = d;
-            }
-            __syncthreads();
-        }
-        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-            const int rr = idx % bs, cc = idx / bs;
+        // end; W_L = L11^{-1} and W_U = U~^{-1} (U11^T = U~ D, so that
+        // (U11^T)^{-1} = D^{-1} W_U) built alongside
+        piv += elim_lu32<T>( sA, sIL, sIU, sD, sX, bs, tau, tid );
+        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+            const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+            if ( rr >= bs || cc >= bs ) continue;
             if ( rr > cc ) {
                 sA[cc][rr] = sA[cc][rr] / sD[cc];
             }
@@ -784,16 +850,18 @@ k_getrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
         __syncthreads();
 
         // (U11^T)^{-1} = D^{-1} W_U: rows scaled by 1/d (W_L is L11^{-1} as is)
-        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-            const int rr = idx % bs, cc = idx / bs;
+        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+            const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+            if ( rr >= bs || cc >= bs ) continue;
             if ( rr >= cc ) {
                 sIU[cc][rr] = sIU[cc][rr] / sD[rr];
             }
         }
         __syncthreads();
 
-        for ( int idx = tid; idx < bs * bs; idx += PXD_THREADS ) {
-            const int rr = idx % bs, cc = idx / bs;
+        for ( int idx = tid; idx < PXD_NB * PXD_NB; idx += PXD_THREADS ) {
+            const int rr = idx % PXD_NB, cc = idx / PXD_NB;
+            if ( rr >= bs || cc >= bs ) continue;
             A[( kb + rr ) + (int64_t)( kb + cc ) * n]  = sA[cc][rr];
             IL[( kb + rr ) + (int64_t)( kb + cc ) * n] = sIL[cc][rr];
             IU[( kb + rr ) + (int64_t)( kb + cc ) * n] = sIU[cc][rr];
