@@ -87,24 +87,29 @@ elim_sym32( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB
             }
         }
         __syncthreads();
+        // every read of the step at once, then the updates as selects: one
+        // LDS round trip per step
         T d = colA[cc];
+        const T ar = colA[r];
+        T ca[4], rw[4];
+#pragma unroll
+        for ( int u = 0; u < 4; u++ ) {
+            ca[u] = colA[g + 8 * u];
+            rw[u] = rowW[g + 8 * u];
+        }
         if ( px_abs( d ) < tau ) {
             d = ( LDLT && px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
             piv += ( tid == 0 );
         }
-        const T rinv = (T)1 / d;
-        if ( r > cc && r < bs ) {
-            const T ar = colA[r];
+        const T    rinv = (T)1 / d;
+        const bool act  = r > cc && r < bs;
 #pragma unroll
-            for ( int u = 0; u < 4; u++ ) {
-                const int c = g + 8 * u;
-                if ( c > cc && c <= r ) {
-                    a[u] -= ar * colA[c] * rinv;
-                }
-                else if ( c <= cc ) {
-                    w[u] -= ar * rinv * rowW[c];
-                }
-            }
+        for ( int u = 0; u < 4; u++ ) {
+            const int c  = g + 8 * u;
+            const T   na = a[u] - ar * ca[u] * rinv;
+            const T   nw = w[u] - ar * rinv * rw[u];
+            a[u] = ( act && c > cc && c <= r ) ? na : a[u];
+            w[u] = ( act && c <= cc ) ? nw : w[u];
         }
         if ( tid == 0 ) {
             sD[cc] = d;
@@ -153,24 +158,29 @@ elim_lu32( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1]
         }
         __syncthreads();
         T d = colA[cc];
+        const T cr = colA[r], rr = rowA[r];
+        T ra[4], rl[4], ru[4];
+#pragma unroll
+        for ( int u = 0; u < 4; u++ ) {
+            ra[u] = rowA[g + 8 * u];
+            rl[u] = rowL[g + 8 * u];
+            ru[u] = rowU[g + 8 * u];
+        }
         if ( px_abs( d ) < tau ) {
             d = ( px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
             piv += ( tid == 0 );
         }
-        const T rinv = (T)1 / d;
-        if ( r > cc && r < bs ) {
-            const T cr = colA[r], rr = rowA[r];
+        const T    rinv = (T)1 / d;
+        const bool act  = r > cc && r < bs;
 #pragma unroll
-            for ( int u = 0; u < 4; u++ ) {
-                const int c = g + 8 * u;
-                if ( c > cc && c < bs ) {
-                    a[u] -= cr * rinv * rowA[c];
-                }
-                else if ( c <= cc ) {
-                    wl[u] -= cr * rinv * rowL[c];
-                    wu[u] -= rr * rinv * rowU[c];
-                }
-            }
+        for ( int u = 0; u < 4; u++ ) {
+            const int c  = g + 8 * u;
+            const T   na = a[u] - cr * rinv * ra[u];
+            const T   nl = wl[u] - cr * rinv * rl[u];
+            const T   nu = wu[u] - rr * rinv * ru[u];
+            a[u]  = ( act && c > cc && c < bs ) ? na : a[u];
+            wl[u] = ( act && c <= cc ) ? nl : wl[u];
+            wu[u] = ( act && c <= cc ) ? nu : wu[u];
         }
         if ( tid == 0 ) {
             sD[cc] = d;
