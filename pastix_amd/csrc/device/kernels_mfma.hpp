@@ -57,13 +57,73 @@ pick4( T v0, T v1, T v2, T v3, int u )
     return ( u == 0 ) ? v0 : ( u == 1 ) ? v1 : ( u == 2 ) ? v2 : v3;
 }
 
+// Complex double: each LDS read of a step as it is needed (the read-ahead
+// form below would not fit the register file of the panel kernels).
+template <typename T, bool LDLT>
+__device__ inline int
+elim_sym32_single( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB], int bs,
+            typename px_real<T>::type tau, int tid )
+{
+    typedef typename px_real<T>::type R;
+    const int r = tid & ( PXD_NB - 1 ), g = tid >> 5;
+    T a[4], w[4];
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        a[u] = sA[g + 8 * u][r];
+        w[u] = sI[g + 8 * u][r];
+    }
+    int piv = 0;
+    for ( int cc = 0; cc < bs; cc++ ) {
+        T *colA = x[2 * ( cc & 1 )], *rowW = x[2 * ( cc & 1 ) + 1];
+        if ( g == ( cc & 7 ) ) {
+            colA[r] = pick4( a[0], a[1], a[2], a[3], cc >> 3 );   // column cc of A
+        }
+        if ( r == cc ) {
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                rowW[g + 8 * u] = w[u];      // row cc of W
+            }
+        }
+        __syncthreads();
+        T d = colA[cc];
+        if ( px_abs( d ) < tau ) {
+            d = ( LDLT && px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
+            piv += ( tid == 0 );
+        }
+        const T rinv = (T)1 / d;
+        if ( r > cc && r < bs ) {
+            const T ar = colA[r];
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                const int c = g + 8 * u;
+                if ( c > cc && c <= r ) {
+                    a[u] -= ar * colA[c] * rinv;
+                }
+                else if ( c <= cc ) {
+                    w[u] -= ar * rinv * rowW[c];
+                }
+            }
+        }
+        if ( tid == 0 ) {
+            sD[cc] = d;
+        }
+    }
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        sA[g + 8 * u][r] = a[u];
+        sI[g + 8 * u][r] = w[u];
+    }
+    __syncthreads();
+    return piv;
+}
+
 // Symmetric: A = U D U^T, U unit lower with multipliers u_rc = a_rc / d_c,
 // on the lower triangle of sA; the same row operations on W = I (sI) leave
 // W = U^{-1}.  A pivot below tau in modulus becomes tau (Cholesky,
 // core_zpotf2sp) or sign(re d) tau (LDL^T, core_zsytf2sp).
 template <typename T, bool LDLT>
 __device__ inline int
-elim_sym32( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB], int bs,
+elim_sym32_ahead( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB], int bs,
             typename px_real<T>::type tau, int tid )
 {
     typedef typename px_real<T>::type R;
@@ -124,12 +184,25 @@ elim_sym32( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB
     return piv;
 }
 
+template <typename T, bool LDLT>
+__device__ inline int
+elim_sym32( T ( *sA )[PXD_NB + 1], T ( *sI )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB], int bs,
+            typename px_real<T>::type tau, int tid )
+{
+    if constexpr ( sizeof( T ) > 8 ) {
+        return elim_sym32_single<T, LDLT>( sA, sI, sD, x, bs, tau, tid );
+    }
+    else {
+        return elim_sym32_ahead<T, LDLT>( sA, sI, sD, x, bs, tau, tid );
+    }
+}
+
 // LU without pivoting: multipliers a_r,cc / d left unscaled in sA; the row
 // operations of L on W_L = I (sIL) leave L11^{-1}, those of U11^T = U~ D on
 // W_U = I (sIU) leave U~^{-1}.  A pivot below tau becomes sign(re d) tau.
 template <typename T>
 __device__ inline int
-elim_lu32( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB],
+elim_lu32_ahead( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB],
            int bs, typename px_real<T>::type tau, int tid )
 {
     typedef typename px_real<T>::type R;
@@ -194,6 +267,84 @@ elim_lu32( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1]
     }
     __syncthreads();
     return piv;
+}
+
+// complex double: each LDS read as it is needed (register file)
+template <typename T>
+__device__ inline int
+elim_lu32_lean( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB],
+           int bs, typename px_real<T>::type tau, int tid )
+{
+    typedef typename px_real<T>::type R;
+    const int r = tid & ( PXD_NB - 1 ), g = tid >> 5;
+    T a[4], wl[4], wu[4];
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        a[u]  = sA[g + 8 * u][r];
+        wl[u] = sIL[g + 8 * u][r];
+        wu[u] = sIU[g + 8 * u][r];
+    }
+    int piv = 0;
+    for ( int cc = 0; cc < bs; cc++ ) {
+        T *colA = x[4 * ( cc & 1 )], *rowA = x[4 * ( cc & 1 ) + 1];
+        T *rowL = x[4 * ( cc & 1 ) + 2], *rowU = x[4 * ( cc & 1 ) + 3];
+        if ( g == ( cc & 7 ) ) {
+            colA[r] = pick4( a[0], a[1], a[2], a[3], cc >> 3 );
+        }
+        if ( r == cc ) {
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                rowA[g + 8 * u] = a[u];
+                rowL[g + 8 * u] = wl[u];
+                rowU[g + 8 * u] = wu[u];
+            }
+        }
+        __syncthreads();
+        T d = colA[cc];
+        if ( px_abs( d ) < tau ) {
+            d = ( px_re( d ) < (R)0 ) ? (T)-tau : (T)tau;
+            piv += ( tid == 0 );
+        }
+        const T rinv = (T)1 / d;
+        if ( r > cc && r < bs ) {
+            const T cr = colA[r], rr = rowA[r];
+#pragma unroll
+            for ( int u = 0; u < 4; u++ ) {
+                const int c = g + 8 * u;
+                if ( c > cc && c < bs ) {
+                    a[u] -= cr * rinv * rowA[c];
+                }
+                else if ( c <= cc ) {
+                    wl[u] -= cr * rinv * rowL[c];
+                    wu[u] -= rr * rinv * rowU[c];
+                }
+            }
+        }
+        if ( tid == 0 ) {
+            sD[cc] = d;
+        }
+    }
+#pragma unroll
+    for ( int u = 0; u < 4; u++ ) {
+        sA[g + 8 * u][r]  = a[u];
+        sIL[g + 8 * u][r] = wl[u];
+        sIU[g + 8 * u][r] = wu[u];
+    }
+    __syncthreads();
+    return piv;
+}
+
+template <typename T>
+__device__ inline int
+elim_lu32( T ( *sA )[PXD_NB + 1], T ( *sIL )[PXD_NB + 1], T ( *sIU )[PXD_NB + 1], T *sD, T ( *x )[PXD_NB],
+           int bs, typename px_real<T>::type tau, int tid )
+{
+    if constexpr ( sizeof( T ) > 8 ) {
+        return elim_lu32_lean<T>( sA, sIL, sIU, sD, x, bs, tau, tid );
+    }
+    else {
+        return elim_lu32_ahead<T>( sA, sIL, sIU, sD, x, bs, tau, tid );
+    }
 }
 
 // Per-workgroup timing for the kernel timing log: when stamps is not null,
