@@ -43,9 +43,9 @@ tri_index( int idx, int &i, int &j )
 // zero beyond bs rows and columns), one barrier per elimination step.  Each
 // of the 256 threads keeps four entries of the block and of each inverse in
 // registers: row r = tid & 31, columns g, g + 8, g + 16, g + 24 (g = tid >>
-// 5).  The column and row that a step reads go through a double-buffered
-// exchange area x[2 * NX][32] in LDS, written by their owners before the
-// step's barrier.  On return the block and the inverses are back in LDS,
+// 5).  The column and rows that a step reads go through a double-buffered
+// exchange area x in LDS (4 rows of 32 for the symmetric forms, 8 for LU),
+// written by their owners before the step's barrier.  On return the block and the inverses are back in LDS,
 // sD holds the pivots, and the result is the number of replaced pivots
 // (counted by thread 0 only; the other threads return 0).
 static_assert( PXD_THREADS == 8 * PXD_NB, "the elimination maps 256 threads on a 32 x 32 block" );
