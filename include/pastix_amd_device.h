@@ -98,6 +98,11 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *                   16 columns per batch
  *   "solve_vec_double" 1: a single factor solves double vectors directly;
  *                   0: vectors rounded to single and back (default)
+ *   "trsm_blk"      real Cholesky, N > 0: levels with at most N TRSM slabs
+ *                   of 64 rows do the off-diagonal TRSM by blocked
+ *                   substitution on the 32 x 32 diagonal-block inverses,
+ *                   the full-tile inversion (read by the solve only)
+ *                   launched after it; 0: TRSM with the full-tile inverse
  */
 int  pxd_set_option( pxd_ctx_t *ctx, const char *key, double value );
 

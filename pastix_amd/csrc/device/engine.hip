@@ -147,6 +147,7 @@ struct pxd_ctx_s {
     // lower the first solve's backward error from 4.5e-7 to 2.7e-7 but the
     // refinement still takes two iterations, and the solve is twice as slow.
     int       solve_vec_double = 0;
+    int       trsm_blk = 0;     // real Cholesky: levels with at most this many TRSM slabs use the blocked TRSM
     // level solve launches with at least this many tiles load 16 columns
     // per batch (higher occupancy); smaller ones load every column at once
     int64_t   solve_wide_max = PXD_SOLVE_WIDE_MAX;
@@ -717,6 +718,12 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     // strict upper triangle, transposed, into the U tiles)
     PXD_CHECK( hipMemsetAsync( L, 0, ctx->coefnbr * sizeof( T ), s ) );
     typedef typename px_real<T>::type R;
+    // TRSM by blocks on the panel's block inverses: the full-tile inversion
+    // is needed by the solve only and leaves the chain panel -> TRSM
+    // (levels with at most trsm_blk slabs of 64 rows: the top of the tree,
+    // where the chain is the critical path)
+    const bool blk_ok = ctx->trsm_blk > 0 && ctx->kernels && !lu && !ldlt && std::is_same<T, R>::value;
+    auto       blk    = [&]( const level_t &lv ) { return blk_ok && lv.t1 - lv.t0 <= ctx->trsm_blk; };
     typedef typename std::conditional<std::is_same<T, R>::value, double, pxcd>::type TI;   // matrix values
     const double ovf = (double)std::numeric_limits<R>::max();
     hipLaunchKernelGGL( ( k_fill<T, TI> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
@@ -846,7 +853,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                 }
             }
             launches++;
-            if ( ni ) {
+            if ( ni && !blk( lv ) ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
                                     ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio, rcap );
                 launches++;
@@ -855,6 +862,16 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
             hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
+            launches++;
+        }
+    };
+    // the full-tile inversion when the TRSM works by blocks (launched after it)
+    auto inv = [&]( const level_t &lv, hipStream_t st ) {
+        const unsigned ni = (unsigned)( lv.i1 - lv.i0 );
+        if ( blk( lv ) && ni ) {
+            const size_t trtri_lds = (size_t)std::max( 1, std::min( lv.maxn - PXD_NB, rcap ) ) * lds_row;
+            hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st, ctx->dinvt + lv.i0,
+                                ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), 0, rcap );
             launches++;
         }
     };
@@ -870,6 +887,12 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
                                 ctx->dcb, ctx->dbk, U, (const T *)DI, 0, stp( st_trsm, lv.t0, 1, n_trsm ), ctx->prio );
             launches++;
+        }
+        else if ( blk( lv ) ) {
+            if constexpr ( std::is_same<T, R>::value ) {
+                hipLaunchKernelGGL( ( k_trsm_blk<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
+                                    ctx->dcb, ctx->dbk, L, (const T *)DI, stp( st_trsm, lv.t0, 0, n_trsm ), ctx->prio );
+            }
         }
         else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
@@ -931,6 +954,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         for ( const level_t &lv : ctx->levels ) {
             panel( lv, s );
             trsm( lv, s );
+            inv( lv, s );
             for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
                 const upd_group &gr = ctx->groups[g];
                 if ( gr.big == 2 ) {
@@ -955,6 +979,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             t_panel += 1e-3 * ev_ms( ea, eb );
             PXD_CHECK( hipEventRecord( ea, s ) );
             trsm( lv, s );
+            inv( lv, s );
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_trsm += 1e-3 * ev_ms( ea, eb );
@@ -979,11 +1004,15 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             panel( lv, s2 );
             trsm( lv, s2 );
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
+            inv( lv, s2 );   // beside the updates of the level: only the solve reads it
             PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
             update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, su, ctx->prio );
             PXD_CHECK( hipEventRecord( ready, su ) );
             update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, su, 0 );
         }
+        // the last inversions ran on s2 after the last handoff
+        PXD_CHECK( hipEventRecord( paneldone, s2 ) );
+        PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
     }
     PXD_CHECK( hipGetLastError() );
     PXD_CHECK( hipEventRecord( e1, s ) );
@@ -1456,6 +1485,10 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     }
     if ( k == "solve_vec_double" ) {
         ctx->solve_vec_double = ( value != 0. );
+        return 0;
+    }
+    if ( k == "trsm_blk" ) {
+        ctx->trsm_blk = (int)value;
         return 0;
     }
     if ( k == "solve_wide_max" ) {

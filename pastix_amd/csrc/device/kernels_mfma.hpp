@@ -1632,6 +1632,129 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     stamp_end( stamps, ti, t_stamp );
 }
 
+// Off-diagonal TRSM of a real Cholesky cblk by blocked substitution on the
+// panel's 32 x 32 diagonal-block inverses, without the inverse of the whole
+// diagonal tile (engine option trsm_blk): for each 32-column block k,
+// Y = B_k - sum_{j<k} X_j L_kj^T and X_k = Y L_kk^{-T}, which is
+// X = B L_kk^{-T} of core_ztrsmsp.c:148-183 by blocks.  Wave w owns rows
+// 16w..16w+15 of the 64-row slab; X_j is stored in place and read back by
+// the later blocks.  The L_kj tiles of JB blocks j are read together.
+template <typename T>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_trsm_blk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const dblok *__restrict__ bk,
+            T *__restrict__ L, const T *__restrict__ Linv, uint64_t *__restrict__ stamps, int prio )
+{
+    typedef mfma16<T>         M;
+    typedef typename M::acc_t acc_t;
+    constexpr int JB = ( sizeof( T ) <= 4 ) ? 4 : 2;
+    __shared__ int64_t sBase[64];
+    __shared__ int32_t sLd[64];
+    __shared__ T       sY[4][16][PXD_NB + 1];   // per wave: Y, [row][col]
+
+    const int32_t  ti   = xcd_task( blockIdx.x, gridDim.x );
+    const row_task t    = tasks[ti];
+    const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
+    const dcblk   c    = cb[t.cblk];
+    const int32_t n    = c.lcolnum - c.fcolnum + 1;
+    const int32_t lb   = cb[t.cblk + 1].fblok;
+    const int     tid  = threadIdx.x;
+    const int     lane = tid & 63, w = tid >> 6;
+    const int     lr   = lane & 15, lg = lane >> 4;
+    const T      *A    = L + c.coefoff;        // diagonal tile, n x n, ld n
+    const T      *LI   = Linv + c.dinvoff;
+
+    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
+    stage_row_keys( skeys, bk, c.fblok, lb );
+    __syncthreads();
+    if ( tid < 64 ) {
+        int32_t s = t.r0 + tid;
+        if ( s < c.stride ) {
+            int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
+            dblok   bb = bk[b];
+            sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
+            sLd[tid]   = bb.lrownum - bb.frownum + 1;
+        }
+        else {
+            sBase[tid] = -1;
+            sLd[tid]   = 0;
+        }
+    }
+    __syncthreads();
+
+    // operand row of this lane, and the four accumulator rows
+    const int64_t abase = sBase[16 * w + lr];
+    const int32_t ald   = sLd[16 * w + lr];
+    int64_t       cbase[4];
+    int32_t       cld[4];
+#pragma unroll
+    for ( int v = 0; v < 4; v++ ) {
+        cbase[v] = sBase[16 * w + M::row( lane, v )];
+        cld[v]   = sLd[16 * w + M::row( lane, v )];
+    }
+
+    for ( int32_t k0 = 0; k0 < n; k0 += PXD_NB ) {
+        const int32_t bkk = min( PXD_NB, n - k0 );
+        const int32_t c0 = k0 + lr, c1 = k0 + 16 + lr;   // columns of the two 16-column halves
+        acc_t         acc[2];
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) {
+            M::set( acc[0], v, ( cbase[v] >= 0 && c0 < n ) ? L[cbase[v] + (int64_t)c0 * cld[v]] : (T)0 );
+            M::set( acc[1], v, ( cbase[v] >= 0 && c1 < n ) ? L[cbase[v] + (int64_t)c1 * cld[v]] : (T)0 );
+        }
+        // Y -= X_j L_kj^T; the blocks j < k are whole (32 columns)
+        for ( int32_t j0 = 0; j0 < k0; j0 += JB * PXD_NB ) {
+            T a[JB][PXD_NB / 4], b0[JB][PXD_NB / 4], b1[JB][PXD_NB / 4];
+#pragma unroll
+            for ( int jj = 0; jj < JB; jj++ ) {
+                const int32_t jc = j0 + jj * PXD_NB;
+                const bool    ok = jc < k0;
+#pragma unroll
+                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                    const int32_t l = jc + 4 * s4 + lg;
+                    a[jj][s4]  = ( ok && abase >= 0 ) ? -L[abase + (int64_t)l * ald] : (T)0;
+                    b0[jj][s4] = ( ok && c0 < n ) ? A[c0 + (int64_t)l * n] : (T)0;
+                    b1[jj][s4] = ( ok && c1 < n ) ? A[c1 + (int64_t)l * n] : (T)0;
+                }
+            }
+#pragma unroll
+            for ( int jj = 0; jj < JB; jj++ ) {
+                if ( j0 + jj * PXD_NB < k0 ) {
+#pragma unroll
+                    for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+                        acc[0] = M::mma( a[jj][s4], b0[jj][s4], acc[0] );
+                        acc[1] = M::mma( a[jj][s4], b1[jj][s4], acc[1] );
+                    }
+                }
+            }
+        }
+        // X_k = Y L_kk^{-T}, Y through this wave's LDS rows
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) {
+            sY[w][M::row( lane, v )][lr]      = M::get( acc[0], v );
+            sY[w][M::row( lane, v )][16 + lr] = M::get( acc[1], v );
+        }
+        __syncthreads();
+        acc_t x0 = M::zero(), x1 = M::zero();
+#pragma unroll
+        for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
+            const int l  = 4 * s4 + lg;
+            const T   ya = sY[w][lr][l];
+            const T   i0 = ( c0 < n && l < bkk ) ? LI[c0 + (int64_t)( k0 + l ) * n] : (T)0;
+            const T   i1 = ( c1 < n && l < bkk ) ? LI[c1 + (int64_t)( k0 + l ) * n] : (T)0;
+            x0 = M::mma( ya, i0, x0 );
+            x1 = M::mma( ya, i1, x1 );
+        }
+#pragma unroll
+        for ( int v = 0; v < 4; v++ ) {
+            if ( cbase[v] >= 0 && c0 < n ) L[cbase[v] + (int64_t)c0 * cld[v]] = M::get( x0, v );
+            if ( cbase[v] >= 0 && c1 < n ) L[cbase[v] + (int64_t)c1 * cld[v]] = M::get( x1, v );
+        }
+        __syncthreads();   // X_k stored (read back by the next blocks) and sY free
+    }
+    stamp_end( stamps, ti, t_stamp );
+}
+
 // ---------------------------------------------------------------------------
 // Supernodal update on the matrix cores: C[f(i)] -= A_i * A_b^T for one
 // 64 x 64 output tile (four waves of 32 x 32), fused row remap into the

@@ -165,3 +165,35 @@ def test_tall_tile_stage_depth_matches_oracle(dtype):
     F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype)
     assert cblk_diff(s, L, F.L, dtype) <= 10.0
     px.finalize(pd)
+
+
+@pytest.mark.parametrize("timing", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx", [5, 12, 20, 28])
+def test_blocked_trsm_matches_oracle(nx, dtype, timing):
+    """TRSM by blocked substitution on the panel's 32 x 32 block inverses
+    (option trsm_blk = 10**6: on every level), the full-tile inversion
+    launched after it: the
+    factor matches the oracle tile by tile (28^3 has 320-column cblks of 10
+    blocks), and the solve, which reads the full-tile inverses, reaches the
+    same backward error as with the default TRSM."""
+    mixed = int(dtype == np.float32)
+    spm = make_problem("lap7", nx)
+    res = {}
+    for blk in (1, 0):
+        pd, iparm, dparm = analyzed(spm, nx, mixed=mixed)
+        px.set_option(pd, "trsm_blk", 10 ** 6 * blk)
+        px.set_option(pd, "timing", timing)
+        px.task_numfact(pd, spm)
+        s = px.solver_export(pd)
+        L = px.get_coeftab(pd, s["coefnbr"], dtype)
+        if blk:
+            F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype, nthreads=8)
+            assert cblk_diff(s, L, F.L, dtype) <= 10.0
+        x0, b = spm.gen_rhs(3, seed=2)
+        x = np.array(b, order="F", copy=True)
+        px.task_solve(pd, spm, x)
+        res[blk] = spm.backward_error(b, x)[0]
+        px.finalize(pd)
+    eps = np.finfo(dtype).eps
+    assert res[1] <= 10 * max(res[0], eps)
