@@ -150,6 +150,40 @@ int pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes );
  */
 int pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx );
 
+/*
+ * x <- op(A)^{-1} x with op(A) = A (trans 0) or A^T (trans 1): for an LU
+ * factor U^T y = b then L^T x = y (reference sopalin/pastix_task_solve.c:
+ * 510-677, the transA cases of its table); Cholesky and LDL^T factors are
+ * symmetric and solve A^T as A.
+ */
+int pxd_solve_trans( pxd_ctx_t *ctx, int trans, int64_t nrhs, void *dx, int64_t ldx );
+
+/*
+ * One step of the solve, in place on nrhs device vectors (vector precision),
+ * as pastix_subtask_trsm / pastix_subtask_diag apply it (reference
+ * sopalin/pastix_task_solve.c:296-360,417-470 -> sopalin_ztrsm,
+ * sopalin/sequential_ztrsm.c:591-652, and sopalin_zdiag):
+ *   PXD_STEP_L_FWD   L y = b       (unit for LU and LDL^T, non-unit for Cholesky)
+ *   PXD_STEP_LT_BWD  L^T x = y
+ *   PXD_STEP_U_BWD   U x = y       (LU only)
+ *   PXD_STEP_UT_FWD  U^T y = b     (LU only)
+ *   PXD_STEP_DIAG    x <- D^{-1} x, D the diagonal of the L-side diagonal
+ *                    tiles (the D of LDL^T; solve_cblk_zdiag,
+ *                    kernels/solve_ztrsmsp.c:629-679, divides by it for any
+ *                    factorization)
+ * A step that does not exist for the factorization returns 7.
+ */
+enum { PXD_STEP_L_FWD = 0, PXD_STEP_LT_BWD = 1, PXD_STEP_U_BWD = 2, PXD_STEP_UT_FWD = 3, PXD_STEP_DIAG = 4 };
+int pxd_solve_step( pxd_ctx_t *ctx, int step, int64_t nrhs, void *dx, int64_t ldx );
+
+/*
+ * dd[i] (device, n entries in the vector precision: double or complex
+ * double) = diagonal entry i of the factor's diagonal tiles in the permuted
+ * numbering: L_ii (Cholesky), D_i (LDL^T), U_ii (LU) (reference
+ * coeftab_zgetdiag, sopalin/coeftab_z.c:659-689, behind pastixGetDiag).
+ */
+int pxd_get_diag( pxd_ctx_t *ctx, void *dd );
+
 /* y <- alpha A x + beta y (nrhs columns). alpha/beta are real. */
 int pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy,
               int64_t nrhs, int64_t ld );

@@ -50,6 +50,19 @@ typedef enum spm_normtype_e {
     SpmMaxNorm       = 177
 } spm_normtype_t;
 
+/* Matrix input drivers (the SPM numbering that PaStiX's old_api.h aliases,
+ * reference include/pastix/old_api.h:142-149). */
+typedef enum spm_driver_e {
+    SpmDriverRSA,
+    SpmDriverHB,
+    SpmDriverIJV,
+    SpmDriverMM,
+    SpmDriverSPM,
+    SpmDriverLaplacian,
+    SpmDriverXLaplacian,
+    SpmDriverGraph
+} spm_driver_t;
+
 typedef enum spm_rhstype_e {
     SpmRhsOne,
     SpmRhsI,
@@ -111,20 +124,37 @@ int spmGenRHSSeed( spm_rhstype_t type, spm_int_t nrhs, const spmatrix_t *spm,
                    void *x, spm_int_t ldx, void *b, spm_int_t ldb );
 
 /*
- * Backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) per
- * right-hand side, its maximum returned in *berr_max.  Returns 0 when the
- * maximum is below eps (eps<0 selects 1e-12 for double, 1e-5 for single).
+ * Check of a solution (the call of example/simple.c:140): returns 0 when the
+ * backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) of every
+ * right-hand side is at most eps (eps < 0 selects 1e-12) and, when x0 is
+ * given, the forward error ||x - x0||_inf / ||x0||_inf is at most 1e4 * eps;
+ * 1 otherwise (a NaN fails).
  */
 int spmCheckAxb( double eps, spm_int_t nrhs, const spmatrix_t *spm,
                  const void *x0, spm_int_t ldx0,
                  const void *b, spm_int_t ldb,
                  const void *x, spm_int_t ldx );
+/* max over the right-hand sides of ||x - x0||_inf / ||x0||_inf */
+double spmForwardError( spm_int_t nrhs, const spmatrix_t *spm, const void *x0, spm_int_t ldx0,
+                        const void *x, spm_int_t ldx );
 double spmBackwardError( spm_int_t nrhs, const spmatrix_t *spm,
                          const void *b, spm_int_t ldb,
                          const void *x, spm_int_t ldx, double *berr_each );
 
 /* Uniform values in (-0.5, 0.5] from the reference's 64-bit LCG (plrnt). */
 void spm_plrnt( spm_int_t n, unsigned long long seed, unsigned long long shift, double *out );
+
+/*
+ * Read a matrix (example/simple.c:63-66): SpmDriverHB and SpmDriverRSA
+ * (Harwell-Boeing files, assembled real / complex / pattern), SpmDriverMM
+ * (Matrix Market coordinate), SpmDriverIJV ("n nnz" then "i j v" lines) and
+ * SpmDriverLaplacian (the string "[<type>:]nx[:ny[:nz[:alpha[:beta]]]]": a
+ * 7-point Laplacian with diagonal alpha * 2 * dims and off-diagonals -beta).
+ * The result is CSC, 0-based, rows sorted, duplicates summed; symmetric and
+ * Hermitian files keep the triangle they store.  Other drivers return
+ * PASTIX_ERR_NOTIMPLEMENTED (3), unreadable files PASTIX_ERR_FILE (8).
+ */
+int spmReadDriver( spm_driver_t driver, const char *filename, spmatrix_t *spm );
 
 /*
  * Grid generators (x-fastest numbering i + nx*j + nx*ny*k), Dirichlet boundary:

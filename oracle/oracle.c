@@ -995,6 +995,20 @@ orc_spmv( int64_t n, const int64_t *colptr, const int64_t *rowtab, const double 
     return 0;
 }
 
+/* the same for complex double values (complex symmetric or Hermitian full
+ * pattern, or the transposed CSC of a general matrix) */
+int
+orc_spmv_z( int64_t n, const int64_t *colptr, const int64_t *rowtab, const double complex *val, double alpha,
+            const double complex *x, double beta, double complex *y )
+{
+    for ( int64_t j = 0; j < n; j++ ) {
+        double complex s = 0.;
+        for ( int64_t e = colptr[j]; e < colptr[j + 1]; e++ ) s += val[e] * x[rowtab[e]];
+        y[j] = alpha * s + ( beta == 0. ? 0. : beta * y[j] );
+    }
+    return 0;
+}
+
 static double
 nrm2( int64_t n, const double *x )
 {
@@ -1070,4 +1084,117 @@ orc_gmres( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *b
     free( V ); free( W ); free( H ); free( cs ); free( sn ); free( g );
     (void)dgemm_tr_wrap_dummy;
     return iters;
+}
+
+/* ------------------------------------------------------------------------ */
+/* One step of a solve on one side of a factor (the sweeps of               */
+/* solve_cblk_ztrsmsp_forward / _backward, kernels/solve_ztrsmsp.c:300-625, */
+/* and solve_cblk_zdiag :629-679), as pastix_subtask_trsm and                */
+/* pastix_subtask_diag apply them.  A is a coeftab whose diagonal tiles are  */
+/* read by their lower triangle (Cholesky / LDL^T L, LU L, LU U^T); unit:    */
+/* ones on the diagonal instead of the stored entries.                       */
+/*   step 0: forward  A y = b (lower)                                        */
+/*   step 1: backward A^T x = y                                              */
+/*   step 2: x <- x / diag(A_kk)                                             */
+
+#define ORC_STEP_TYPED( T, SFX )                                                                    \
+    static void solvestep_##SFX( int64_t nc, const int64_t *cblk, const int64_t *blok,              \
+                                 const int64_t *browtab, const T *L, int step, int unit,            \
+                                 int64_t nrhs, T *X, int64_t ldx )                                  \
+    {                                                                                               \
+        if ( step == 0 ) {                                                                          \
+            for ( int64_t k = 0; k < nc; k++ ) {                                                    \
+                int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                 \
+                const T *A = L + CB( k, C_COEFOFF );                                                \
+                for ( int64_t q = 0; q < nrhs; q++ ) {                                              \
+                    T *xq = X + CB( k, C_FCOL ) + q * ldx;                                          \
+                    for ( int64_t j = 0; j < n; j++ ) {                                             \
+                        T s = xq[j];                                                                \
+                        for ( int64_t l = 0; l < j; l++ ) s -= A[j + l * n] * xq[l];                \
+                        xq[j] = unit ? s : s / A[j + j * n];                                        \
+                    }                                                                               \
+                    /* X[rows of b] -= A_b x_k for the off-diagonal bloks */                        \
+                    for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {       \
+                        int64_t  m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                        \
+                        const T *Ab = A + BK( b, B_COEFIND );                                       \
+                        T       *xr = X + BK( b, B_FROW ) + q * ldx;                                \
+                        for ( int64_t c = 0; c < n; c++ )                                           \
+                            for ( int64_t r = 0; r < m; r++ ) xr[r] -= Ab[r + c * m] * xq[c];       \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+        else if ( step == 1 ) {                                                                     \
+            for ( int64_t k = nc - 1; k >= 0; k-- ) {                                               \
+                int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                 \
+                const T *A = L + CB( k, C_COEFOFF );                                                \
+                for ( int64_t q = 0; q < nrhs; q++ ) {                                              \
+                    T *xq = X + CB( k, C_FCOL ) + q * ldx;                                          \
+                    /* x_k -= A_b^T X[rows of b] for the cblk's own off-diagonal bloks */           \
+                    for ( int64_t b = CB( k, C_FBLOK ) + 1; b < CB( k + 1, C_FBLOK ); b++ ) {       \
+                        int64_t  m  = BK( b, B_LROW ) - BK( b, B_FROW ) + 1;                        \
+                        const T *Ab = A + BK( b, B_COEFIND );                                       \
+                        const T *xr = X + BK( b, B_FROW ) + q * ldx;                                \
+                        for ( int64_t c = 0; c < n; c++ ) {                                         \
+                            T acc = 0;                                                              \
+                            for ( int64_t r = 0; r < m; r++ ) acc += Ab[r + c * m] * xr[r];         \
+                            xq[c] -= acc;                                                           \
+                        }                                                                           \
+                    }                                                                               \
+                    for ( int64_t j = n - 1; j >= 0; j-- ) {                                        \
+                        T s = xq[j];                                                                \
+                        for ( int64_t l = j + 1; l < n; l++ ) s -= A[l + j * n] * xq[l];            \
+                        xq[j] = unit ? s : s / A[j + j * n];                                        \
+                    }                                                                               \
+                }                                                                                   \
+            }                                                                                       \
+        }                                                                                           \
+        else {                                                                                      \
+            for ( int64_t k = 0; k < nc; k++ ) {                                                    \
+                int64_t  n = CB( k, C_LCOL ) - CB( k, C_FCOL ) + 1;                                 \
+                const T *A = L + CB( k, C_COEFOFF );                                                \
+                for ( int64_t q = 0; q < nrhs; q++ )                                                \
+                    for ( int64_t j = 0; j < n; j++ ) X[CB( k, C_FCOL ) + j + q * ldx] /= A[j + j * n]; \
+            }                                                                                       \
+        }                                                                                           \
+    }
+
+ORC_STEP_TYPED( double, d )
+ORC_STEP_TYPED( float, s )
+ORC_STEP_TYPED( double complex, z )
+ORC_STEP_TYPED( float complex, c )
+
+/* x (double / complex double) <- one solve step with the tiles A of a factor
+ * in flt (a single factor works on x rounded to single and back). */
+int
+orc_solve_step( int flt, int64_t n, int64_t nc, const int64_t *cblk, const int64_t *blok,
+                const int64_t *browtab, const void *A, int step, int unit, int64_t nrhs, void *x, int64_t ldx )
+{
+    if ( flt == 3 ) {
+        solvestep_d( nc, cblk, blok, browtab, (const double *)A, step, unit, nrhs, (double *)x, ldx );
+    }
+    else if ( flt == 5 ) {
+        solvestep_z( nc, cblk, blok, browtab, (const double complex *)A, step, unit, nrhs, (double complex *)x, ldx );
+    }
+    else if ( flt == 2 ) {
+        float  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float ) );
+        double *xd = (double *)x;
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float)xd[i + q * ldx];
+        solvestep_s( nc, cblk, blok, browtab, (const float *)A, step, unit, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double)xs[i + q * n];
+        free( xs );
+    }
+    else {
+        float complex  *xs = malloc( ( n * nrhs > 0 ? n * nrhs : 1 ) * sizeof( float complex ) );
+        double complex *xd = (double complex *)x;
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xs[i + q * n] = (float complex)xd[i + q * ldx];
+        solvestep_c( nc, cblk, blok, browtab, (const float complex *)A, step, unit, nrhs, xs, n );
+        for ( int64_t q = 0; q < nrhs; q++ )
+            for ( int64_t i = 0; i < n; i++ ) xd[i + q * ldx] = (double complex)xs[i + q * n];
+        free( xs );
+    }
+    return 0;
 }

@@ -118,7 +118,11 @@ _sig("spmGenRHSSeed", _I, _I, pastix_int, ctypes.POINTER(spmatrix_t), ctypes.c_u
      _p, pastix_int)
 _sig("spmBackwardError", ctypes.c_double, pastix_int, ctypes.POINTER(spmatrix_t), _p, pastix_int, _p,
      pastix_int, _p)
+_sig("spmCheckAxb", _I, ctypes.c_double, pastix_int, ctypes.POINTER(spmatrix_t), _p, pastix_int, _p, pastix_int,
+     _p, pastix_int)
+_sig("spmForwardError", ctypes.c_double, pastix_int, ctypes.POINTER(spmatrix_t), _p, pastix_int, _p, pastix_int)
 _sig("spm_plrnt", None, pastix_int, ctypes.c_ulonglong, ctypes.c_ulonglong, _p)
+_sig("spmReadDriver", _I, _I, ctypes.c_char_p, ctypes.POINTER(spmatrix_t))
 _sig("spmGenLaplacian7", _I, ctypes.POINTER(spmatrix_t), _I, pastix_int, pastix_int, pastix_int,
      ctypes.c_double, ctypes.c_double)
 _sig("spmGenConvDiff27", _I, ctypes.POINTER(spmatrix_t), pastix_int, pastix_int, pastix_int,
@@ -142,7 +146,19 @@ _sig("pastix_amd_set_option", _I, _p, ctypes.c_char_p, ctypes.c_double)
 _sig("pastix_amd_device_handle", _I, _p, ctypes.POINTER(_p))
 _sig("pastix_amd_solve_dev", _I, _p, pastix_int, _p, pastix_int)
 _sig("pastix_amd_refine_dev", _I, _p, pastix_int, _p, _p, _PI)
+_sig("pastixRhsInit", _I, ctypes.POINTER(_p))
+_sig("pastixRhsFinalize", _I, _p)
+_sig("pastixRhsDoubletoSingle", _I, _p, _p)
+_sig("pastixRhsSingleToDouble", _I, _p, _p)
+_sig("pastix_subtask_applyorder", _I, _p, _I, pastix_int, pastix_int, _p, pastix_int, _p)
+_sig("pastix_subtask_trsm", _I, _p, _I, _I, _I, _I, _p)
+_sig("pastix_subtask_diag", _I, _p, _p)
+_sig("pastix_subtask_solve", _I, _p, _p)
+_sig("pastix_subtask_solve_adv", _I, _p, _I, _p)
+_sig("pastix_subtask_refine", _I, _p, _p, _p)
+_sig("pastixGetDiag", _I, _p, _p, pastix_int)
 _sig("pxd_device_count", _I)
+_sig("pxd_spmv", _I, _p, ctypes.c_double, _p, ctypes.c_double, _p, pastix_int, pastix_int)
 _sig("pxd_malloc", _I, _p, ctypes.c_size_t, ctypes.POINTER(_p))
 _sig("pxd_free", _I, _p, _p)
 _sig("pxd_h2d", _I, _p, _p, _p, ctypes.c_size_t)
@@ -203,6 +219,14 @@ class Spm:
         return m
 
     @classmethod
+    def read(cls, driver, filename):
+        """spmReadDriver: driver is an spm_driver_t value (SpmDriverHB, SpmDriverMM, ...)."""
+        m = cls()
+        _check("spmReadDriver", lib.spmReadDriver(driver, str(filename).encode(), ctypes.byref(m.s)))
+        m._owned_by_c = True
+        return m
+
+    @classmethod
     def from_csc(cls, n, colptr, rowptr, values, mtxtype=None, baseval=0):
         m = cls()
         colptr = np.ascontiguousarray(colptr, dtype=np.int64)
@@ -258,7 +282,7 @@ class Spm:
     def matmat(self, x, alpha=1.0, beta=0.0, y=None):
         x = np.asfortranarray(x, dtype=self.dtype)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
-        y = np.zeros_like(x) if y is None else np.asfortranarray(y, dtype=self.dtype)
+        y = np.zeros_like(x) if y is None else np.array(y, dtype=self.dtype, order="F", copy=True)
         _check("spmMatMat", lib.spmMatMat(ENUMS["PastixNoTrans"], nrhs, alpha, ctypes.byref(self.s),
                                           _ptr(x), self.n, beta, _ptr(y), self.n))
         return y
@@ -286,6 +310,29 @@ class Spm:
                 lib.spmExit(ctypes.byref(self.s))
         except Exception:
             pass
+
+
+def check_axb(spm, eps, x0, b, x):
+    """spmCheckAxb (0: the solution passes); x0 may be None."""
+    b = np.asfortranarray(b, dtype=spm.dtype)
+    x = np.asfortranarray(x, dtype=spm.dtype)
+    nrhs = 1 if b.ndim == 1 else b.shape[1]
+    x0p = None
+    if x0 is not None:
+        x0 = np.asfortranarray(x0, dtype=spm.dtype)
+        x0p = _ptr(x0)
+    return lib.spmCheckAxb(eps, nrhs, ctypes.byref(spm.s), x0p, spm.n, _ptr(b), spm.n, _ptr(x), spm.n)
+
+
+def forward_error(x0, x, spm=None):
+    """max over columns of ||x - x0||_inf / ||x0||_inf (spmForwardError when an spm is given)."""
+    x0 = np.asfortranarray(x0)
+    x = np.asfortranarray(x, dtype=x0.dtype)
+    if x0.ndim == 1:
+        x0, x = x0[:, None], x[:, None]
+    if spm is not None:
+        return lib.spmForwardError(x0.shape[1], ctypes.byref(spm.s), _ptr(x0), spm.n, _ptr(x), spm.n)
+    return float(max(np.abs(x[:, c] - x0[:, c]).max() / np.abs(x0[:, c]).max() for c in range(x0.shape[1])))
 
 
 def plrnt(n, seed=0, shift=0):
@@ -400,20 +447,104 @@ def task_numfact(pastix_data, spm):
     _check("pastix_task_numfact", lib.pastix_task_numfact(pastix_data, ctypes.byref(spm.s)))
 
 
-def task_solve(pastix_data, spm, x, nrhs=-1):
-    x = np.asarray(x)
+def _rhs_array(name, a, spm, nrhs):
+    """Check a right-hand-side array handed to the C API: an ndarray of the
+    matrix's dtype, n rows, column-major (leading dimension n) and at least
+    nrhs columns, so that the C side reads and writes exactly its memory."""
+    if not isinstance(a, np.ndarray):
+        raise ValueError("%s: expected a numpy.ndarray" % name)
+    if a.dtype != spm.dtype:
+        raise ValueError("%s: dtype %s does not match the matrix (%s)" % (name, a.dtype, np.dtype(spm.dtype)))
+    if a.ndim not in (1, 2) or a.shape[0] != spm.n:
+        raise ValueError("%s: expected %d rows" % (name, spm.n))
+    if a.ndim == 2 and not a.flags.f_contiguous:
+        raise ValueError("%s: a 2-D right-hand side must be Fortran-ordered (column-major)" % name)
+    if a.ndim == 1 and not a.flags.c_contiguous:
+        raise ValueError("%s: a 1-D right-hand side must be contiguous" % name)
+    cols = 1 if a.ndim == 1 else a.shape[1]
     if nrhs < 0:
-        nrhs = 1 if x.ndim == 1 else x.shape[1]
-    assert x.flags.f_contiguous or x.ndim == 1
+        nrhs = cols
+    if nrhs > cols:
+        raise ValueError("%s: %d right-hand sides asked for, %d given" % (name, nrhs, cols))
+    return nrhs
+
+
+def task_solve(pastix_data, spm, x, nrhs=-1):
+    nrhs = _rhs_array("task_solve: x", x, spm, nrhs)
     _check("pastix_task_solve", lib.pastix_task_solve(pastix_data, spm.n, nrhs, _ptr(x), spm.n))
 
 
 def task_refine(pastix_data, spm, b, x, nrhs=-1):
-    b = np.asarray(b)
-    if nrhs < 0:
-        nrhs = 1 if b.ndim == 1 else b.shape[1]
+    nrhs = _rhs_array("task_refine: b", b, spm, nrhs)
+    _rhs_array("task_refine: x", x, spm, nrhs)
     _check("pastix_task_refine", lib.pastix_task_refine(pastix_data, spm.n, nrhs, _ptr(b), spm.n,
                                                          _ptr(x), spm.n))
+
+
+# ---------------------------------------------------------------------------
+# Right-hand-side descriptors and the solve subtasks (pypastix's rhsInit,
+# subtask_applyorder, subtask_trsm, subtask_diag, ...)
+
+class pastix_rhs_s(ctypes.Structure):
+    _fields_ = [("allocated", ctypes.c_int8), ("flttype", ctypes.c_int), ("m", pastix_int),
+                ("n", pastix_int), ("ld", pastix_int), ("b", _p), ("cblkb", _p), ("rhscomm", _p),
+                ("Ploc2Pglob", _p)]
+
+
+def rhsInit():
+    rhs = _p()
+    _check("pastixRhsInit", lib.pastixRhsInit(ctypes.byref(rhs)))
+    return rhs
+
+
+def rhsFinalize(rhs):
+    _check("pastixRhsFinalize", lib.pastixRhsFinalize(rhs))
+
+
+def rhs_array(rhs):
+    """A copy of the vectors a descriptor holds (permuted numbering)."""
+    r = ctypes.cast(rhs, ctypes.POINTER(pastix_rhs_s)).contents
+    dt = {ENUMS["SpmDouble"]: np.float64, ENUMS["SpmFloat"]: np.float32,
+          ENUMS["SpmComplex64"]: np.complex128, ENUMS["SpmComplex32"]: np.complex64}[r.flttype]
+    es = np.dtype(dt).itemsize
+    buf = (ctypes.c_char * (r.ld * r.n * es)).from_address(r.b)
+    a = np.frombuffer(buf, dtype=dt).reshape(r.n, r.ld).T[:r.m]
+    return np.array(a, order="F", copy=True)
+
+
+def subtask_applyorder(pastix_data, direction, b, Bp):
+    m = b.shape[0]
+    n = 1 if b.ndim == 1 else b.shape[1]
+    if b.ndim == 2 and not b.flags.f_contiguous:
+        raise ValueError("subtask_applyorder: b must be Fortran-ordered")
+    _check("pastix_subtask_applyorder", lib.pastix_subtask_applyorder(pastix_data, direction, m, n, _ptr(b), m, Bp))
+
+
+def subtask_trsm(pastix_data, side, uplo, trans, diag, b):
+    _check("pastix_subtask_trsm", lib.pastix_subtask_trsm(pastix_data, side, uplo, trans, diag, b))
+
+
+def subtask_diag(pastix_data, b):
+    _check("pastix_subtask_diag", lib.pastix_subtask_diag(pastix_data, b))
+
+
+def subtask_solve(pastix_data, b):
+    _check("pastix_subtask_solve", lib.pastix_subtask_solve(pastix_data, b))
+
+
+def subtask_solve_adv(pastix_data, transA, b):
+    _check("pastix_subtask_solve_adv", lib.pastix_subtask_solve_adv(pastix_data, transA, b))
+
+
+def subtask_refine(pastix_data, b, x):
+    _check("pastix_subtask_refine", lib.pastix_subtask_refine(pastix_data, b, x))
+
+
+def getDiag(pastix_data, n, dtype=np.float64, incx=1):
+    """pastixGetDiag: the factor's diagonal in the permuted numbering."""
+    out = np.zeros(n * incx, dtype=dtype)
+    _check("pastixGetDiag", lib.pastixGetDiag(pastix_data, _ptr(out), incx))
+    return out[::incx]
 
 
 def set_option(pastix_data, key, value):

@@ -157,7 +157,7 @@ struct pxd_ctx_s {
     int32_t  *dsf_facing = nullptr, *dsf_need = nullptr;
     uint32_t *dsf_cnt = nullptr;
     int64_t   nsf_fw = 0, nsf_bw = 0;
-    uint32_t  sf_epoch = 0;
+    uint32_t  sf_epoch = 0, sf_epoch_bw = 0;   // passes of each sweep (their counters grow by one pass's worth)
 
     // options / stats
     int          timing = 0;
@@ -1062,24 +1062,28 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
 // vector (ld n); factor entries T.  The forward launch opens a pass (epoch).
 template <typename T, typename V, int NR>
 int
-sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
+sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y, bool swap )
 {
     constexpr int B  = 128 / (int)sizeof( T );   // columns per load batch of a forward tile
     const bool    lu = ( ctx->facto == FACT_GETRF );
-    const T      *L  = (const T *)ctx->dL;
-    const T      *DI = (const T *)ctx->ddinv;
-    const T      *LB = lu ? (const T *)ctx->dU : L;
-    const T      *DB = lu ? (const T *)ctx->dudinv : DI;
+    // forward: L (or U^T when swap), backward: U via U^T for LU (or L^T when swap)
+    const T      *L  = (const T *)( swap ? ctx->dU : ctx->dL );
+    const T      *DI = (const T *)( swap ? ctx->dudinv : ctx->ddinv );
+    const T      *LB = (const T *)( lu && !swap ? ctx->dU : ctx->dL );
+    const T      *DB = (const T *)( lu && !swap ? ctx->dudinv : ctx->ddinv );
     const size_t  sh = sf_lds_bytes<V, NR>( ctx->colmax );
     if ( ctx->colmax > PXD_TRSM_NT * 16 || sh > lds_room( (const void *)k_sf_forward<T, V, NR, B>, ctx->lds_max ) ||
          sh > lds_room( (const void *)k_sf_backward<T, V, NR>, ctx->lds_max ) ) {
         return 3;
     }
+    if ( ( forward ? ctx->nsf_fw : ctx->nsf_bw ) == 0 ) {
+        return 0;
+    }
     if ( forward ) {
         ctx->sf_epoch++;
     }
-    if ( ( forward ? ctx->nsf_fw : ctx->nsf_bw ) == 0 ) {
-        return 0;
+    else {
+        ctx->sf_epoch_bw++;
     }
     const int64_t nc  = ctx->cblknbr;
     uint32_t     *cnt = ctx->dsf_cnt;
@@ -1092,7 +1096,7 @@ sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
     else {
         hipLaunchKernelGGL( ( k_sf_backward<T, V, NR> ), dim3( (unsigned)ctx->nsf_bw ), dim3( PXD_THREADS ), sh,
                             ctx->stream, ctx->dsf_bw, ctx->dsf_facing, ctx->dcb, ctx->dbk, LB, DB, X, ldx, Y, ctx->n,
-                            nrhs, ctx->sf_epoch, cnt + 2 * nc, cnt + 3 * nc, err );
+                            nrhs, ctx->sf_epoch_bw, cnt + 2 * nc, cnt + 3 * nc, err );
     }
     PXD_CHECK( hipGetLastError() );
     return 0;
@@ -1100,9 +1104,9 @@ sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
 
 template <typename T, typename V>
 int
-sf_sweep( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y )
+sf_sweep( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y, bool swap )
 {
-    return sf_launch<T, V, PXD_SF_NR>( ctx, forward, X, ldx, nrhs, Y );
+    return sf_launch<T, V, PXD_SF_NR>( ctx, forward, X, ldx, nrhs, Y, swap );
 }
 
 // Right-hand sides per solve pass.
@@ -1124,30 +1128,50 @@ sf_check( pxd_ctx_t *ctx )
     if ( e != 0 ) {
         PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, ctx->cblknbr ) ) );
         PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
-        ctx->sf_epoch = 0;
+        ctx->sf_epoch    = 0;
+        ctx->sf_epoch_bw = 0;
         fprintf( stderr, "pastix_amd device: a solve task timed out waiting for its dependencies\n" );
         return 6;
     }
     return 0;
 }
 
-// Forward and backward sweeps for nrhs columns of X (vector precision V,
-// leading dimension ldx) with the factor of type T: V is T, or its double
-// precision counterpart for a single factor (option solve_vec_double; the
-// matrix-core kernels only).
+// Steps of a solve (bit mask): the forward sweep, the diagonal D^{-1} of an
+// LDL^T factor, the backward sweep.
+enum { SOLVE_FWD = 1, SOLVE_DIAG = 2, SOLVE_BWD = 4, SOLVE_ALL = 7 };
+
+// The sweeps of `what` for nrhs columns of X (vector precision V, leading
+// dimension ldx) with the factor of type T: V is T, or its double precision
+// counterpart for a single factor (option solve_vec_double; the matrix-core
+// kernels only).  swap exchanges the two sides of an LU factor: the forward
+// sweep runs on U^T and the backward sweep on L^T (the solve with A^T,
+// reference sopalin/pastix_task_solve.c:597-601).
 template <typename T, typename V>
 int
-solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
+solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL, bool swap = false )
 {
-    const T    *L  = (const T *)ctx->dL;
-    const T    *DI = (const T *)ctx->ddinv;
+    // forward sweep: L (unit for LU and LDL^T); with swap, U^T
+    const T    *L  = (const T *)( swap ? ctx->dU : ctx->dL );
+    const T    *DI = (const T *)( swap ? ctx->dudinv : ctx->ddinv );
     // backward sweep: Cholesky L^T; LU: U, read from the U^T tiles with
-    // the inverses of the U_kk^T
+    // the inverses of the U_kk^T; with swap, L^T
     const bool  lu = ( ctx->facto == FACT_GETRF );
-    const T    *LB = lu ? (const T *)ctx->dU : L;
-    const T    *DB = lu ? (const T *)ctx->dudinv : DI;
+    const T    *LB = (const T *)( lu && !swap ? ctx->dU : ctx->dL );
+    const T    *DB = (const T *)( lu && !swap ? ctx->dudinv : ctx->ddinv );
     hipStream_t s  = ctx->stream;
     const int   nl = (int)ctx->levels.size();
+
+    if ( what == SOLVE_DIAG ) {
+        // x <- D^{-1} x in place, D the diagonal of the L-side diagonal tiles
+        // (solve_cblk_zdiag divides by it whatever the factorization)
+        hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
+                            ctx->dcol2cblk, ctx->dcb, (const T *)ctx->dL, X, ldx, nrhs );
+        PXD_CHECK( hipGetLastError() );
+        return 0;
+    }
+    if ( !ctx->kernels && what != SOLVE_ALL ) {
+        return 3;   // the scalar kernels solve in place, both sweeps at once
+    }
 
     if ( ctx->kernels ) {
         // matrix-core path, diagonal solves chunked and out of place:
@@ -1175,7 +1199,12 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
                           shmax > lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, ctx->lds_max ) ) ) {
             return 3;
         }
-        for ( int li = 0; li < nlv; li++ ) {
+        if ( !( what & SOLVE_FWD ) ) {
+            // the backward sweep reads its input from Y
+            hipLaunchKernelGGL( ( k_copy_cols<V> ), dim3( nblocks( ctx->n, 256 ), nrhs ), dim3( 256 ), 0, s, ctx->n,
+                                (const V *)X, ldx, Y, ldy );
+        }
+        for ( int li = 0; li < nlv && ( what & SOLVE_FWD ); li++ ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
             unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
@@ -1193,14 +1222,21 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs )
             }
         }
         int rc = 0;
-        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( ( what & SOLVE_FWD ) && use_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y, swap ) ) != 0 ) {
             return rc;
         }
-        if ( ctx->facto == FACT_SYTRF ) {
+        if ( ( what & SOLVE_DIAG ) && ctx->facto == FACT_SYTRF ) {
             hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
-                                ctx->dcol2cblk, ctx->dcb, L, Y, ldy, nrhs );
+                                ctx->dcol2cblk, ctx->dcb, (const T *)ctx->dL, Y, ldy, nrhs );
         }
-        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y ) ) != 0 ) {
+        if ( !( what & SOLVE_BWD ) ) {
+            // the forward sweep leaves its result in Y
+            hipLaunchKernelGGL( ( k_copy_cols<V> ), dim3( nblocks( ctx->n, 256 ), nrhs ), dim3( 256 ), 0, s, ctx->n,
+                                (const V *)Y, ldy, X, ldx );
+            PXD_CHECK( hipGetLastError() );
+            return 0;
+        }
+        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y, swap ) ) != 0 ) {
             return rc;
         }
         for ( int li = nlv - 1; li >= 0; li-- ) {
@@ -1318,14 +1354,14 @@ set_lds_attributes( size_t lds )
 // mixed solve does.
 template <typename TV, typename T>
 int
-solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
+solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = SOLVE_ALL, bool swap = false )
 {
     int rc = 0;
     if constexpr ( std::is_same<TV, T>::value ) {
         const int pass = solve_pass_rhs<T>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq );
+            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq, what, swap );
         }
         return rc;
     }
@@ -1333,7 +1369,7 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
         const int pass = solve_pass_rhs<TV>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq );
+            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq, what, swap );
         }
         return rc;
     }
@@ -1352,12 +1388,56 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
                 hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
             }
-            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq );
+            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq, what, swap );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<TV, T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, xs + q * ctx->n, (TV *)dx + ( q0 + q ) * ldx );
             }
         }
+    }
+    return rc;
+}
+
+// The solve steps `what` (swap: the sides of an LU factor exchanged) on
+// nrhs device vectors in the double (complex double) precision.
+int
+solve_steps( pxd_ctx_t *ctx, int what, bool swap, int64_t nrhs, void *dx, int64_t ldx )
+{
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    if ( ctx->dL == nullptr || nrhs < 0 || ( nrhs > 0 && ( dx == nullptr || ldx < ctx->n ) ) ) {
+        return 7;
+    }
+    if ( nrhs == 0 || ctx->n == 0 ) {
+        return 0;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int        rc = 0;
+    if ( hipEventCreate( &e0 ) != hipSuccess || hipEventCreate( &e1 ) != hipSuccess ) {
+        if ( e0 ) (void)hipEventDestroy( e0 );
+        return 3;
+    }
+    if ( hipEventRecord( e0, ctx->stream ) != hipSuccess ) {
+        rc = 3;
+    }
+    if ( rc == 0 ) {
+        switch ( ctx->fflt ) {
+        case FT_DOUBLE: rc = solve_vectors<double, double>( ctx, nrhs, dx, ldx, what, swap ); break;
+        case FT_FLOAT: rc = solve_vectors<double, float>( ctx, nrhs, dx, ldx, what, swap ); break;
+        case FT_CPLX64: rc = solve_vectors<pxcd, pxcd>( ctx, nrhs, dx, ldx, what, swap ); break;
+        default: rc = solve_vectors<pxcd, pxcf>( ctx, nrhs, dx, ldx, what, swap ); break;
+        }
+    }
+    if ( hipEventRecord( e1, ctx->stream ) != hipSuccess || hipEventSynchronize( e1 ) != hipSuccess ) {
+        rc = rc ? rc : 3;
+    }
+    else {
+        ctx->stats.solve_seconds = 1e-3 * ev_ms( e0, e1 );
+    }
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    if ( ctx->kernels && ctx->solve_sf ) {
+        const int rc2 = sf_check( ctx );
+        rc = rc ? rc : rc2;
     }
     return rc;
 }
@@ -1795,30 +1875,60 @@ pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes )
 int
 pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
 {
+    return solve_steps( ctx, SOLVE_ALL, false, nrhs, dx, ldx );
+}
+
+int
+pxd_solve_trans( pxd_ctx_t *ctx, int trans, int64_t nrhs, void *dx, int64_t ldx )
+{
+    // (L U)^T = U^T L^T; Cholesky and LDL^T factors are symmetric (complex
+    // symmetric for LDL^T), so A^T is solved as A
+    return solve_steps( ctx, SOLVE_ALL, trans != 0 && ctx->facto == FACT_GETRF, nrhs, dx, ldx );
+}
+
+int
+pxd_solve_step( pxd_ctx_t *ctx, int step, int64_t nrhs, void *dx, int64_t ldx )
+{
+    const bool lu = ( ctx->facto == FACT_GETRF );
+    switch ( step ) {
+    case PXD_STEP_L_FWD: return solve_steps( ctx, SOLVE_FWD, false, nrhs, dx, ldx );
+    case PXD_STEP_LT_BWD: return solve_steps( ctx, SOLVE_BWD, lu, nrhs, dx, ldx );
+    case PXD_STEP_U_BWD: return lu ? solve_steps( ctx, SOLVE_BWD, false, nrhs, dx, ldx ) : 7;
+    case PXD_STEP_UT_FWD: return lu ? solve_steps( ctx, SOLVE_FWD, true, nrhs, dx, ldx ) : 7;
+    case PXD_STEP_DIAG: return solve_steps( ctx, SOLVE_DIAG, false, nrhs, dx, ldx );
+    default: return 7;
+    }
+}
+
+int
+pxd_get_diag( pxd_ctx_t *ctx, void *dd )
+{
     PXD_CHECK( hipSetDevice( ctx->device ) );
-    if ( ctx->dL == nullptr ) {
+    if ( ctx->dL == nullptr || dd == nullptr ) {
         return 7;
     }
-    hipEvent_t e0, e1;
-    PXD_CHECK( hipEventCreate( &e0 ) );
-    PXD_CHECK( hipEventCreate( &e1 ) );
-    PXD_CHECK( hipEventRecord( e0, ctx->stream ) );
-    int rc = 0;
+    const unsigned nb = nblocks( ctx->n, 256 );
     switch ( ctx->fflt ) {
-    case FT_DOUBLE: rc = solve_vectors<double, double>( ctx, nrhs, dx, ldx ); break;
-    case FT_FLOAT: rc = solve_vectors<double, float>( ctx, nrhs, dx, ldx ); break;
-    case FT_CPLX64: rc = solve_vectors<pxcd, pxcd>( ctx, nrhs, dx, ldx ); break;
-    default: rc = solve_vectors<pxcd, pxcf>( ctx, nrhs, dx, ldx ); break;
+    case FT_DOUBLE:
+        hipLaunchKernelGGL( ( k_get_diag<double, double> ), dim3( nb ), dim3( 256 ), 0, ctx->stream, ctx->n,
+                            ctx->dcol2cblk, ctx->dcb, (const double *)ctx->dL, (double *)dd );
+        break;
+    case FT_FLOAT:
+        hipLaunchKernelGGL( ( k_get_diag<float, double> ), dim3( nb ), dim3( 256 ), 0, ctx->stream, ctx->n,
+                            ctx->dcol2cblk, ctx->dcb, (const float *)ctx->dL, (double *)dd );
+        break;
+    case FT_CPLX64:
+        hipLaunchKernelGGL( ( k_get_diag<pxcd, pxcd> ), dim3( nb ), dim3( 256 ), 0, ctx->stream, ctx->n,
+                            ctx->dcol2cblk, ctx->dcb, (const pxcd *)ctx->dL, (pxcd *)dd );
+        break;
+    default:
+        hipLaunchKernelGGL( ( k_get_diag<pxcf, pxcd> ), dim3( nb ), dim3( 256 ), 0, ctx->stream, ctx->n,
+                            ctx->dcol2cblk, ctx->dcb, (const pxcf *)ctx->dL, (pxcd *)dd );
+        break;
     }
-    PXD_CHECK( hipEventRecord( e1, ctx->stream ) );
-    PXD_CHECK( hipEventSynchronize( e1 ) );
-    ctx->stats.solve_seconds = 1e-3 * ev_ms( e0, e1 );
-    (void)hipEventDestroy( e0 );
-    (void)hipEventDestroy( e1 );
-    if ( rc == 0 && ctx->kernels && ctx->solve_sf ) {
-        rc = sf_check( ctx );
-    }
-    return rc;
+    PXD_CHECK( hipGetLastError() );
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    return 0;
 }
 
 int
@@ -1826,7 +1936,8 @@ pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy, i
 {
     const int64_t *cp = ctx->dcolptr, *rt = ctx->drowtab;
     const void    *vv = ctx->dval;
-    if ( ctx->mtx == 111 /* general: rows of A are the columns of the transpose */ ) {
+    if ( ctx->dtval != nullptr ) {
+        // general or Hermitian: rows of A are the columns of the transpose
         cp = ctx->dtcolptr;
         rt = ctx->dtrowtab;
         vv = ctx->dtval;

@@ -453,6 +453,30 @@ k_convert( int64_t n, const TI *__restrict__ in, TO *__restrict__ out )
     if ( i < n ) out[i] = px_from_any<TO>( in[i] );
 }
 
+// Y(:, q) <- X(:, q) for the columns of the grid's y dimension
+template <typename V>
+__global__ void
+k_copy_cols( int64_t n, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y, int64_t ldy )
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i < n ) Y[i + blockIdx.y * ldy] = X[i + blockIdx.y * ldx];
+}
+
+// dd[i] = the diagonal entry i of the factor's diagonal tiles (permuted
+// numbering), widened to the vector precision (coeftab_zgetdiag,
+// sopalin/coeftab_z.c:659-689)
+template <typename T, typename V>
+__global__ void
+k_get_diag( int64_t n, const int32_t *__restrict__ col2cblk, const dcblk *__restrict__ cb,
+            const T *__restrict__ L, V *__restrict__ dd )
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n ) return;
+    const dcblk   c  = cb[col2cblk[i]];
+    const int32_t nk = c.lcolnum - c.fcolnum + 1;
+    dd[i] = px_cvt<V>( L[c.coefoff + (int64_t)( i - c.fcolnum ) * ( nk + 1 )] );
+}
+
 // LDL^T solve, between the sweeps: y_i <- y_i / D_i, D read from the
 // diagonal of the cblk's diagonal tile (solve_cblk_zdiag, kernels/solve_ztrsmsp.c:629-679);
 // factor entries T, vectors V (T or its double-precision counterpart)

@@ -608,6 +608,10 @@ pastix_subtask_spm2bcsc( pastix_data_t *pd, spmatrix_t *spm )
     }
     pd->dparm[DPARM_A_NORM] = pd->bcsc->Anorm_f;
     pd->csc = spm;
+    /* the analysis depends on the pattern only: the values given here may be
+     * of another arithmetic than those of the ordering step, and they decide
+     * the type of the factor and of the right-hand sides from now on */
+    pd->iparm[IPARM_FLOAT] = spm->flttype;
     pd->steps &= ~( STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE );
     pd->steps |= STEP_CSC2BCSC;
     return PASTIX_SUCCESS;
@@ -819,6 +823,20 @@ pastixRhsSingleToDouble( const pastix_rhs_t sB, pastix_rhs_t dB )
     return PASTIX_SUCCESS;
 }
 
+/* Arithmetic of the right-hand sides: that of the matrix given to numfact
+ * (the bcsc), else of the one given to the analysis. */
+static int
+px_rhs_flttype( const pastix_data_t *pd )
+{
+    if ( pd->bcsc ) {
+        return pd->bcsc->flttype;
+    }
+    if ( pd->csc ) {
+        return pd->csc->flttype;
+    }
+    return (int)pd->iparm[IPARM_FLOAT];
+}
+
 /* b_perm[permtab[i]] = b[i] (forward) or the inverse (backward). */
 int
 pastix_subtask_applyorder( pastix_data_t *pd, pastix_dir_t dir, pastix_int_t m, pastix_int_t n,
@@ -831,7 +849,9 @@ pastix_subtask_applyorder( pastix_data_t *pd, pastix_dir_t dir, pastix_int_t m, 
         return PASTIX_ERR_BADPARAMETER;
     }
     perm = pd->ordemesh->permtab;
-    cplx = ( pd->iparm[IPARM_FLOAT] == SpmComplex64 );
+    /* the vectors have the type of the matrix in use (pastix_data->csc,
+     * sopalin/pastix_task_solve.c:197), as every later step sizes them */
+    cplx = ( px_rhs_flttype( pd ) == SpmComplex64 );
     es   = cplx ? 16 : 8;
     if ( dir == PastixDirForward ) {
         if ( Bp->allocated <= 0 || Bp->m != m || Bp->n != n ) {
@@ -864,7 +884,116 @@ pastix_subtask_applyorder( pastix_data_t *pd, pastix_dir_t dir, pastix_int_t m, 
 static size_t
 px_vec_es( const pastix_data_t *pd )
 {
-    return ( pd->bcsc && pd->bcsc->flttype == SpmComplex64 ) ? 16 : 8;
+    return ( px_rhs_flttype( pd ) == SpmComplex64 ) ? 16 : 8;
+}
+
+/*
+ * A right-hand-side descriptor on the device: the vectors in double (complex
+ * double), leading dimension m.  A single-precision descriptor (the result of
+ * pastixRhsDoubletoSingle) is widened on the way up and rounded on the way
+ * back; the engine converts to the factor precision itself.
+ */
+static int
+px_rhs_check( const pastix_data_t *pd, const pastix_rhs_t b, const char *who )
+{
+    int want = px_rhs_flttype( pd );
+    int flt  = b->flttype;
+    int ok   = ( flt == want ) || ( want == SpmDouble && flt == SpmFloat ) ||
+             ( want == SpmComplex64 && flt == SpmComplex32 );
+    if ( !ok || b->m != pd->solvmatr->nodenbr || b->n < 0 || ( b->n > 0 && ( b->b == NULL || b->ld < b->m ) ) ) {
+        px_print_error( "%s: the right-hand side does not match the factorized matrix", who );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return PASTIX_SUCCESS;
+}
+
+static int
+px_rhs_upload( pastix_data_t *pd, const pastix_rhs_t b, void **dx )
+{
+    pastix_int_t m = b->m, n = b->n;
+    int          single = ( b->flttype == SpmFloat || b->flttype == SpmComplex32 );
+    int          cplx   = ( b->flttype == SpmComplex64 || b->flttype == SpmComplex32 );
+    size_t       es     = cplx ? 16 : 8;
+    size_t       bytes  = (size_t)m * n * es;
+    void        *h      = NULL;
+    int          rc;
+    *dx = NULL;
+    if ( ( rc = pxd_malloc( pd->dev, bytes > 0 ? bytes : 16, dx ) ) != PASTIX_SUCCESS ) {
+        *dx = NULL;
+        return rc;
+    }
+    if ( single || b->ld != m ) {
+        h = px_malloc( bytes > 0 ? bytes : 16 );
+        for ( pastix_int_t j = 0; j < n; j++ ) {
+            for ( pastix_int_t i = 0; i < m; i++ ) {
+                if ( b->flttype == SpmFloat ) {
+                    ( (double *)h )[i + j * m] = ( (const float *)b->b )[i + j * b->ld];
+                }
+                else if ( b->flttype == SpmComplex32 ) {
+                    ( (double complex *)h )[i + j * m] = ( (const float complex *)b->b )[i + j * b->ld];
+                }
+                else {
+                    memcpy( (char *)h + ( i + j * m ) * es, (const char *)b->b + ( i + j * b->ld ) * es, es );
+                }
+            }
+        }
+    }
+    rc = pxd_h2d( pd->dev, *dx, h ? h : b->b, bytes );
+    free( h );
+    if ( rc != PASTIX_SUCCESS ) {
+        pxd_free( pd->dev, *dx );
+        *dx = NULL;
+    }
+    return rc;
+}
+
+static int
+px_rhs_download( pastix_data_t *pd, void *dx, pastix_rhs_t b )
+{
+    pastix_int_t m = b->m, n = b->n;
+    int          single = ( b->flttype == SpmFloat || b->flttype == SpmComplex32 );
+    int          cplx   = ( b->flttype == SpmComplex64 || b->flttype == SpmComplex32 );
+    size_t       es     = cplx ? 16 : 8;
+    size_t       bytes  = (size_t)m * n * es;
+    void        *h      = ( single || b->ld != m ) ? px_malloc( bytes > 0 ? bytes : 16 ) : b->b;
+    int          rc     = pxd_d2h( pd->dev, h, dx, bytes );
+    if ( rc == PASTIX_SUCCESS && h != b->b ) {
+        for ( pastix_int_t j = 0; j < n; j++ ) {
+            for ( pastix_int_t i = 0; i < m; i++ ) {
+                if ( b->flttype == SpmFloat ) {
+                    ( (float *)b->b )[i + j * b->ld] = (float)( (const double *)h )[i + j * m];
+                }
+                else if ( b->flttype == SpmComplex32 ) {
+                    ( (float complex *)b->b )[i + j * b->ld] = (float complex)( (const double complex *)h )[i + j * m];
+                }
+                else {
+                    memcpy( (char *)b->b + ( i + j * b->ld ) * es, (const char *)h + ( i + j * m ) * es, es );
+                }
+            }
+        }
+    }
+    if ( h != b->b ) {
+        free( h );
+    }
+    return rc;
+}
+
+/* Run one device solve call on the descriptor b (upload, call, download). */
+typedef int ( *px_solve_fn_t )( pxd_ctx_t *, int, int64_t, void *, int64_t );
+static int
+px_rhs_apply( pastix_data_t *pd, pastix_rhs_t b, px_solve_fn_t fn, int arg )
+{
+    void *dx;
+    int   rc = px_rhs_upload( pd, b, &dx );
+    if ( rc != PASTIX_SUCCESS ) {
+        return rc;
+    }
+    rc = fn( pd->dev, arg, b->n, dx, b->m );
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = px_rhs_download( pd, dx, b );
+    }
+    pxd_free( pd->dev, dx );
+    return rc;
 }
 
 int
@@ -880,27 +1009,31 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_int_t nrhs, void *dx, pastix_int
 int
 pastix_subtask_solve_adv( pastix_data_t *pd, pastix_trans_t transA, pastix_rhs_t b )
 {
-    void  *dx;
-    size_t bytes;
     int    rc;
+    int    cplx;
     double t0;
-    if ( pd == NULL || b == NULL ) {
+    if ( pd == NULL ) {
+        px_print_error( "pastix_task_solve: wrong pastix_data parameter" );
         return PASTIX_ERR_BADPARAMETER;
     }
     if ( !( pd->steps & STEP_NUMFACT ) ) {
-        px_print_error( "pastix_subtask_solve: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
+        px_print_error( "pastix_task_solve: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
         return PASTIX_ERR_BADPARAMETER;
     }
-    if ( transA != PastixNoTrans && pd->iparm[IPARM_FACTORIZATION] == PastixFactLU ) {
-        return PASTIX_ERR_NOTIMPLEMENTED;
+    if ( b == NULL || ( rc = px_rhs_check( pd, b, "pastix_task_solve" ) ) != PASTIX_SUCCESS ) {
+        return PASTIX_ERR_BADPARAMETER;
     }
-    t0    = px_wtime();
-    bytes = (size_t)b->m * b->n * px_vec_es( pd );
-    if ( ( rc = pxd_malloc( pd->dev, bytes, &dx ) ) != PASTIX_SUCCESS ) return rc;
-    pxd_h2d( pd->dev, dx, b->b, bytes );
-    rc = pxd_solve( pd->dev, b->n, dx, b->m );
-    pxd_d2h( pd->dev, b->b, dx, bytes );
-    pxd_free( pd->dev, dx );
+    /* op(A) = A^T is solved as U^T L^T for LU and as A itself for the
+     * symmetric factorizations; conjugate transposes would need conj(L)
+     * (sopalin/pastix_task_solve.c:541-572) */
+    cplx = ( b->flttype == SpmComplex64 || b->flttype == SpmComplex32 );
+    if ( transA != PastixNoTrans && !( transA == PastixTrans || ( !cplx && transA == PastixConjTrans &&
+                                                                  pd->iparm[IPARM_FACTORIZATION] != PastixFactLU ) ) ) {
+        px_print_error( "pastix_task_solve: transA incompatible with the factotype used (require extra conj(L) not handled)" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    t0 = px_wtime();
+    rc = px_rhs_apply( pd, b, pxd_solve_trans, transA != PastixNoTrans );
     pd->dparm[DPARM_SOLV_TIME] = px_wtime() - t0;
     return rc;
 }
@@ -911,21 +1044,82 @@ pastix_subtask_solve( pastix_data_t *pd, pastix_rhs_t b )
     return pastix_subtask_solve_adv( pd, PastixNoTrans, b );
 }
 
+/*
+ * One triangular sweep (reference sopalin/pastix_task_solve.c:296-360).  The
+ * engine keeps the explicit inverses of the diagonal blocks that the
+ * factorization fixes (unit L for LU and LDL^T, non-unit for Cholesky), so
+ * the accepted (uplo, trans, diag) are those the reference's own solves use:
+ *   Lower NoTrans          L y = b     (diag: Unit for LU / LDL^T, NonUnit for Cholesky)
+ *   Lower Trans/ConjTrans  L^T x = y   (same diag rule; ConjTrans on real data only)
+ *   Upper NoTrans          U x = y     (LU, NonUnit)
+ *   Upper Trans            U^T y = b   (LU, NonUnit)
+ * Right-side application is not handled (the reference asserts on it).
+ */
 int
 pastix_subtask_trsm( pastix_data_t *pd, pastix_side_t side, pastix_uplo_t uplo, pastix_trans_t trans,
                      pastix_diag_t diag, pastix_rhs_t b )
 {
-    (void)pd; (void)side; (void)uplo; (void)trans; (void)diag; (void)b;
-    px_print_error( "pastix_subtask_trsm: the device engine performs both sweeps in pastix_subtask_solve" );
-    return PASTIX_ERR_NOTIMPLEMENTED;
+    int facto, cplx, step, unit_l;
+    if ( pd == NULL ) {
+        px_print_error( "pastix_subtask_trsm: wrong pastix_data parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( b == NULL ) {
+        px_print_error( "pastix_subtask_trsm: wrong Bp parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !( pd->steps & STEP_NUMFACT ) ) {
+        px_print_error( "pastix_subtask_trsm: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( px_rhs_check( pd, b, "pastix_subtask_trsm" ) != PASTIX_SUCCESS ) {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    facto  = (int)pd->iparm[IPARM_FACTORIZATION];
+    cplx   = ( b->flttype == SpmComplex64 || b->flttype == SpmComplex32 );
+    unit_l = ( facto == PastixFactLU || facto == PastixFactLDLT || facto == PastixFactLDLH );
+    if ( !cplx && trans == PastixConjTrans ) {
+        trans = PastixTrans;
+    }
+    step = -1;
+    if ( side == PastixLeft && uplo == PastixLower && trans == PastixNoTrans ) {
+        step = ( diag == ( unit_l ? PastixUnit : PastixNonUnit ) ) ? PXD_STEP_L_FWD : -1;
+    }
+    else if ( side == PastixLeft && uplo == PastixLower && trans == PastixTrans ) {
+        step = ( diag == ( unit_l ? PastixUnit : PastixNonUnit ) ) ? PXD_STEP_LT_BWD : -1;
+    }
+    else if ( side == PastixLeft && uplo == PastixUpper && facto == PastixFactLU && diag == PastixNonUnit ) {
+        step = ( trans == PastixNoTrans ) ? PXD_STEP_U_BWD : ( trans == PastixTrans ) ? PXD_STEP_UT_FWD : -1;
+    }
+    if ( step < 0 ) {
+        px_print_error( "pastix_subtask_trsm: (side, uplo, trans, diag) = (%d, %d, %d, %d) is not a sweep of this factorization",
+                        (int)side, (int)uplo, (int)trans, (int)diag );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return px_rhs_apply( pd, b, pxd_solve_step, step );
 }
 
+/* x <- D^{-1} x, D the diagonal of the factor's diagonal tiles
+ * (reference sopalin/pastix_task_solve.c:417-470 -> sopalin_zdiag). */
 int
 pastix_subtask_diag( pastix_data_t *pd, pastix_rhs_t b )
 {
-    (void)pd; (void)b;
-    px_print_error( "pastix_subtask_diag: the device engine applies D inside pastix_subtask_solve" );
-    return PASTIX_ERR_NOTIMPLEMENTED;
+    if ( pd == NULL ) {
+        px_print_error( "pastix_subtask_diag: wrong pastix_data parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( b == NULL ) {
+        px_print_error( "pastix_subtask_diag: wrong Bp parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !( pd->steps & STEP_NUMFACT ) ) {
+        px_print_error( "pastix_subtask_trsm: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( px_rhs_check( pd, b, "pastix_subtask_diag" ) != PASTIX_SUCCESS ) {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return px_rhs_apply( pd, b, pxd_solve_step, PXD_STEP_DIAG );
 }
 
 int
@@ -955,8 +1149,7 @@ pastix_task_solve( pastix_data_t *pd, pastix_int_t m, pastix_int_t nrhs, void *B
 int
 pastix_subtask_refine( pastix_data_t *pd, pastix_rhs_t Bp, pastix_rhs_t Xp )
 {
-    void        *db, *dx;
-    size_t       bytes;
+    void        *db = NULL, *dx = NULL;
     int          rc;
     pastix_int_t iters = 0;
     double       t0;
@@ -967,16 +1160,23 @@ pastix_subtask_refine( pastix_data_t *pd, pastix_rhs_t Bp, pastix_rhs_t Xp )
         px_print_error( "pastix_subtask_refine: Numerical factorization hasn't been done." );
         return PASTIX_ERR_BADPARAMETER;
     }
-    t0    = px_wtime();
-    bytes = (size_t)Bp->m * Bp->n * px_vec_es( pd );
-    if ( ( rc = pxd_malloc( pd->dev, bytes, &db ) ) != PASTIX_SUCCESS ) return rc;
-    if ( ( rc = pxd_malloc( pd->dev, bytes, &dx ) ) != PASTIX_SUCCESS ) return rc;
-    pxd_h2d( pd->dev, db, Bp->b, bytes );
-    pxd_h2d( pd->dev, dx, Xp->b, bytes );
-    rc = px_refine_run( pd, Bp->n, db, dx, &iters );
-    pxd_d2h( pd->dev, Xp->b, dx, bytes );
-    pxd_free( pd->dev, db );
-    pxd_free( pd->dev, dx );
+    if ( px_rhs_check( pd, Bp, "pastix_subtask_refine" ) != PASTIX_SUCCESS ||
+         px_rhs_check( pd, Xp, "pastix_subtask_refine" ) != PASTIX_SUCCESS || Bp->n != Xp->n ) {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    t0 = px_wtime();
+    rc = px_rhs_upload( pd, Bp, &db );
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = px_rhs_upload( pd, Xp, &dx );
+    }
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = px_refine_run( pd, Bp->n, db, dx, &iters );
+    }
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = px_rhs_download( pd, dx, Xp );
+    }
+    if ( db ) pxd_free( pd->dev, db );
+    if ( dx ) pxd_free( pd->dev, dx );
     pd->iparm[IPARM_NBITER]      = iters;
     pd->dparm[DPARM_REFINE_TIME] = px_wtime() - t0;
     return rc;
@@ -1032,11 +1232,59 @@ pastix( pastix_data_t **pastix_data, PASTIX_Comm comm, pastix_int_t n, pastix_in
     return PASTIX_ERR_NOTIMPLEMENTED;
 }
 
+/*
+ * D[incD * i] = diagonal entry i of the factor's diagonal tiles, in the
+ * permuted numbering (reference sopalin/diag.c:54-101 with coeftab_zgetdiag,
+ * sopalin/coeftab_z.c:659-689): L_ii for Cholesky, D_i for LDL^T, U_ii for
+ * LU.  D has the type of IPARM_FLOAT (the matrix); a single-precision factor
+ * is widened.
+ */
 int
-pastixGetDiag( const pastix_data_t *pd, void *x, pastix_int_t incx )
+pastixGetDiag( const pastix_data_t *pd, void *D, pastix_int_t incD )
 {
-    (void)pd; (void)x; (void)incx;
-    return PASTIX_ERR_NOTIMPLEMENTED;
+    pastix_int_t n;
+    int          cplx, rc;
+    void        *dd = NULL;
+    void        *h;
+    if ( pd == NULL ) {
+        px_print_error( "pastix_getDiag: wrong pastix_data parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( D == NULL ) {
+        px_print_error( "pastix_getDiag: D parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( incD <= 0 ) {
+        px_print_error( "pastix_getDiag: incD parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !( pd->steps & STEP_NUMFACT ) ) {
+        px_print_error( "pastix_getDiag: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    n    = pd->solvmatr->nodenbr;
+    cplx = ( px_rhs_flttype( pd ) == SpmComplex64 );
+    if ( ( rc = pxd_malloc( pd->dev, (size_t)( n > 0 ? n : 1 ) * ( cplx ? 16 : 8 ), &dd ) ) != PASTIX_SUCCESS ) {
+        return rc;
+    }
+    h  = px_malloc( (size_t)( n > 0 ? n : 1 ) * ( cplx ? 16 : 8 ) );
+    rc = pxd_get_diag( pd->dev, dd );
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = pxd_d2h( pd->dev, h, dd, (size_t)n * ( cplx ? 16 : 8 ) );
+    }
+    if ( rc == PASTIX_SUCCESS ) {
+        for ( pastix_int_t i = 0; i < n; i++ ) {
+            if ( cplx ) {
+                ( (double complex *)D )[i * incD] = ( (double complex *)h )[i];
+            }
+            else {
+                ( (double *)D )[i * incD] = ( (double *)h )[i];
+            }
+        }
+    }
+    free( h );
+    pxd_free( pd->dev, dd );
+    return rc;
 }
 
 void

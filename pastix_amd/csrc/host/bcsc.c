@@ -142,7 +142,9 @@ px_bcsc_init( px_bcsc_t *b, const spmatrix_t *spm, const pastix_order_t *ord )
         }
     }
     build_csc( n, ne, ri, ci, vv, cplx, &b->colptr, &b->rowtab, &b->Lvalues, &b->nnz );
-    if ( spm->mtxtype == SpmGeneral ) {
+    /* the transpose, for the row gathers of the SpMV and the U side of LU:
+     * a general matrix, and a Hermitian one (A^T = conj(A) != A) */
+    if ( spm->mtxtype == SpmGeneral || ( cplx && spm->mtxtype == SpmHermitian ) ) {
         pastix_int_t tnnz;
         build_csc( n, ne, ci, ri, vv, cplx, &b->trowptr, &b->tcoltab, &b->Uvalues, &tnnz );
     }

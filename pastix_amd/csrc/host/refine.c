@@ -184,6 +184,11 @@ gmres_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int
     V = W = dh = dy = NULL;
     if ( ref_alloc( r, blk * im1, &V ) || ref_alloc( r, blk * im, &W ) ||
          ref_alloc( r, im1 * nr * r->es, &dh ) || ref_alloc( r, im * nr * r->es, &dy ) ) {
+        /* hand back what was taken before the failure */
+        ref_release( r, V, blk * im1 );
+        ref_release( r, W, blk * im );
+        ref_release( r, dh, im1 * nr * r->es );
+        ref_release( r, dy, im * nr * r->es );
         return PASTIX_ERR_OUTOFMEMORY;
     }
     H      = px_calloc( (size_t)nr * im1 * im, sizeof( zc ) );
@@ -334,6 +339,10 @@ sr_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
     int          rc = PASTIX_SUCCESS;
 
     if ( ref_alloc( r, blk, &R ) ) {
+        free( normb );
+        free( target );
+        free( conv );
+        free( one );
         return PASTIX_ERR_OUTOFMEMORY;
     }
     pxd_matrix_norm_inf( r->dev, &normA );
@@ -395,7 +404,8 @@ bicgstab_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_
 {
     pastix_int_t n = r->n, nr = r->nrhs, iters = 0;
     size_t       blk = (size_t)n * nr * r->es;
-    void        *R, *R2, *P, *Yv, *V, *S, *Z, *Tv, *G2;
+    void        *R = NULL, *R2 = NULL, *P = NULL, *Yv = NULL, *V = NULL, *S = NULL, *Z = NULL, *Tv = NULL,
+                *G2 = NULL;
     void       **vec[] = { &R, &R2, &P, &Yv, &V, &S, &Z, &Tv, &G2 };
     double      *normb = px_calloc( nr, sizeof( double ) ), *target = px_calloc( nr, sizeof( double ) );
     double      *rn = px_calloc( nr, sizeof( double ) );
@@ -408,6 +418,9 @@ bicgstab_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_
 
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
         if ( ref_alloc( r, blk, vec[i] ) ) {
+            for ( size_t j = 0; j < i; j++ ) ref_release( r, *vec[j], blk );
+            free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( beta );
+            free( om ); free( t1 ); free( t2 );
             return PASTIX_ERR_OUTOFMEMORY;
         }
     }
@@ -511,7 +524,7 @@ cg_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
 {
     pastix_int_t n = r->n, nr = r->nrhs, iters = 0;
     size_t       blk = (size_t)n * nr * r->es;
-    void        *R, *P, *Z, *Q;
+    void        *R = NULL, *P = NULL, *Z = NULL, *Q = NULL;
     void       **vec[] = { &R, &P, &Z, &Q };
     double      *normb = px_calloc( nr, sizeof( double ) ), *target = px_calloc( nr, sizeof( double ) );
     double      *rn = px_calloc( nr, sizeof( double ) );
@@ -523,6 +536,8 @@ cg_run( ref_ctx_t *r, void *db, void *dx, double eps, double berr, pastix_int_t 
 
     for ( size_t i = 0; i < sizeof( vec ) / sizeof( vec[0] ); i++ ) {
         if ( ref_alloc( r, blk, vec[i] ) ) {
+            for ( size_t j = 0; j < i; j++ ) ref_release( r, *vec[j], blk );
+            free( normb ); free( target ); free( rn ); free( conv ); free( alpha ); free( rz ); free( t1 );
             return PASTIX_ERR_OUTOFMEMORY;
         }
     }
@@ -602,7 +617,10 @@ px_refine_run( pastix_data_t *pd, pastix_int_t nrhs, void *db, void *dx, pastix_
     r.n    = pd->bcsc->n;
     r.cplx = ( pd->bcsc->flttype == SpmComplex64 );
     r.es   = r.cplx ? 16 : 8;
+    r.dscal = NULL;
+    r.dnrm  = NULL;
     if ( ref_alloc( &r, 64 * 16, &r.dscal ) || ref_alloc( &r, 64 * 8, (void **)&r.dnrm ) ) {
+        ref_release( &r, r.dscal, 64 * 16 );
         return PASTIX_ERR_OUTOFMEMORY;
     }
 

@@ -382,19 +382,61 @@ spmBackwardError( spm_int_t nrhs, const spmatrix_t *spm, const void *b, spm_int_
     return worst;
 }
 
+/* max over the columns of ||x - x0||_inf / ||x0||_inf (NaN kept) */
+double
+spmForwardError( spm_int_t nrhs, const spmatrix_t *spm, const void *x0, spm_int_t ldx0, const void *x,
+                 spm_int_t ldx )
+{
+    spm_int_t n     = spm->n;
+    int       cplx  = ( spm->flttype == SpmComplex64 );
+    double    worst = 0.;
+    for ( spm_int_t c = 0; c < nrhs; c++ ) {
+        double nd = 0., n0 = 0.;
+        for ( spm_int_t i = 0; i < n; i++ ) {
+            if ( cplx ) {
+                double complex a = ( (const double complex *)x0 )[i + c * ldx0];
+                double complex v = ( (const double complex *)x )[i + c * ldx];
+                nd = max_nan( nd, cabs( v - a ) );
+                n0 = max_nan( n0, cabs( a ) );
+            }
+            else {
+                double a = ( (const double *)x0 )[i + c * ldx0];
+                double v = ( (const double *)x )[i + c * ldx];
+                nd = max_nan( nd, fabs( v - a ) );
+                n0 = max_nan( n0, fabs( a ) );
+            }
+        }
+        worst = max_nan( worst, ( n0 > 0. ) ? nd / n0 : nd );
+    }
+    return worst;
+}
+
+/*
+ * 0 when the backward error of every column is at most eps and, with x0
+ * given, the forward error ||x - x0||_inf / ||x0||_inf is at most
+ * PX_CHECK_FWD_FACTOR * eps (a forward error may exceed the backward error
+ * by the condition number, hence the margin); 1 otherwise.  A NaN fails.
+ * SPM's own check lives outside the reference tree, so its exact
+ * tolerances are not reproduced here.
+ */
+#define PX_CHECK_FWD_FACTOR 1e4
 int
 spmCheckAxb( double eps, spm_int_t nrhs, const spmatrix_t *spm,
              const void *x0, spm_int_t ldx0, const void *b, spm_int_t ldb,
              const void *x, spm_int_t ldx )
 {
     double berr;
-    (void)x0;
-    (void)ldx0;
+    int    fail;
     if ( eps < 0. ) {
         eps = 1e-12;
     }
     berr = spmBackwardError( nrhs, spm, b, ldb, x, ldx, NULL );
-    return ( berr <= eps ) ? 0 : 1;
+    fail = !( berr <= eps );
+    if ( x0 != NULL ) {
+        double ferr = spmForwardError( nrhs, spm, x0, ldx0, x, ldx );
+        fail        = fail || !( ferr <= PX_CHECK_FWD_FACTOR * eps );
+    }
+    return fail;
 }
 
 /* ------------------------------------------------------------------------ */

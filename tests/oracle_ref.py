@@ -41,6 +41,11 @@ def lib():
                                             ctypes.c_double, _p, _p]
         _lib.orc_solve_ldlt.restype = ctypes.c_int
         _lib.orc_solve_ldlt.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _i64, _p, _i64]
+        _lib.orc_solve_step.restype = ctypes.c_int
+        _lib.orc_solve_step.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, ctypes.c_int, ctypes.c_int,
+                                        _i64, _p, _i64]
+        _lib.orc_spmv_z.restype = ctypes.c_int
+        _lib.orc_spmv_z.argtypes = [_i64, _p, _p, _p, ctypes.c_double, _p, ctypes.c_double, _p]
         _lib.orc_solve_lu.restype = ctypes.c_int
         _lib.orc_solve_lu.argtypes = [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p, _i64]
     return _lib
@@ -153,11 +158,34 @@ class OracleLDLT:
 
 
 def spmv(cp, rt, val, x, alpha=1.0, beta=0.0, y=None):
+    """y = alpha A x + beta y on a full-pattern CSC (column j is row j of the
+    operator), real or complex double."""
     n = len(cp) - 1
-    x = np.ascontiguousarray(x, dtype=np.float64)
-    y = np.zeros(n) if y is None else np.array(y, dtype=np.float64, copy=True)
-    lib().orc_spmv(n, _ptr(cp), _ptr(rt), _ptr(val), alpha, _ptr(x), beta, _ptr(y))
+    cplx = np.iscomplexobj(val)
+    dt = np.complex128 if cplx else np.float64
+    x = np.ascontiguousarray(x, dtype=dt)
+    val = np.ascontiguousarray(val, dtype=dt)
+    y = np.zeros(n, dtype=dt) if y is None else np.array(y, dtype=dt, copy=True)
+    f = lib().orc_spmv_z if cplx else lib().orc_spmv
+    f(n, _ptr(cp), _ptr(rt), _ptr(val), alpha, _ptr(x), beta, _ptr(y))
     return y
+
+
+# solve steps of orc_solve_step: forward sweep, backward (transposed) sweep,
+# division by the diagonal of the diagonal tiles
+STEP_FWD, STEP_BWD, STEP_DIAG = 0, 1, 2
+
+
+def solve_step(solver, A, step, unit, b):
+    """One sweep of a solve (kernels/solve_ztrsmsp.c) on the tiles A (any of
+    the four arithmetics); b double or complex double, returned solved."""
+    flt = _FLT[np.dtype(A.dtype)]
+    x = np.array(b, dtype=np.complex128 if flt in (4, 5) else np.float64, order="F", copy=True)
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    lib().orc_solve_step(flt, solver["nodenbr"], solver["cblknbr"], _ptr(solver["cblk"]), _ptr(solver["blok"]),
+                         _ptr(solver["browtab"]), _ptr(np.ascontiguousarray(A)), step, int(unit), nrhs, _ptr(x),
+                         solver["nodenbr"])
+    return x
 
 
 def tiles(solver, L):
