@@ -21,7 +21,6 @@
 extern "C" {
 #endif
 
-typedef uintptr_t PASTIX_Comm;          /* no MPI (reference include/pastix.h:40) */
 typedef float  _Complex pastix_complex32_t;
 typedef double _Complex pastix_complex64_t;
 typedef spm_coeftype_t pastix_coeftype_t;
@@ -36,7 +35,6 @@ typedef spm_mtxtype_t  pastix_mtxtype_t;
 #define PastixSymmetric SpmSymmetric
 #define PastixHermitian SpmHermitian
 
-typedef struct pastix_data_s pastix_data_t;
 
 /*
  * Right-hand side descriptor (reference common/pastixdata.h:150-160).  In this
@@ -103,6 +101,22 @@ int pastix_subtask_solve     ( pastix_data_t *pastix_data, pastix_rhs_t b );
 int pastix_subtask_solve_adv ( pastix_data_t *pastix_data, pastix_trans_t transA, pastix_rhs_t b );
 int pastix_subtask_refine    ( pastix_data_t *pastix_data, pastix_rhs_t b, pastix_rhs_t x );
 
+/*
+ * Schur complement (reference sopalin/schur.c): the unknowns of the list
+ * (numbered from the spm's baseval) are ordered last; with
+ * IPARM_SCHUR_FACT_MODE = PastixFactModeLocal they are not factorized and
+ * pastixGetSchur returns S = A22 - A21 A11^{-1} A12 (lower triangle for
+ * Cholesky and LDL^T, whole for LU); IPARM_SCHUR_SOLV_MODE selects the parts
+ * the solves apply; pastixRhsSchurGet / Set move the Schur rows of a
+ * permuted right-hand side.
+ */
+void pastixSetSchurUnknownList( pastix_data_t *pastix_data, pastix_int_t n, const pastix_int_t *list );
+int  pastixGetSchur( const pastix_data_t *pastix_data, void *S, pastix_int_t lds );
+int  pastixRhsSchurGet( const pastix_data_t *pastix_data, pastix_int_t m, pastix_int_t n, pastix_rhs_t rhsB,
+                        void *B, pastix_int_t ldb );
+int  pastixRhsSchurSet( const pastix_data_t *pastix_data, pastix_int_t m, pastix_int_t n, void *B,
+                        pastix_int_t ldb, pastix_rhs_t rhsB );
+
 /* Right-hand side helpers (reference sopalin/pastix_rhs.c). */
 int pastixRhsInit          ( pastix_rhs_t *rhs );
 int pastixRhsFinalize      ( pastix_rhs_t rhs );
@@ -111,6 +125,15 @@ int pastixRhsSingleToDouble( const pastix_rhs_t sB, pastix_rhs_t dB );
 
 /* Diagonal of the factor (reference sopalin/diag.c:54). */
 int pastixGetDiag( const pastix_data_t *pastix_data, void *x, pastix_int_t incx );
+
+/* Constant-dof problem expanded to scalar unknowns: ordering, symbol and
+ * matrix (reference sopalin/pastix.c:506-537). */
+void pastixExpand( const pastix_data_t *pastix_data, spmatrix_t *spm );
+
+/* Command-line options of the examples and tests (reference
+ * common/get_options.c:149-330). */
+void pastixGetOptions( int argc, char **argv, pastix_int_t *iparm, double *dparm, int *check,
+                       int *scatter, spm_driver_t *driver, char **filename );
 
 /* Parameters. */
 void pastixDumpParam ( const pastix_data_t *pastix_data );

@@ -53,9 +53,29 @@ void pastixOrderExpand( pastix_order_t *ordeptr, const spmatrix_t *spm );
 /* Nested dissection of an nx x ny x nz grid (reference order/order_grids.c:306). */
 int  pastixOrderGrid ( pastix_order_t **myorder, pastix_int_t nx, pastix_int_t ny, pastix_int_t nz );
 
-/* Text ordering file in the reference's format (reference order/order_io.c). */
-int  pastixOrderLoadFile( pastix_order_t *ordeptr, const char *filename );
-int  pastixOrderSaveFile( const pastix_order_t *ordeptr, const char *filename );
+#ifndef PASTIX_AMD_DATA_DEFINED
+#define PASTIX_AMD_DATA_DEFINED
+typedef struct pastix_data_s pastix_data_t;
+typedef uintptr_t PASTIX_Comm;          /* no MPI (reference include/pastix.h:40) */
+#endif
+
+/* Identity ordering with one supernode (reference order/order.c:123-149). */
+int  pastixOrderAllocId( pastix_order_t *ordeptr, pastix_int_t vertnbr );
+/* The ordering of an instance (reference order/order.c:649-656). */
+pastix_order_t *pastixOrderGet( const pastix_data_t *pastix_data );
+/* One process: nothing to broadcast (reference order/order.c:679-). */
+void pastixOrderBcast( pastix_order_t *ordemesh, int root, PASTIX_Comm pastix_comm );
+
+/*
+ * Ordering files in the reference's format (reference order/order_io.c:
+ * 132-175, 293-337): version, cblknbr, vertnbr, rangtab, permtab and treetab.
+ * pastixOrderLoad reads the file named by the environment variable
+ * PASTIX_FILE_ORDER, else "ordername" in the current directory;
+ * pastixOrderSave writes PASTIX_FILE_ORDER (default "ordergen") into the
+ * instance's output directory "<PASTIX_OUTPUT_DIR or pastix>-XXXXXX".
+ */
+int  pastixOrderLoad( const pastix_data_t *pastix_data, pastix_order_t *ordeptr );
+int  pastixOrderSave( pastix_data_t *pastix_data, const pastix_order_t *ordeptr );
 
 #ifdef __cplusplus
 }

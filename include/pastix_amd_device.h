@@ -98,6 +98,7 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *                   16 columns per batch
  *   "solve_vec_double" 1: a single factor solves double vectors directly;
  *                   0: vectors rounded to single and back (default)
+ *   "schur_solv_mode" 0: local, 1: interface, 2: whole matrix (see pxd_set_schur)
  *   "trsm_blk"      real Cholesky, N > 0: levels with at most N TRSM slabs
  *                   of 64 rows do the off-diagonal TRSM by blocked
  *                   substitution on the 32 x 32 diagonal-block inverses,
@@ -121,6 +122,18 @@ int pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t blokn
                        const int64_t *browtab, int64_t coefnbr );
 
 /*
+ * Schur complement (reference sopalin/schur.c, blend's CBLK_IN_SCHUR): the
+ * cblks from column schurfcol on hold the Schur unknowns (schurfcol must be a
+ * cblk boundary; n, the default, means none).  fact_local = 1
+ * (PastixFactModeLocal): they receive the updates of the other cblks but are
+ * not factorized, so their tiles hold S = A22 - A21 A11^{-1} A12 after
+ * pxd_factorize.  The option "schur_solv_mode" (0 local, 1 interface, 2
+ * whole matrix, reference pastix_solv_mode_t) selects what the solves apply.
+ * Called after pxd_set_structure.
+ */
+int pxd_set_schur( pxd_ctx_t *ctx, int64_t schurfcol, int fact_local );
+
+/*
  * Upload the permuted matrix P A P^T as a full-pattern CSC (both triangles,
  * rows sorted in every column).  flttype is the spm coefficient type of the
  * values (SpmDouble or SpmComplex64).  For general matrices the transpose is
@@ -142,6 +155,8 @@ int pxd_factorize( pxd_ctx_t *ctx, int factotype, int factor_flttype,
 
 /* Download one side of the factor (0: L, 1: U^T) in the factor precision. */
 int pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes );
+/* ... or the count entries from offset on (entries of the factor precision). */
+int pxd_get_coeftab_range( pxd_ctx_t *ctx, int side, int64_t offset, int64_t count, void *host );
 
 /*
  * x <- A^{-1} x in place for nrhs columns of a device array in the vector

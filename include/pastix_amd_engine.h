@@ -31,6 +31,27 @@ int pastix_amd_order_export( const pastix_data_t *pd, pastix_int_t *permtab, pas
 int pastix_amd_symbol_sizes ( const pastix_data_t *pd, pastix_int_t *cblknbr, pastix_int_t *bloknbr );
 int pastix_amd_symbol_export( const pastix_data_t *pd, pastix_int_t *cblk, pastix_int_t *blok );
 
+/*
+ * Symbol files (reference symbol/symbol_io.c:51-127, 147-175; the symbol
+ * structure is internal to the reference too, symbol/symbol.h): version,
+ * cblknbr, bloknbr, nodenbr, baseval, then (fcolnum, lcolnum, bloknum) per
+ * cblk and (frownum, lrownum, fcblknm) per blok.  Return 0 on success.
+ * pastix_subtask_symbfact loads "symbname" with IPARM_IO_STRATEGY
+ * PastixIOLoad and writes "symbgen" into the output directory with
+ * PastixIOSave; pastix_amd_symbol gives the instance's symbol.
+ */
+typedef struct px_symbol_s symbol_matrix_t;
+int pastixSymbolLoad( symbol_matrix_t *symbptr, FILE *stream );
+int pastixSymbolSave( const symbol_matrix_t *symbptr, FILE *stream );
+symbol_matrix_t *pastix_amd_symbol( pastix_data_t *pd );
+void pastixSymbolExit( symbol_matrix_t *symbptr );
+
+/* iparm / dparm index and enum value by name, as pastixGetOptions parses
+ * them (reference common/parse_options.c): -1 when unknown. */
+int pastix_amd_parse_iparm( const char *name );
+int pastix_amd_parse_dparm( const char *name );
+int pastix_amd_parse_enum ( const char *name );
+
 /* Export the permuted full-pattern CSC (bcsc) built by pastix_subtask_spm2bcsc. */
 int pastix_amd_bcsc_sizes ( const pastix_data_t *pd, pastix_int_t *n, pastix_int_t *nnz );
 int pastix_amd_bcsc_export( const pastix_data_t *pd, pastix_int_t *colptr, pastix_int_t *rowtab,

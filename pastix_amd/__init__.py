@@ -107,8 +107,19 @@ _sig("pastixOrderGrid", _I, ctypes.POINTER(ctypes.POINTER(pastix_order_t)), past
 _sig("pastixOrderExit", None, ctypes.POINTER(pastix_order_t))
 _sig("pastixOrderCheck", _I, ctypes.POINTER(pastix_order_t))
 _sig("pastixOrderExpand", None, ctypes.POINTER(pastix_order_t), ctypes.POINTER(spmatrix_t))
-_sig("pastixOrderSaveFile", _I, ctypes.POINTER(pastix_order_t), ctypes.c_char_p)
-_sig("pastixOrderLoadFile", _I, ctypes.POINTER(pastix_order_t), ctypes.c_char_p)
+_sig("pastixOrderSave", _I, _p, ctypes.POINTER(pastix_order_t))
+_sig("pastixOrderLoad", _I, _p, ctypes.POINTER(pastix_order_t))
+_sig("pastixOrderAllocId", _I, ctypes.POINTER(pastix_order_t), pastix_int)
+_sig("pastixOrderGet", ctypes.POINTER(pastix_order_t), _p)
+_sig("pastixExpand", None, _p, ctypes.POINTER(spmatrix_t))
+_sig("pastixGetOptions", None, _I, _p, _p, _p, _p, _p, _p, _p)
+_sig("pastix_amd_symbol", _p, _p)
+_sig("pastixSymbolSave", _I, _p, _p)
+_sig("pastixSymbolLoad", _I, _p, _p)
+_sig("pastixSymbolExit", None, _p)
+_sig("pastix_amd_parse_iparm", _I, ctypes.c_char_p)
+_sig("pastix_amd_parse_dparm", _I, ctypes.c_char_p)
+_sig("pastix_amd_parse_enum", _I, ctypes.c_char_p)
 _sig("spmInit", None, ctypes.POINTER(spmatrix_t))
 _sig("spmExit", None, ctypes.POINTER(spmatrix_t))
 _sig("spmNorm", ctypes.c_double, _I, ctypes.POINTER(spmatrix_t))
@@ -157,6 +168,10 @@ _sig("pastix_subtask_solve", _I, _p, _p)
 _sig("pastix_subtask_solve_adv", _I, _p, _I, _p)
 _sig("pastix_subtask_refine", _I, _p, _p, _p)
 _sig("pastixGetDiag", _I, _p, _p, pastix_int)
+_sig("pastixSetSchurUnknownList", None, _p, pastix_int, _p)
+_sig("pastixGetSchur", _I, _p, _p, pastix_int)
+_sig("pastixRhsSchurGet", _I, _p, pastix_int, pastix_int, _p, _p, pastix_int)
+_sig("pastixRhsSchurSet", _I, _p, pastix_int, pastix_int, _p, pastix_int, _p)
 _sig("pxd_device_count", _I)
 _sig("pxd_spmv", _I, _p, ctypes.c_double, _p, ctypes.c_double, _p, pastix_int, pastix_int)
 _sig("pxd_malloc", _I, _p, ctypes.c_size_t, ctypes.POINTER(_p))
@@ -538,6 +553,31 @@ def subtask_solve_adv(pastix_data, transA, b):
 
 def subtask_refine(pastix_data, b, x):
     _check("pastix_subtask_refine", lib.pastix_subtask_refine(pastix_data, b, x))
+
+
+def setSchurUnknownList(pastix_data, schur_list):
+    schur_list = np.ascontiguousarray(schur_list, dtype=np.int64)
+    lib.pastixSetSchurUnknownList(pastix_data, schur_list.shape[0], _ptr(schur_list))
+
+
+def getSchur(pastix_data, S):
+    """pastixGetSchur into the Fortran-ordered square array S."""
+    if not S.flags.f_contiguous:
+        raise ValueError("getSchur: S must be Fortran-ordered")
+    _check("pastixGetSchur", lib.pastixGetSchur(pastix_data, _ptr(S), S.shape[0]))
+
+
+def rhsSchurGet(pastix_data, rhsB, B):
+    m = B.shape[0]
+    n = 1 if B.ndim == 1 else B.shape[1]
+    _check("pastixRhsSchurGet", lib.pastixRhsSchurGet(pastix_data, m, n, rhsB, _ptr(B), m))
+
+
+def rhsSchurSet(pastix_data, B, rhsB):
+    m = B.shape[0]
+    n = 1 if B.ndim == 1 else B.shape[1]
+    B = np.asfortranarray(B)
+    _check("pastixRhsSchurSet", lib.pastixRhsSchurSet(pastix_data, m, n, _ptr(B), m, rhsB))
 
 
 def getDiag(pastix_data, n, dtype=np.float64, incx=1):

@@ -37,8 +37,8 @@ struct level_t {
     int64_t u0, uc, u1; // update tasks (64 x 64 tiles); [u0, uc) face the next level
     int64_t b0, bc, b1; // update tasks (tall tiles); [b0, bc) face the next level
     int64_t s0, sc, s1; // small updates (one wavefront each); [s0, sc) face the next level
-    int64_t f0, f1;     // solve row tasks (64-row tiles of off-diagonal rows)
-    int64_t d0, d1;     // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks)
+    int64_t f0, fs, f1; // solve row tasks (64-row tiles of off-diagonal rows); [fs, f1): Schur cblks
+    int64_t d0, ds, d1; // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks); [ds, d1): Schur
     int64_t i0, i1;     // block-column inversion tasks
     int64_t g0, g1;     // blok updates (upd_group)
     int64_t q0, q1;     // multi-workgroup panel tasks (32-row blocks of the diagonal tiles)
@@ -158,6 +158,16 @@ struct pxd_ctx_s {
     uint32_t *dsf_cnt = nullptr;
     int64_t   nsf_fw = 0, nsf_bw = 0;
     uint32_t  sf_epoch = 0, sf_epoch_bw = 0;   // passes of each sweep (their counters grow by one pass's worth)
+
+    // Schur complement: the cblks from column schurfcol on (the last ones)
+    // hold the Schur unknowns.  schur_fact_local: they receive the updates of
+    // the other cblks but are not factorized (PastixFactModeLocal).
+    // schur_solv_mode: 0 local (x11 = A11^{-1} b11, rows of the Schur part
+    // untouched), 1 interface (plus b22 -= L21 x11 forward, b11 -= L21^T x22
+    // backward), 2 the whole matrix (reference pastix_solv_mode_t)
+    int64_t      schurfcol = 0;
+    int          schur_fact_local = 1;
+    int          schur_solv_mode = 0;
 
     // options / stats
     int          timing = 0;
@@ -412,10 +422,18 @@ build_schedule( pxd_ctx_t *ctx )
         return 3;
     }
 
+    // within a level, the cblks of the Schur part come after the others
+    // (their solve tasks can then be left out as a suffix of the level)
     std::vector<std::vector<int32_t>> bylev( nlev );
-    for ( int64_t k = 0; k < nc; k++ ) {
-        bylev[level[k]].push_back( (int32_t)k );
+    for ( int pass = 0; pass < 2; pass++ ) {
+        for ( int64_t k = 0; k < nc; k++ ) {
+            if ( ( ctx->hcb[k].fcolnum >= ctx->schurfcol ) == ( pass == 1 ) ) {
+                bylev[level[k]].push_back( (int32_t)k );
+            }
+        }
     }
+    // cblks of the Schur part that the factorization leaves alone
+    auto unfactored = [&]( int32_t k ) { return ctx->schur_fact_local && ctx->hcb[k].fcolnum >= ctx->schurfcol; };
 
     std::vector<int32_t>  list;
     std::vector<pan_task> pan;
@@ -444,9 +462,24 @@ build_schedule( pxd_ctx_t *ctx )
         lv.g0   = ctx->groups.size();
         lv.maxn = 0;
         ivs.clear();
+        lv.ds = -1;
+        lv.fs = -1;
         for ( int32_t k : bylev[L] ) {
             const dcblk &c  = ctx->hcb[k];
             int32_t      nk = c.lcolnum - c.fcolnum + 1;
+            if ( lv.ds < 0 && c.fcolnum >= ctx->schurfcol ) {
+                lv.ds = diag.size();
+                lv.fs = solv.size();
+            }
+            for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
+                solv.push_back( row_task{ k, r0 } );
+            }
+            for ( int32_t r0 = 0; r0 < nk; r0 += PXD_DIAG_ROWS ) {
+                diag.push_back( row_task{ k, r0 } );
+            }
+            if ( unfactored( k ) ) {
+                continue;
+            }
             lv.maxn = std::max( lv.maxn, nk );
             list.push_back( k );
             {
@@ -458,15 +491,13 @@ build_schedule( pxd_ctx_t *ctx )
             for ( int32_t r0 = nk; r0 < c.stride; r0 += ctx->trsm_rt ) {
                 trsm.push_back( row_task{ k, r0 } );
             }
-            for ( int32_t r0 = nk; r0 < c.stride; r0 += 64 ) {
-                solv.push_back( row_task{ k, r0 } );
-            }
-            for ( int32_t r0 = 0; r0 < nk; r0 += PXD_DIAG_ROWS ) {
-                diag.push_back( row_task{ k, r0 } );
-            }
             for ( int32_t J = 0; J + PXD_NB < nk; J += PXD_NB ) {
                 invt.push_back( inv_task{ k, J } );
             }
+        }
+        if ( lv.ds < 0 ) {
+            lv.ds = diag.size();
+            lv.fs = solv.size();
         }
         // update tiles: first those into cblks of the next level (the
         // critical part that the next level's panels wait for), then the rest
@@ -479,6 +510,9 @@ build_schedule( pxd_ctx_t *ctx )
             for ( int32_t k : bylev[L] ) {
                 const dcblk &c  = ctx->hcb[k];
                 int32_t      lb = ctx->hcb[k + 1].fblok;
+                if ( unfactored( k ) ) {
+                    continue;
+                }
                 for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
                     const dblok &bb = ctx->hbk[b];
                     if ( ( level[bb.fcblknm] == L + 1 ) != ( pass == 0 ) ) {
@@ -806,6 +840,9 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     auto panel = [&]( const level_t &lv, hipStream_t st ) {
         unsigned np = (unsigned)( lv.c1 - lv.c0 );
         unsigned ni = (unsigned)( lv.i1 - lv.i0 );
+        if ( np == 0 ) {
+            return;   // a level of unfactored Schur cblks only
+        }
         // rows below a block column of the widest cblk of the level
         const size_t trtri_lds = (size_t)std::max( 1, std::min( lv.maxn - PXD_NB, rcap ) ) * lds_row;
         if ( lu ) {
@@ -1160,17 +1197,40 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
     const T    *DB = (const T *)( lu && !swap ? ctx->dudinv : ctx->ddinv );
     hipStream_t s  = ctx->stream;
     const int   nl = (int)ctx->levels.size();
+    // Schur complement solved apart (solve modes local and interface): the
+    // rows [sf0, n) belong to cblks whose diagonal and tiles are left out
+    // (reference kernels/solve_ztrsmsp.c:358-376,513-557)
+    const bool    schur = ctx->schurfcol < ctx->n && ctx->schur_solv_mode != 2;
+    const bool    iface = ctx->schur_solv_mode == 1;
+    const int64_t sf0   = schur ? ctx->schurfcol : ctx->n;
+    const int64_t ns    = ctx->n - sf0;
+    // rows of the Schur part: copied between the vectors, or cleared
+    auto schur_copy = [&]( const V *from, int64_t ldf, V *to, int64_t ldt ) {
+        if ( ns > 0 ) {
+            hipLaunchKernelGGL( ( k_copy_cols<V> ), dim3( nblocks( ns, 256 ), nrhs ), dim3( 256 ), 0, s, ns,
+                                from + sf0, ldf, to + sf0, ldt );
+        }
+    };
+    auto schur_zero = [&]( V *to, int64_t ldt ) {
+        if ( ns > 0 ) {
+            hipLaunchKernelGGL( ( k_zero_cols<V> ), dim3( nblocks( ns, 256 ), nrhs ), dim3( 256 ), 0, s, ns,
+                                to + sf0, ldt );
+        }
+    };
 
     if ( what == SOLVE_DIAG ) {
         // x <- D^{-1} x in place, D the diagonal of the L-side diagonal tiles
-        // (solve_cblk_zdiag divides by it whatever the factorization)
-        hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
-                            ctx->dcol2cblk, ctx->dcb, (const T *)ctx->dL, X, ldx, nrhs );
+        // (solve_cblk_zdiag divides by it whatever the factorization), the
+        // Schur part aside
+        if ( sf0 > 0 ) {
+            hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( sf0, 256 ) ), dim3( 256 ), 0, s, sf0,
+                                ctx->dcol2cblk, ctx->dcb, (const T *)ctx->dL, X, ldx, nrhs );
+        }
         PXD_CHECK( hipGetLastError() );
         return 0;
     }
-    if ( !ctx->kernels && what != SOLVE_ALL ) {
-        return 3;   // the scalar kernels solve in place, both sweeps at once
+    if ( !ctx->kernels && ( what != SOLVE_ALL || ctx->schurfcol < ctx->n ) ) {
+        return 3;   // the scalar kernels solve in place, both sweeps at once, without a Schur part
     }
 
     if ( ctx->kernels ) {
@@ -1189,7 +1249,7 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
         const int64_t ldy = ctx->n;
         // levels [0, nlv) run level by level, the others in one launch per
         // sweep (passes of at most PXD_SF_NR right-hand sides)
-        const bool    use_sf = ctx->solve_sf && nrhs <= PXD_SF_NR;
+        const bool    use_sf = ctx->solve_sf && nrhs <= PXD_SF_NR && !schur;
         const int     nlv    = use_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
@@ -1204,13 +1264,18 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
             hipLaunchKernelGGL( ( k_copy_cols<V> ), dim3( nblocks( ctx->n, 256 ), nrhs ), dim3( 256 ), 0, s, ctx->n,
                                 (const V *)X, ldx, Y, ldy );
         }
+        if ( ( what & SOLVE_FWD ) && schur && !iface ) {
+            schur_copy( X, ldx, Y, ldy );   // local: b22 is left as it is
+        }
         for ( int li = 0; li < nlv && ( what & SOLVE_FWD ); li++ ) {
             const level_t &lv = ctx->levels[li];
-            unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
-            unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+            unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
+            unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
-            hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
-                                ctx->ddiag + lv.d0, ctx->dcb, DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
+            if ( nd ) {
+                hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
+                                    ctx->ddiag + lv.d0, ctx->dcb, DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
+            }
             if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
                 hipLaunchKernelGGL( ( k_solve_fwd_upd<T, V, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
                                     ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const V *)Y, ldy );
@@ -1225,8 +1290,11 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
         if ( ( what & SOLVE_FWD ) && use_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y, swap ) ) != 0 ) {
             return rc;
         }
-        if ( ( what & SOLVE_DIAG ) && ctx->facto == FACT_SYTRF ) {
-            hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s, ctx->n,
+        if ( ( what & SOLVE_FWD ) && schur && iface ) {
+            schur_copy( X, ldx, Y, ldy );   // interface: b22 - L21 x11
+        }
+        if ( ( what & SOLVE_DIAG ) && ctx->facto == FACT_SYTRF && sf0 > 0 ) {
+            hipLaunchKernelGGL( ( k_diag_scale<T, V> ), dim3( nblocks( sf0, 256 ) ), dim3( 256 ), 0, s, sf0,
                                 ctx->dcol2cblk, ctx->dcb, (const T *)ctx->dL, Y, ldy, nrhs );
         }
         if ( !( what & SOLVE_BWD ) ) {
@@ -1236,13 +1304,22 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
             PXD_CHECK( hipGetLastError() );
             return 0;
         }
+        if ( schur ) {
+            // the gathers read x22 (interface) or nothing of it (local)
+            if ( iface ) {
+                schur_copy( Y, ldy, X, ldx );
+            }
+            else {
+                schur_zero( X, ldx );
+            }
+        }
         if ( use_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y, swap ) ) != 0 ) {
             return rc;
         }
         for ( int li = nlv - 1; li >= 0; li-- ) {
             const level_t &lv = ctx->levels[li];
-            unsigned       nd = (unsigned)( lv.d1 - lv.d0 );
-            unsigned       nf = (unsigned)( lv.f1 - lv.f0 );
+            unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
+            unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
             size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
             if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
                 hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, V, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
@@ -1253,8 +1330,13 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
                                     dim3( PXD_THREADS ), 0, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB,
                                     (const V *)X, ldx, Y, ldy, nrhs );
             }
-            hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
-                                ctx->ddiag + lv.d0, ctx->dcb, DB, (const V *)Y, ldy, X, ldx, nrhs, 1 );
+            if ( nd ) {
+                hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
+                                    ctx->ddiag + lv.d0, ctx->dcb, DB, (const V *)Y, ldy, X, ldx, nrhs, 1 );
+            }
+        }
+        if ( schur && !iface ) {
+            schur_copy( Y, ldy, X, ldx );   // local: the rows of the Schur part are untouched
         }
         PXD_CHECK( hipGetLastError() );
         return 0;
@@ -1409,6 +1491,9 @@ solve_steps( pxd_ctx_t *ctx, int what, bool swap, int64_t nrhs, void *dx, int64_
     }
     if ( nrhs == 0 || ctx->n == 0 ) {
         return 0;
+    }
+    if ( ctx->schurfcol < ctx->n && ctx->schur_solv_mode == 2 && ctx->schur_fact_local ) {
+        return 7;   // the Schur part was not factorized
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int        rc = 0;
@@ -1601,6 +1686,13 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
         ctx->timing = ( value != 0. );
         return 0;
     }
+    if ( k == "schur_solv_mode" ) {
+        if ( value < 0. || value > 2. ) {
+            return 7;
+        }
+        ctx->schur_solv_mode = (int)value;
+        return 0;
+    }
     if ( k == "kernels" ) {
         // takes effect at the next factorization
         ctx->kernels     = ( value != 0. );
@@ -1658,6 +1750,26 @@ pxd_d2h( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes )
 }
 
 int
+pxd_set_schur( pxd_ctx_t *ctx, int64_t schurfcol, int fact_local )
+{
+    if ( schurfcol < 0 || schurfcol > ctx->n ) {
+        return 7;
+    }
+    for ( int64_t k = 0; k < ctx->cblknbr; k++ ) {
+        // the Schur part must start at a cblk boundary
+        if ( ctx->hcb[k].fcolnum < schurfcol && ctx->hcb[k].lcolnum >= schurfcol ) {
+            return 7;
+        }
+    }
+    if ( ctx->schurfcol != schurfcol || ctx->schur_fact_local != ( fact_local != 0 ) ) {
+        ctx->schurfcol        = schurfcol;
+        ctx->schur_fact_local = fact_local != 0;
+        ctx->tasks_dirty      = 1;
+    }
+    return 0;
+}
+
+int
 pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
                    const pxd_cblk_t *cblks, const pxd_blok_t *bloks, const int64_t *browtab,
                    int64_t coefnbr )
@@ -1667,6 +1779,7 @@ pxd_set_structure( pxd_ctx_t *ctx, int64_t n, int64_t cblknbr, int64_t bloknbr,
     }
     PXD_CHECK( hipSetDevice( ctx->device ) );
     ctx->n       = n;
+    ctx->schurfcol = n;   // no Schur part until pxd_set_schur
     ctx->cblknbr = cblknbr;
     ctx->bloknbr = bloknbr;
     ctx->coefnbr = coefnbr;
@@ -1869,6 +1982,19 @@ pxd_get_coeftab( pxd_ctx_t *ctx, int side, void *host, size_t bytes )
     }
     PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
     PXD_CHECK( hipMemcpy( host, src, bytes, hipMemcpyDeviceToHost ) );
+    return 0;
+}
+
+int
+pxd_get_coeftab_range( pxd_ctx_t *ctx, int side, int64_t offset, int64_t count, void *host )
+{
+    const void  *src = ( side == 0 ) ? ctx->dL : ctx->dU;
+    const size_t es  = elt_size( ctx->fflt );
+    if ( src == nullptr || offset < 0 || count < 0 || (size_t)( offset + count ) * es > ctx->coef_bytes ) {
+        return 7;
+    }
+    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    PXD_CHECK( hipMemcpy( host, (const char *)src + (size_t)offset * es, (size_t)count * es, hipMemcpyDeviceToHost ) );
     return 0;
 }
 

@@ -462,6 +462,15 @@ k_copy_cols( int64_t n, const V *__restrict__ X, int64_t ldx, V *__restrict__ Y,
     if ( i < n ) Y[i + blockIdx.y * ldy] = X[i + blockIdx.y * ldx];
 }
 
+// Y(:, q) <- 0 for the columns of the grid's y dimension
+template <typename V>
+__global__ void
+k_zero_cols( int64_t n, V *__restrict__ Y, int64_t ldy )
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i < n ) Y[i + blockIdx.y * ldy] = (V)0;
+}
+
 // dd[i] = the diagonal entry i of the factor's diagonal tiles (permuted
 // numbering), widened to the vector precision (coeftab_zgetdiag,
 // sopalin/coeftab_z.c:659-689)

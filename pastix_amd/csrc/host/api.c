@@ -194,6 +194,7 @@ pastixFinalize( pastix_data_t **pastix_data )
     pastixModelsFree( pd->cpu_models );
     pastixModelsFree( pd->gpu_models );
     free( pd->dir_global );
+    free( pd->schur_list );
     free( pd );
     *pastix_data = NULL;
 }
@@ -201,7 +202,7 @@ pastixFinalize( pastix_data_t **pastix_data )
 /* ------------------------------------------------------------------------ */
 /* Supernodes for a user permutation given without ranges: the fundamental
  * supernodes of the elimination tree of the permuted pattern. */
-static void
+void
 px_order_fundamental_supernodes( const px_graph_t *g, pastix_order_t *ord )
 {
     pastix_int_t  n      = g->n;
@@ -325,12 +326,16 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
     px_graph_prepare( spm, pd->graph );
 
     if ( iparm[IPARM_IO_STRATEGY] & PastixIOLoad ) {
-        const char *fn = getenv( "PASTIX_FILE_ORDER" );
-        int         rc = pastixOrderLoadFile( ord, fn ? fn : "ordername" );
+        int rc = pastixOrderLoad( pd, ord );
         if ( rc != PASTIX_SUCCESS ) {
             return rc;
         }
         pastixOrderBase( ord, 0 );
+        if ( ord->vertnbr != n ) {
+            px_print_error( "pastix_subtask_order: the loaded ordering has %ld vertices, the matrix %ld",
+                            (long)ord->vertnbr, (long)n );
+            return PASTIX_ERR_BADPARAMETER;
+        }
     }
     else if ( iparm[IPARM_ORDERING] == PastixOrderPersonal ) {
         pastixOrderAlloc( ord, n, 0 );
@@ -365,10 +370,28 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
             }
         }
     }
+    else if ( iparm[IPARM_ORDERING] == PastixOrderScotch || iparm[IPARM_ORDERING] == PastixOrderPtScotch ||
+              iparm[IPARM_ORDERING] == PastixOrderMetis || iparm[IPARM_ORDERING] == PastixOrderParMetis ) {
+        /* Scotch and Metis are not part of this build: the built-in nested
+         * dissection orders the graph instead */
+        if ( iparm[IPARM_VERBOSE] > PastixVerboseNot ) {
+            px_print_warning( "pastix_subtask_order: ordering %d is served by the built-in nested dissection",
+                              (int)iparm[IPARM_ORDERING] );
+        }
+        if ( pd->schur_n > 0 ) {
+            /* the Schur unknowns isolated at the end (they are the last
+             * supernode, already in level order) */
+            int rc = px_order_with_schur( pd, spm, ord, px_order_nd );
+            if ( rc != PASTIX_SUCCESS ) {
+                return rc;
+            }
+        }
+        else if ( px_order_nd( pd->graph, ord ) != PASTIX_SUCCESS ) {
+            return PASTIX_ERR_INTERNAL;
+        }
+    }
     else {
-        px_print_error( "pastix_subtask_order: ordering %d is not available in this build (no Scotch/Metis); "
-                        "use PastixOrderPersonal with pastixOrderGrid or a permutation, or PastixIOLoad",
-                        (int)iparm[IPARM_ORDERING] );
+        px_print_error( "pastix_subtask_order: unknown ordering %d", (int)iparm[IPARM_ORDERING] );
         return PASTIX_ERR_BADPARAMETER;
     }
 
@@ -379,15 +402,38 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
         px_print_error( "pastix_subtask_order: invalid ordering" );
         return PASTIX_ERR_BADPARAMETER;
     }
-    px_order_apply_level_order( ord, iparm[IPARM_TASKS2D_LEVEL], iparm[IPARM_TASKS2D_WIDTH] );
+    if ( pd->schur_n > 0 && pd->schur_n <= n &&
+         ( iparm[IPARM_ORDERING] == PastixOrderPersonal || ( iparm[IPARM_IO_STRATEGY] & PastixIOLoad ) ) ) {
+        /* the user's ordering numbers the Schur unknowns last: they become
+         * the last supernode */
+        int rc = px_order_schur_boundary( pd, spm, ord );
+        if ( rc != PASTIX_SUCCESS ) {
+            return rc;
+        }
+    }
+    if ( pd->schur_n == 0 || iparm[IPARM_ORDERING] == PastixOrderPersonal || ( iparm[IPARM_IO_STRATEGY] & PastixIOLoad ) ) {
+        px_order_apply_level_order( ord, iparm[IPARM_TASKS2D_LEVEL], iparm[IPARM_TASKS2D_WIDTH] );
+    }
 
+    if ( pd->schur_n > 0 ) {
+        for ( pastix_int_t s = 0; s < pd->schur_n; s++ ) {
+            pastix_int_t v = pd->schur_list[s] - spm->baseval;
+            if ( v < 0 || v >= n || ord->permtab[v] < n - pd->schur_n ) {
+                px_print_error( "pastix_subtask_order: the Schur unknowns are not the last ones of the ordering" );
+                return PASTIX_ERR_BADPARAMETER;
+            }
+        }
+    }
     ord->sndenbr = ord->cblknbr;
     free( ord->sndetab );
     ord->sndetab = px_malloc( ( ord->sndenbr + 1 ) * sizeof( pastix_int_t ) );
     memcpy( ord->sndetab, ord->rangtab, ( ord->sndenbr + 1 ) * sizeof( pastix_int_t ) );
 
     if ( iparm[IPARM_IO_STRATEGY] & PastixIOSave ) {
-        pastixOrderSaveFile( ord, "ordergen" );
+        int rc = pastixOrderSave( pd, ord );
+        if ( rc != PASTIX_SUCCESS ) {
+            return rc;
+        }
     }
     /* return the ordering to the user for non-personal orderings */
     if ( iparm[IPARM_ORDERING] != PastixOrderPersonal && myorder != NULL ) {
@@ -403,29 +449,41 @@ pastix_subtask_order( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *
     return PASTIX_SUCCESS;
 }
 
-int
-pastix_subtask_symbfact( pastix_data_t *pd )
+/* The symbolic factorization: fax on the graph, or with load set and
+ * IPARM_IO_STRATEGY PastixIOLoad the file "symbname". */
+static int
+px_symbfact_run( pastix_data_t *pd, int load )
 {
-    double t0;
-    if ( pd == NULL ) {
-        px_print_error( "pastix_subtask_symbfact: wrong pastix_data parameter" );
-        return PASTIX_ERR_BADPARAMETER;
-    }
-    if ( !( pd->steps & STEP_ORDERING ) || pd->graph == NULL || pd->ordemesh == NULL ) {
-        px_print_error( "pastix_subtask_symbfact: pastix_subtask_order() has to be called before calling this function" );
-        return PASTIX_ERR_BADPARAMETER;
-    }
-    t0 = px_wtime();
+    double t0 = px_wtime();
+    int    rc;
     if ( pd->symbmtx ) {
         px_symbol_exit( pd->symbmtx );
     }
     else {
         pd->symbmtx = px_calloc( 1, sizeof( px_symbol_t ) );
     }
-    px_symbol_fax_direct( pd->symbmtx, pd->graph, pd->ordemesh );
+    if ( load && ( pd->iparm[IPARM_IO_STRATEGY] & PastixIOLoad ) ) {
+        if ( ( rc = px_symbol_load_default( pd->symbmtx ) ) != PASTIX_SUCCESS ) {
+            return rc;
+        }
+        if ( pd->symbmtx->nodenbr != pd->ordemesh->vertnbr ) {
+            px_print_error( "pastix_subtask_symbfact: the loaded symbol does not match the ordering" );
+            return PASTIX_ERR_BADPARAMETER;
+        }
+    }
+    else {
+        px_symbol_fax_direct( pd->symbmtx, pd->graph, pd->ordemesh );
+    }
     if ( px_symbol_check( pd->symbmtx ) != PASTIX_SUCCESS ) {
         px_print_error( "pastix_subtask_symbfact: inconsistent symbol matrix" );
         return PASTIX_ERR_INTERNAL;
+    }
+    /* the Schur part: the last schur_n columns (pastix_subtask_symbfact.c:238) */
+    pd->symbmtx->schurfcol = pd->symbmtx->nodenbr - ( pd->schur_n > 0 ? pd->schur_n : 0 );
+    if ( pd->iparm[IPARM_IO_STRATEGY] & PastixIOSave ) {
+        if ( ( rc = px_symbol_save_default( pd, pd->symbmtx ) ) != PASTIX_SUCCESS ) {
+            return rc;
+        }
     }
     px_symbol_flops( pd->symbmtx, (int)pd->iparm[IPARM_FLOAT], (int)pd->iparm[IPARM_FACTORIZATION],
                      &pd->dparm[DPARM_FACT_THFLOPS], &pd->dparm[DPARM_FACT_RLFLOPS] );
@@ -434,6 +492,26 @@ pastix_subtask_symbfact( pastix_data_t *pd )
     pd->steps &= ~( STEP_ANALYSE | STEP_CSC2BCSC | STEP_BCSC2CTAB | STEP_NUMFACT | STEP_SOLVE | STEP_REFINE );
     pd->steps |= STEP_SYMBFACT;
     return PASTIX_SUCCESS;
+}
+
+symbol_matrix_t *
+pastix_amd_symbol( pastix_data_t *pd )
+{
+    return pd ? pd->symbmtx : NULL;
+}
+
+int
+pastix_subtask_symbfact( pastix_data_t *pd )
+{
+    if ( pd == NULL ) {
+        px_print_error( "pastix_subtask_symbfact: wrong pastix_data parameter" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( !( pd->steps & STEP_ORDERING ) || pd->graph == NULL || pd->ordemesh == NULL ) {
+        px_print_error( "pastix_subtask_symbfact: pastix_subtask_order() has to be called before calling this function" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    return px_symbfact_run( pd, 1 );
 }
 
 int
@@ -453,10 +531,13 @@ pastix_subtask_reordering( pastix_data_t *pd )
     px_symbol_reordering( pd->symbmtx, pd->ordemesh, pd->iparm[IPARM_REORDERING_SPLIT],
                           pd->iparm[IPARM_REORDERING_STOP], pd->nthreads );
     if ( pd->iparm[IPARM_IO_STRATEGY] & PastixIOSave ) {
-        pastixOrderSaveFile( pd->ordemesh, "ordergen" );
+        if ( ( rc = pastixOrderSave( pd, pd->ordemesh ) ) != PASTIX_SUCCESS ) {
+            return rc;
+        }
     }
-    /* rebuild the symbolic structure with the new row order */
-    rc = pastix_subtask_symbfact( pd );
+    /* rebuild the symbolic structure with the new row order (from the graph,
+     * also when the first one was loaded from a file) */
+    rc = px_symbfact_run( pd, 0 );
     pd->dparm[DPARM_REORDER_TIME] = px_wtime() - t0;
     return rc;
 }
@@ -694,6 +775,11 @@ pastix_subtask_sopalin( pastix_data_t *pd )
     }
     pd->solvmatr->diagthreshold = threshold;
 
+    if ( ( rc = pxd_set_schur( pd->dev, pd->symbmtx ? pd->symbmtx->schurfcol : pd->solvmatr->nodenbr,
+                               iparm[IPARM_SCHUR_FACT_MODE] == PastixFactModeLocal ) ) != PASTIX_SUCCESS ) {
+        px_print_error( "pastix_subtask_sopalin: the Schur part does not start at a column block" );
+        return rc;
+    }
     t0 = px_wtime();
     {
         /* for real matrices LL^T is LL^H (POTRF) and LDL^H is LDL^T (SYTRF),
@@ -988,6 +1074,7 @@ px_rhs_apply( pastix_data_t *pd, pastix_rhs_t b, px_solve_fn_t fn, int arg )
     if ( rc != PASTIX_SUCCESS ) {
         return rc;
     }
+    pxd_set_option( pd->dev, "schur_solv_mode", (double)pd->iparm[IPARM_SCHUR_SOLV_MODE] );
     rc = fn( pd->dev, arg, b->n, dx, b->m );
     if ( rc == PASTIX_SUCCESS ) {
         rc = px_rhs_download( pd, dx, b );
@@ -1003,6 +1090,7 @@ pastix_amd_solve_dev( pastix_data_t *pd, pastix_int_t nrhs, void *dx, pastix_int
         px_print_error( "pastix_subtask_solve: All steps from pastix_task_init() to pastix_task_numfact() have to be called before calling this function" );
         return PASTIX_ERR_BADPARAMETER;
     }
+    pxd_set_option( pd->dev, "schur_solv_mode", (double)pd->iparm[IPARM_SCHUR_SOLV_MODE] );
     return pxd_solve( pd->dev, nrhs, dx, ldx );
 }
 
@@ -1164,6 +1252,11 @@ pastix_subtask_refine( pastix_data_t *pd, pastix_rhs_t Bp, pastix_rhs_t Xp )
          px_rhs_check( pd, Xp, "pastix_subtask_refine" ) != PASTIX_SUCCESS || Bp->n != Xp->n ) {
         return PASTIX_ERR_BADPARAMETER;
     }
+    if ( pd->schur_n > 0 && pd->iparm[IPARM_SCHUR_SOLV_MODE] != PastixSolvModeLocal ) {
+        px_print_error( "Refinement is not available with Schur complement when non local solve is required" );
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    pxd_set_option( pd->dev, "schur_solv_mode", (double)pd->iparm[IPARM_SCHUR_SOLV_MODE] );
     t0 = px_wtime();
     rc = px_rhs_upload( pd, Bp, &db );
     if ( rc == PASTIX_SUCCESS ) {

@@ -582,80 +582,3 @@ px_order_apply_level_order( pastix_order_t *order, pastix_int_t level_tasks2d,
     return PASTIX_SUCCESS;
 }
 
-/* ------------------------------------------------------------------------ */
-/* Ordering file: version, cblknbr, vertnbr, rangtab, permtab, treetab
- * (reference order/order_io.c:42-103 and :198-263).                        */
-
-int
-pastixOrderLoadFile( pastix_order_t *ord, const char *filename )
-{
-    FILE *f = fopen( filename, "r" );
-    long  vers, cblknbr, vertnbr, v;
-    int   ok = 1;
-
-    if ( f == NULL ) {
-        px_print_error( "pastixOrderLoad: cannot open %s", filename );
-        return PASTIX_ERR_FILE;
-    }
-    if ( fscanf( f, "%ld %ld %ld", &vers, &cblknbr, &vertnbr ) != 3 ||
-         ( vers != 0 && vers != 1 ) || cblknbr > vertnbr ) {
-        fclose( f );
-        px_print_error( "pastixOrderLoad: bad input (1)" );
-        return PASTIX_ERR_FILE;
-    }
-    pastixOrderAlloc( ord, vertnbr, cblknbr );
-    for ( long c = 0; ok && c <= cblknbr; c++ ) {
-        ok = ( fscanf( f, "%ld", &v ) == 1 );
-        ord->rangtab[c] = v;
-    }
-    for ( long i = 0; ok && i < vertnbr; i++ ) {
-        ok = ( fscanf( f, "%ld", &v ) == 1 );
-        ord->permtab[i] = v;
-    }
-    if ( vers == 1 ) {
-        for ( long c = 0; ok && c < cblknbr; c++ ) {
-            ok = ( fscanf( f, "%ld", &v ) == 1 );
-            ord->treetab[c] = v;
-        }
-    }
-    else {
-        free( ord->treetab );
-        ord->treetab = NULL;
-    }
-    fclose( f );
-    if ( !ok ) {
-        pastixOrderExit( ord );
-        px_print_error( "pastixOrderLoad: bad input (2)" );
-        return PASTIX_ERR_FILE;
-    }
-    ord->baseval = ord->rangtab[0];
-    for ( pastix_int_t i = 0; i < vertnbr; i++ ) {
-        ord->peritab[ord->permtab[i] - ord->baseval] = i + ord->baseval;
-    }
-    return PASTIX_SUCCESS;
-}
-
-int
-pastixOrderSaveFile( const pastix_order_t *ord, const char *filename )
-{
-    FILE *f;
-    if ( ord->permtab == NULL || ord->rangtab == NULL || ord->treetab == NULL ) {
-        return PASTIX_ERR_BADPARAMETER;
-    }
-    f = fopen( filename, "w" );
-    if ( f == NULL ) {
-        return PASTIX_ERR_FILE;
-    }
-    fprintf( f, "1\n%ld\t%ld\n", (long)ord->cblknbr, (long)ord->vertnbr );
-    for ( pastix_int_t c = 0; c <= ord->cblknbr; c++ ) {
-        fprintf( f, "%ld%c", (long)ord->rangtab[c], ( ( c & 7 ) == 7 || c == ord->cblknbr ) ? '\n' : '\t' );
-    }
-    for ( pastix_int_t i = 0; i < ord->vertnbr; i++ ) {
-        fprintf( f, "%ld%c", (long)ord->permtab[i], ( ( i & 7 ) == 7 || i == ord->vertnbr - 1 ) ? '\n' : '\t' );
-    }
-    for ( pastix_int_t c = 0; c < ord->cblknbr; c++ ) {
-        fprintf( f, "%ld%c", (long)ord->treetab[c], ( ( c & 7 ) == 7 || c == ord->cblknbr - 1 ) ? '\n' : '\t' );
-    }
-    fclose( f );
-    return PASTIX_SUCCESS;
-}

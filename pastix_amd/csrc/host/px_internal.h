@@ -61,6 +61,8 @@ void px_graph_exit( px_graph_t *graph );
 
 /* ----------------------------------------------------------------------- */
 /* Order helpers (order.c).                                                 */
+int px_order_nd( const px_graph_t *graph, pastix_order_t *order );
+void px_order_fundamental_supernodes( const px_graph_t *g, pastix_order_t *ord );
 int px_order_apply_level_order( pastix_order_t *order, pastix_int_t level_tasks2d,
                                 pastix_int_t width_tasks2d );
 
@@ -195,10 +197,24 @@ struct pastix_data_s {
     int              refine_nrhs_batch;   /* 1: reference behaviour (one RHS at a time) */
     void            *ref_cache[PX_REF_CACHE];       /* device buffers kept for the refinement */
     size_t           ref_cache_bytes[PX_REF_CACHE];
+    pastix_int_t     schur_n;             /* Schur unknowns (pastixSetSchurUnknownList)       */
+    pastix_int_t    *schur_list;
     pastix_model_t  *cpu_models;          /* kernel cost models (common/pastixdata.h:106-107) */
     pastix_model_t  *gpu_models;
 };
 
 int px_device_ready( pastix_data_t *pd );
+
+/* schur.c */
+int px_order_with_schur( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *ord,
+                         int ( *order_fn )( const px_graph_t *, pastix_order_t * ) );
+int px_order_schur_boundary( pastix_data_t *pd, const spmatrix_t *spm, pastix_order_t *ord );
+
+/* io.c */
+const char *px_gendirectories( pastix_data_t *pd );
+int  px_symbol_load_default( px_symbol_t *symb );
+int  px_symbol_save_default( pastix_data_t *pd, const px_symbol_t *symb );
+void px_symbol_expand( px_symbol_t *symb, pastix_int_t dof );
+int  px_spm_expand( spmatrix_t *spm );
 
 #endif /* PX_INTERNAL_H */

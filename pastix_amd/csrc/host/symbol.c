@@ -179,6 +179,10 @@ px_symbol_fax_direct( px_symbol_t *s, const px_graph_t *g, const pastix_order_t 
         pastix_int_t fcol = ord->rangtab[c], lcol = ord->rangtab[c + 1] - 1;
         pastix_int_t nr = 0;
 
+        /* the bloks of cblk c start here: this also closes the blok range of
+         * cblk c - 1, which may be one of the children read below */
+        s->cblktab[c].bloknum = bloknbr;
+
         /* pattern of A below the cblk */
         for ( pastix_int_t j = fcol; j <= lcol; j++ ) {
             pastix_int_t v = ord->peritab[j];

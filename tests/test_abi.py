@@ -60,9 +60,45 @@ def test_step_order_errors_without_gpu():
     px.finalize(pd)
 
 
-def test_scotch_ordering_is_refused_cleanly():
-    iparm, dparm = px.initParam()
-    pd = px.init(iparm, dparm)
-    spm = px.Spm.laplacian7(3, 3, 3)
-    assert px.lib.pastix_task_analyze(pd, ctypes.byref(spm.s)) == px.PASTIX_ERR_BADPARAMETER
-    px.finalize(pd)
+def test_scotch_and_metis_orderings_run_the_built_in_nested_dissection():
+    for ordering in (px.PastixOrderScotch, px.PastixOrderMetis):
+        iparm, dparm = px.initParam()
+        iparm[px.IPARM_ORDERING] = ordering
+        pd = px.init(iparm, dparm)
+        spm = px.Spm.laplacian7(3, 3, 3)
+        assert px.lib.pastix_task_analyze(pd, ctypes.byref(spm.s)) == px.PASTIX_SUCCESS
+        px.finalize(pd)
+
+
+def _reference_prototypes():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    try:
+        import make_reference_api
+    finally:
+        sys.path.pop(0)
+    ours = {}
+    for h in ("include/pastix.h", "include/pastix/order.h"):
+        ours.update(make_reference_api.prototypes(open(os.path.join(ROOT, h)).read()))
+    ref = {}
+    with open(os.path.join(ROOT, "tests", "golden", "reference_api.txt")) as f:
+        for line in f:
+            line = line.strip()
+            if line:
+                ref[line.split("|")[0]] = line
+    return ref, ours
+
+
+def test_public_prototypes_match_the_reference_headers():
+    # every prototype of the reference's include/pastix.h and
+    # include/pastix/order.h, with the same return and parameter types
+    ref, ours = _reference_prototypes()
+    assert len(ref) >= 45
+    diff = ["%s != %s" % (ours.get(n), v) for n, v in sorted(ref.items()) if ours.get(n) != v]
+    assert not diff, diff
+
+
+def test_reference_functions_are_exported():
+    ref, _ = _reference_prototypes()
+    missing = [n for n in ref if not hasattr(px.lib, n)]
+    assert not missing, missing
