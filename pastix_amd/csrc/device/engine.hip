@@ -392,7 +392,9 @@ build_sf_schedule( pxd_ctx_t *ctx )
     PXD_CHECK( hipMalloc( (void **)&ctx->dsf_cnt, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
     PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
     PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
-    ctx->sf_epoch = 0;
+    // the counters start from zero again: so do the passes of both sweeps
+    ctx->sf_epoch    = 0;
+    ctx->sf_epoch_bw = 0;
     return 0;
 }
 

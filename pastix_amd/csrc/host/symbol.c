@@ -486,6 +486,224 @@ hamming( const hvec_t *h, pastix_int_t xi, pastix_int_t xj, pastix_int_t stop )
     return ( sum >= stop ) ? stop : sum;
 }
 
+/*
+ * Distances from one row (`cur`) to the others, in O(1) each.
+ *
+ * The lists hold cblk numbers in increasing order, so for two rows
+ * |A xor B| = |A| + |B| - 2 |A and B|.  The common counts of the current row
+ * with every earlier row come from inverted lists (the rows that hold each
+ * cblk number, in increasing order), at a cost of the lengths of those lists
+ * rather than one merge per pair: the insertion compares every new row with
+ * every row placed so far, O(size^2) merges for the widest separators.
+ * min(|A xor B|, stop) is what the merge of hamming() returns.  The upper
+ * lists are compared at stop 1 only (equal or not), so each row gets the
+ * number of its class of equal lists.  Lists that are not strictly
+ * increasing fall back to hamming().
+ */
+typedef struct tsp_rows_s {
+    const hvec_t *lw, *up;
+    pastix_int_t  stop;
+    int           fast;
+    pastix_int_t  cur;
+    pastix_int_t *gptr, *grow;   /* rows of each cblk number (CSR by group)  */
+    pastix_int_t *rptr, *rgrp;   /* groups of each row's lower list (CSR)    */
+    pastix_int_t *common;        /* |lw[cur] and lw[x]| for the rows x < cur */
+    pastix_int_t *ucls;          /* class of each row's upper list           */
+    pastix_int_t  uempty;        /* class of the empty list (the ghost's)    */
+} tsp_rows_t;
+
+static int
+increasing( const hvec_t *h, pastix_int_t size )
+{
+    for ( pastix_int_t i = 0; i < size; i++ ) {
+        for ( pastix_int_t k = 1; k < h->sz[i]; k++ ) {
+            if ( h->v[i][k - 1] >= h->v[i][k] ) {
+                return 0;
+            }
+        }
+    }
+    return 1;
+}
+
+static int
+cmp_pair( const void *x, const void *y )
+{
+    const pastix_int_t *a = (const pastix_int_t *)x, *b = (const pastix_int_t *)y;
+    if ( a[0] != b[0] ) {
+        return ( a[0] > b[0] ) - ( a[0] < b[0] );
+    }
+    return ( a[1] > b[1] ) - ( a[1] < b[1] );
+}
+
+static void
+tsp_rows_init( tsp_rows_t *t, pastix_int_t size, const hvec_t *lw, const hvec_t *up, pastix_int_t stop )
+{
+    pastix_int_t ne = 0, ng = 0, *pr, *hc;
+    memset( t, 0, sizeof( *t ) );
+    t->lw   = lw;
+    t->up   = up;
+    t->stop = stop;
+    t->cur  = -1;
+    t->fast = increasing( lw, size ) && increasing( up, size );
+    if ( !t->fast ) {
+        return;
+    }
+    for ( pastix_int_t i = 0; i < size; i++ ) {
+        ne += lw->sz[i];
+    }
+    /* (cblk number, row) pairs sorted: groups of equal numbers, rows rising */
+    pr = px_malloc( 2 * ( ne > 0 ? ne : 1 ) * sizeof( pastix_int_t ) );
+    ne = 0;
+    for ( pastix_int_t i = 0; i < size; i++ ) {
+        for ( pastix_int_t k = 0; k < lw->sz[i]; k++ ) {
+            pr[2 * ne]     = lw->v[i][k];
+            pr[2 * ne + 1] = i;
+            ne++;
+        }
+    }
+    qsort( pr, ne, 2 * sizeof( pastix_int_t ), cmp_pair );
+    t->gptr   = px_malloc( ( ne + 1 ) * sizeof( pastix_int_t ) );
+    t->grow   = px_malloc( ( ne > 0 ? ne : 1 ) * sizeof( pastix_int_t ) );
+    t->rptr   = px_calloc( size + 1, sizeof( pastix_int_t ) );
+    t->rgrp   = px_malloc( ( ne > 0 ? ne : 1 ) * sizeof( pastix_int_t ) );
+    t->common = px_calloc( size > 0 ? size : 1, sizeof( pastix_int_t ) );
+    for ( pastix_int_t p = 0; p < ne; p++ ) {
+        if ( p == 0 || pr[2 * p] != pr[2 * p - 2] ) {
+            t->gptr[ng++] = p;
+        }
+        t->grow[p] = pr[2 * p + 1];
+        t->rptr[pr[2 * p + 1] + 1]++;
+    }
+    t->gptr[ng] = ne;
+    for ( pastix_int_t i = 0; i < size; i++ ) {
+        t->rptr[i + 1] += t->rptr[i];
+    }
+    {
+        pastix_int_t *fill = px_malloc( ( size > 0 ? size : 1 ) * sizeof( pastix_int_t ) );
+        memcpy( fill, t->rptr, size * sizeof( pastix_int_t ) );
+        for ( pastix_int_t g = 0; g < ng; g++ ) {
+            for ( pastix_int_t p = t->gptr[g]; p < t->gptr[g + 1]; p++ ) {
+                t->rgrp[fill[t->grow[p]]++] = g;
+            }
+        }
+        free( fill );
+    }
+    free( pr );
+
+    /* classes of equal upper lists: sort the rows by (hash, row), then split
+     * each run of equal hashes by comparing the lists themselves */
+    t->ucls = px_malloc( ( size > 0 ? size : 1 ) * sizeof( pastix_int_t ) );
+    hc      = px_malloc( 2 * ( size > 0 ? size : 1 ) * sizeof( pastix_int_t ) );
+    for ( pastix_int_t i = 0; i < size; i++ ) {
+        uint64_t h = 1469598103934665603ull ^ (uint64_t)up->sz[i];
+        for ( pastix_int_t k = 0; k < up->sz[i]; k++ ) {
+            h = ( h ^ (uint64_t)up->v[i][k] ) * 1099511628211ull;
+        }
+        hc[2 * i]     = (pastix_int_t)( h >> 1 );
+        hc[2 * i + 1] = i;
+    }
+    qsort( hc, size, 2 * sizeof( pastix_int_t ), cmp_pair );
+    {
+        pastix_int_t ncls = 0, r0 = 0;
+        t->uempty = -1;
+        while ( r0 < size ) {
+            pastix_int_t r1 = r0, first_cls = ncls;
+            while ( r1 < size && hc[2 * r1] == hc[2 * r0] ) {
+                r1++;
+            }
+            for ( pastix_int_t r = r0; r < r1; r++ ) {
+                pastix_int_t x = hc[2 * r + 1], c = -1;
+                /* the representatives of this run are its earlier rows that
+                 * opened a class */
+                for ( pastix_int_t q = r0; q < r && c < 0; q++ ) {
+                    pastix_int_t y = hc[2 * q + 1];
+                    if ( t->ucls[y] >= first_cls && up->sz[x] == up->sz[y] &&
+                         memcmp( up->v[x], up->v[y], up->sz[x] * sizeof( pastix_int_t ) ) == 0 ) {
+                        c = t->ucls[y];
+                    }
+                }
+                t->ucls[x] = ( c >= 0 ) ? c : ncls++;
+                if ( up->sz[x] == 0 ) {
+                    t->uempty = t->ucls[x];
+                }
+            }
+            r0 = r1;
+        }
+        if ( t->uempty < 0 ) {
+            t->uempty = ncls;   /* no row has an empty list: the ghost is alone */
+        }
+    }
+    free( hc );
+}
+
+static void
+tsp_rows_exit( tsp_rows_t *t )
+{
+    free( t->gptr );
+    free( t->grow );
+    free( t->rptr );
+    free( t->rgrp );
+    free( t->common );
+    free( t->ucls );
+}
+
+/* counts of the row i with the rows before it (add 1, or clear with 0) */
+static void
+tsp_rows_count( tsp_rows_t *t, pastix_int_t i, int clear )
+{
+    for ( pastix_int_t q = t->rptr[i]; q < t->rptr[i + 1]; q++ ) {
+        pastix_int_t g = t->rgrp[q];
+        for ( pastix_int_t p = t->gptr[g]; p < t->gptr[g + 1] && t->grow[p] < i; p++ ) {
+            if ( clear ) {
+                t->common[t->grow[p]] = 0;
+            }
+            else {
+                t->common[t->grow[p]]++;
+            }
+        }
+    }
+}
+
+/* the current row becomes i (its distances to the rows before it) */
+static void
+tsp_rows_set( tsp_rows_t *t, pastix_int_t i )
+{
+    if ( !t->fast ) {
+        t->cur = i;
+        return;
+    }
+    if ( t->cur >= 0 ) {
+        tsp_rows_count( t, t->cur, 1 );
+    }
+    t->cur = i;
+    tsp_rows_count( t, i, 0 );
+}
+
+/* hamming( lw, cur, x, stop ) for a row x before cur or the ghost */
+static inline pastix_int_t
+tsp_lw( const tsp_rows_t *t, pastix_int_t x )
+{
+    pastix_int_t d;
+    if ( !t->fast ) {
+        return hamming( t->lw, t->cur, x, t->stop );
+    }
+    if ( x == -1 ) {
+        return t->lw->sz[t->cur];
+    }
+    d = t->lw->sz[t->cur] + t->lw->sz[x] - 2 * t->common[x];
+    return ( d >= t->stop ) ? t->stop : d;
+}
+
+/* hamming( up, cur, x, 1 ) < 1: the upper lists are equal */
+static inline int
+tsp_up_equal( const tsp_rows_t *t, pastix_int_t x )
+{
+    if ( !t->fast ) {
+        return hamming( t->up, t->cur, x, 1 ) < 1;
+    }
+    return t->ucls[t->cur] == ( ( x == -1 ) ? t->uempty : t->ucls[x] );
+}
+
 /* Cheapest insertion of the rows into a cycle that starts with a ghost row;
  * writes the resulting order of the rows into `perm` (size entries). */
 static void
@@ -495,17 +713,20 @@ reorder_tsp( pastix_int_t size, const hvec_t *lw, const hvec_t *up, pastix_int_t
     pastix_int_t *inv = px_malloc( ( size + 1 ) * sizeof( pastix_int_t ) );
     pastix_int_t *len = px_calloc( size + 1, sizeof( pastix_int_t ) );
     pastix_int_t  d, elected = 0;
+    tsp_rows_t    t;
 
+    tsp_rows_init( &t, size, lw, up, stop );
     inv[0] = -1;
     inv[1] = 0;
-    d      = hamming( lw, 0, -1, stop );
+    d      = lw->sz[0];   /* row 0 to the ghost */
     len[0] = d;
     len[1] = d;
 
     for ( pastix_int_t i = 1; i < size; i++ ) {
-        pastix_int_t lw_before = hamming( lw, i, inv[0], stop );
-        pastix_int_t lw_after  = hamming( lw, i, inv[1], stop );
-        pastix_int_t up_after  = hamming( up, i, inv[1], 1 );
+        tsp_rows_set( &t, i );
+        pastix_int_t lw_before = tsp_lw( &t, inv[0] );
+        pastix_int_t lw_after  = tsp_lw( &t, inv[1] );
+        pastix_int_t up_after  = tsp_up_equal( &t, inv[1] ) ? 0 : 1;
         pastix_int_t minl      = lw_before + lw_after - len[0];
         pastix_int_t mpos      = 1;
         pastix_int_t min_cut   = -1;
@@ -513,15 +734,15 @@ reorder_tsp( pastix_int_t size, const hvec_t *lw, const hvec_t *up, pastix_int_t
 
         for ( pastix_int_t j = 1; j < i; j++ ) {
             pastix_int_t up_before = up_after;
-            up_after = hamming( up, i, inv[j + 1], 1 );
+            up_after = tsp_up_equal( &t, inv[j + 1] ) ? 0 : 1;
             if ( up_before < 1 || up_after < 1 ) {
                 if ( lw_after == -1 ) {
-                    lw_before = hamming( lw, i, inv[j], stop );
+                    lw_before = tsp_lw( &t, inv[j] );
                 }
                 else {
                     lw_before = lw_after;
                 }
-                lw_after = hamming( lw, i, inv[j + 1], stop );
+                lw_after = tsp_lw( &t, inv[j + 1] );
                 l        = lw_before + lw_after - len[j];
                 if ( l == minl ) {
                     if ( lw_before < min_cut ) {
@@ -555,10 +776,10 @@ reorder_tsp( pastix_int_t size, const hvec_t *lw, const hvec_t *up, pastix_int_t
             }
         }
 
-        first     = hamming( lw, i, inv[0], stop );
-        last      = hamming( lw, i, inv[i], stop );
-        lw_before = hamming( lw, i, inv[mpos - 1], stop );
-        lw_after  = hamming( lw, i, inv[mpos], stop );
+        first     = tsp_lw( &t, inv[0] );
+        last      = tsp_lw( &t, inv[i] );
+        lw_before = tsp_lw( &t, inv[mpos - 1] );
+        lw_after  = tsp_lw( &t, inv[mpos] );
         l         = first + last - len[i];
         if ( l < minl ) {
             minl = l;
@@ -591,6 +812,7 @@ reorder_tsp( pastix_int_t size, const hvec_t *lw, const hvec_t *up, pastix_int_t
     for ( pastix_int_t i = 0; i < size; i++ ) {
         perm[i] = inv[( i + 1 + elected ) % ( size + 1 )];
     }
+    tsp_rows_exit( &t );
     free( inv );
     free( len );
 }
