@@ -166,6 +166,22 @@ int pxd_get_coeftab_range( pxd_ctx_t *ctx, int side, int64_t offset, int64_t cou
 int pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx );
 
 /*
+ * pxd_solve without any allocation: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * sweeps' scratch (one pass of at most 8 columns of the solve's vectors,
+ * plus their single-precision copies for a single factor) lives in a static
+ * device array of PXD_NOALLOC_BYTES per GPU, held until the process ends; a
+ * solve that needs more returns 7 (PASTIX_ERR_BADPARAMETER) with a message,
+ * and pxd_solve is then the way.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (from several threads or instances) race.
+ * Same arguments, results and return codes as pxd_solve otherwise; no solve
+ * time is recorded in the statistics.
+ */
+#define PXD_NOALLOC_BYTES ( (size_t)32 << 20 )
+int pxd_solve_noalloc( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx );
+
+/*
  * x <- op(A)^{-1} x with op(A) = A (trans 0) or A^T (trans 1): for an LU
  * factor U^T y = b then L^T x = y (reference sopalin/pastix_task_solve.c:
  * 510-677, the transA cases of its table); Cholesky and LDL^T factors are

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <string>
 #include <limits>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 #include "kernels_solve.hpp"
@@ -1187,7 +1188,8 @@ enum { SOLVE_FWD = 1, SOLVE_DIAG = 2, SOLVE_BWD = 4, SOLVE_ALL = 7 };
 // reference sopalin/pastix_task_solve.c:597-601).
 template <typename T, typename V>
 int
-solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL, bool swap = false )
+solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL, bool swap = false,
+              V *Yscr = nullptr )
 {
     // forward sweep: L (unit for LU and LDL^T); with swap, U^T
     const T    *L  = (const T *)( swap ? ctx->dU : ctx->dL );
@@ -1241,13 +1243,17 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
         //             update reads x_k from Y and subtracts into X;
         // backward -- the gather reads the solved rows from X and subtracts
         //             into Y; the diagonal reads Y and writes X.
-        size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( V );
-        if ( ctx->dsolvy_bytes < need ) {
-            if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
-            PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
-            ctx->dsolvy_bytes = need;
+        // Y: the instance's buffer (allocated on first use), or the scratch
+        // the caller hands over (n x nrhs entries)
+        if ( Yscr == nullptr ) {
+            size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( V );
+            if ( ctx->dsolvy_bytes < need ) {
+                if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
+                PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
+                ctx->dsolvy_bytes = need;
+            }
         }
-        V            *Y   = (V *)ctx->dsolvy;
+        V            *Y   = Yscr ? Yscr : (V *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
         // levels [0, nlv) run level by level, the others in one launch per
         // sweep (passes of at most PXD_SF_NR right-hand sides)
@@ -1438,14 +1444,15 @@ set_lds_attributes( size_t lds )
 // mixed solve does.
 template <typename TV, typename T>
 int
-solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = SOLVE_ALL, bool swap = false )
+solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = SOLVE_ALL, bool swap = false,
+               void *scr = nullptr )
 {
     int rc = 0;
     if constexpr ( std::is_same<TV, T>::value ) {
         const int pass = solve_pass_rhs<T>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq, what, swap );
+            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq, what, swap, (T *)scr );
         }
         return rc;
     }
@@ -1453,26 +1460,35 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = S
         const int pass = solve_pass_rhs<TV>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq, what, swap );
+            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq, what, swap, (TV *)scr );
         }
         return rc;
     }
     else {
         const int pass = solve_pass_rhs<T>( ctx );
         size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, pass ) * sizeof( T );
-        if ( ctx->dvecs_bytes < need ) {
-            if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
-            PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
-            ctx->dvecs_bytes = need;
+        // the factor-precision copies: the instance's buffer, or the start of
+        // the caller's scratch with the sweeps' Y after them
+        T *xs = (T *)scr;
+        T *ys = nullptr;
+        if ( scr == nullptr ) {
+            if ( ctx->dvecs_bytes < need ) {
+                if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
+                PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
+                ctx->dvecs_bytes = need;
+            }
+            xs = (T *)ctx->dvecs;
         }
-        T *xs = (T *)ctx->dvecs;
+        else {
+            ys = (T *)( (char *)scr + ( ( need + 255 ) & ~(size_t)255 ) );
+        }
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
             }
-            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq, what, swap );
+            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq, what, swap, ys );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<TV, T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, xs + q * ctx->n, (TV *)dx + ( q0 + q ) * ldx );
@@ -1529,6 +1545,69 @@ solve_steps( pxd_ctx_t *ctx, int what, bool swap, int64_t nrhs, void *dx, int64_
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve (pxd_solve_noalloc): the sweeps'
+// out-of-place vectors and, for a single-precision factor, the copies of the
+// right-hand sides in the factor precision.  One copy of the array per GPU,
+// held until the process ends; all calls on one GPU share it.  Host code
+// never passes the array's name to a kernel (on the host it only stands for
+// the symbol): it uses the device address that hipGetSymbolAddress returns
+// for each device, kept in scratch_ptr.
+}  // namespace
+
+static __device__ __align__( 256 ) unsigned char scratch[PXD_NOALLOC_BYTES];
+
+namespace {
+
+std::mutex          scratch_mu;
+std::vector<void *> scratch_ptr;   // per device ordinal
+
+// Once per device, with that device current (from pxd_create).
+int
+scratch_setup( int device )
+{
+    std::lock_guard<std::mutex> lk( scratch_mu );
+    if ( (int)scratch_ptr.size() <= device ) {
+        scratch_ptr.resize( device + 1, nullptr );
+    }
+    if ( scratch_ptr[device] != nullptr ) {
+        return 0;
+    }
+    void      *ptr = nullptr;
+    hipError_t e   = hipGetSymbolAddress( (void **)&ptr, HIP_SYMBOL( scratch ) );
+    if ( e != hipSuccess || ptr == nullptr ) {
+        (void)hipGetLastError();
+        return 6;
+    }
+    scratch_ptr[device] = ptr;
+    return 0;
+}
+
+void *
+scratch_address( int device )
+{
+    std::lock_guard<std::mutex> lk( scratch_mu );
+    return ( device >= 0 && device < (int)scratch_ptr.size() ) ? scratch_ptr[device] : nullptr;
+}
+
+// Bytes of scratch a solve of nrhs columns takes: Y of one pass (at most
+// PXD_MAXRHS columns) in the sweeps' vector type, plus for a single factor
+// on double vectors the factor-precision copies (256-byte aligned).
+size_t
+noalloc_need( const pxd_ctx_t *ctx, int64_t nrhs )
+{
+    const size_t cols = (size_t)std::min<int64_t>( nrhs, PXD_MAXRHS );
+    const size_t n    = (size_t)ctx->n;
+    auto         al   = []( size_t b ) { return ( b + 255 ) & ~(size_t)255; };
+    const bool   wide = ctx->kernels && ctx->solve_vec_double;
+    switch ( ctx->fflt ) {
+    case FT_DOUBLE: return n * cols * sizeof( double );
+    case FT_CPLX64: return n * cols * sizeof( pxcd );
+    case FT_FLOAT: return wide ? n * cols * sizeof( double ) : al( n * cols * sizeof( float ) ) + n * cols * sizeof( float );
+    default: return wide ? n * cols * sizeof( pxcd ) : al( n * cols * sizeof( pxcf ) ) + n * cols * sizeof( pxcf );
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1567,6 +1646,11 @@ pxd_create( pxd_ctx_t **pctx, int device )
     ctx->lds_max = prop.sharedMemPerBlock;
     PXD_CHECK( hipMalloc( (void **)&ctx->dscratch, 64 * sizeof( int ) ) );
     PXD_CHECK( hipMalloc( &ctx->dred, 4096 ) );
+    if ( scratch_setup( device ) != 0 ) {
+        // every other entry point works without it
+        fprintf( stderr, "pastix_amd device: no address for the allocation-free solve's scratch on device %d; "
+                         "pxd_solve_noalloc will refuse\n", device );
+    }
     set_lds_attributes<double>( ctx->lds_max );
     set_lds_attributes<float>( ctx->lds_max );
     set_lds_attributes_mfma<pxcd>( ctx->lds_max );
@@ -2004,6 +2088,49 @@ int
 pxd_solve( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
 {
     return solve_steps( ctx, SOLVE_ALL, false, nrhs, dx, ldx );
+}
+
+int
+pxd_solve_noalloc( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
+{
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    if ( ctx->dL == nullptr || nrhs < 0 || ( nrhs > 0 && ( dx == nullptr || ldx < ctx->n ) ) ) {
+        return 7;
+    }
+    if ( nrhs == 0 || ctx->n == 0 ) {
+        return 0;
+    }
+    if ( ctx->schurfcol < ctx->n && ctx->schur_solv_mode == 2 && ctx->schur_fact_local ) {
+        return 7;   // the Schur part was not factorized
+    }
+    void *scr = scratch_address( ctx->device );
+    if ( scr == nullptr ) {
+        fprintf( stderr, "pxd_solve_noalloc: the scratch of device %d has no address\n", ctx->device );
+        return 6;
+    }
+    const size_t need = noalloc_need( ctx, nrhs );
+    if ( need > (size_t)PXD_NOALLOC_BYTES ) {
+        fprintf( stderr,
+                 "pxd_solve_noalloc: a solve of %lld unknowns x %lld right-hand sides needs %zu bytes of scratch, "
+                 "more than its %zu (use pxd_solve)\n",
+                 (long long)ctx->n, (long long)nrhs, need, (size_t)PXD_NOALLOC_BYTES );
+        return 7;
+    }
+    int rc = 0;
+    switch ( ctx->fflt ) {
+    case FT_DOUBLE: rc = solve_vectors<double, double>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
+    case FT_FLOAT: rc = solve_vectors<double, float>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
+    case FT_CPLX64: rc = solve_vectors<pxcd, pxcd>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
+    default: rc = solve_vectors<pxcd, pxcf>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
+    }
+    if ( ctx->kernels && ctx->solve_sf ) {
+        const int rc2 = sf_check( ctx );   // also waits for the solve
+        rc = rc ? rc : rc2;
+    }
+    else if ( hipStreamSynchronize( ctx->stream ) != hipSuccess ) {
+        rc = rc ? rc : 6;
+    }
+    return rc;
 }
 
 int
