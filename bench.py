@@ -8,6 +8,14 @@ right-hand sides already resident in HBM.  The analysis (grid nested
 dissection, symbolic factorization, reordering, split) is done once before
 timing, as the reference's ordering_grid example does.
 
+The line also carries the supernodal-update rooflines of both paths
+(matrix-core peak), the HBM rates of one solve and one SpMV, the forward
+error of the FP32+IR solution against the FP64 factorization's, and the
+CPU baseline: the oracle's static-scheduler FP64 Cholesky + solve of the
+same 100^3 problem on the host cores this process may use (affinity mask
+capped by the cgroup quota), after a small warm-up factorization
+(`--cpu-reps` timed repetitions, default 1, about 20 s on 16 cores).
+
 Prints one JSON line (the driver's contract).  `--gpus` above 1 is refused:
 the engine drives one MI355X.
 """
@@ -36,7 +44,10 @@ def parse():
     ap.add_argument("--n", type=int, default=100, help="grid edge (100 -> 1,000,000 unknowns)")
     ap.add_argument("--nrhs", type=int, default=4)
     ap.add_argument("--berr", type=float, default=1e-12)
-    ap.add_argument("--cpu-sample", type=int, default=60, help="grid edge of the CPU baseline sample")
+    ap.add_argument("--cpu-grid", type=int, default=None,
+                    help="grid edge of the CPU baseline (default: the workload's own, --n)")
+    ap.add_argument("--cpu-reps", type=int, default=1,
+                    help="timed repetitions of the CPU baseline (after a small warm-up factorization)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
@@ -160,37 +171,134 @@ def measure(run, steps, warmup):
                 berr=float(berr), ferr=ferr, x=x)
 
 
-def cpu_baseline(px, nsample, nrhs):
-    """The oracle's static-scheduler Cholesky + solve on the host cores, on a
-    bounded sample of the workload (a nsample^3 Laplacian, same pipeline)."""
+def host_cores():
+    """Threads for the CPU baseline: the CPUs of this process's affinity mask,
+    capped by the cgroup CPU quota when one is set (a GPU box grants each GPU
+    a share of its host cores).  Returns (threads, affinity, quota, model)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return (min(aff, quota) if quota else aff), aff, quota, model
+
+
+def cpu_baseline(px, ngrid, nrhs, reps):
+    """The oracle's static-scheduler FP64 Cholesky + nrhs-RHS solve of the
+    ngrid^3 Laplacian on the host cores (same analysis as the GPU paths):
+    one small warm-up factorization, then reps timed repetitions (min and
+    median)."""
+    import gc
     import oracle_ref as orc
-    spm = px.Spm.laplacian7(nsample, nsample, nsample)
-    iparm, dparm = px.initParam()
-    iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
-    iparm[px.IPARM_FACTORIZATION] = px.PastixFactPOTRF
-    pd = px.init(iparm, dparm)
-    px.analyze_grid(pd, spm, nsample, nsample, nsample)
-    px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s))
+    threads, aff, quota, model = host_cores()
+
+    def setup(g):
+        spm = px.Spm.laplacian7(g, g, g)
+        iparm, dparm = px.initParam()
+        iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
+        iparm[px.IPARM_FACTORIZATION] = px.PastixFactPOTRF
+        iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
+        pd = px.init(iparm, dparm)
+        px.analyze_grid(pd, spm, g, g, g)
+        px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s))
+        tau = np.finfo(np.float64).eps / 2 * dparm[px.DPARM_A_NORM]
+        return spm, pd, dparm, tau
+
+    spm, pd, dparm, tau = setup(min(ngrid, 30))
+    orc.OracleFactor(px.solver_export(pd), px.bcsc_export(pd), tau, np.float64, nthreads=threads)
+    px.finalize(pd)
+    spm, pd, dparm, tau = setup(ngrid)
     s = px.solver_export(pd)
     bc = px.bcsc_export(pd)
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    tau = np.finfo(np.float64).eps / 2 * dparm[px.DPARM_A_NORM]
-    F = orc.OracleFactor(s, bc, tau, np.float64, nthreads=cores)
     fl = dparm[px.DPARM_FACT_THFLOPS]
     _, b = spm.gen_rhs(nrhs, seed=0)
     perm = px.order_export(pd)[0]
     bp = np.zeros_like(b)
     bp[perm] = b
-    t0 = time.perf_counter()
-    F.solve(bp)
-    tsolve = time.perf_counter() - t0
     px.finalize(pd)
-    return dict(value=fl / F.seconds / 1e9, unit="GFLOP/s", cores=cores, kind="port",
-                sample="FP64 Cholesky of the %d^3 7-point Laplacian (%d unknowns, %.3e flop) with the "
-                       "oracle's static scheduler (%s): factorization %.2f s, %d-RHS solve %.2f s"
-                       % (nsample, nsample ** 3, fl, orc.blas_name(), F.seconds, nrhs, tsolve),
-                fact_seconds=F.seconds, solve_seconds=tsolve, thflops=fl)
+    tf, ts = [], []
+    for r in range(max(1, reps)):
+        F = orc.OracleFactor(s, bc, tau, np.float64, nthreads=threads)
+        t0 = time.perf_counter()
+        F.solve(bp)
+        ts.append(time.perf_counter() - t0)
+        tf.append(F.seconds)
+        log("cpu rep %d: factorization %.2f s, solve %.2f s" % (r, tf[-1], ts[-1]))
+        del F
+        gc.collect()
+    tts = [a + c for a, c in zip(tf, ts)]
+    med = float(np.median(tts))
+    return dict(value=fl / float(np.median(tf)) / 1e9, unit="GFLOP/s", cores=threads, kind="port",
+                sample="FP64 Cholesky + %d-RHS solve of the %d^3 7-point Laplacian (%d unknowns, %.3e flop), "
+                       "the oracle's static scheduler with %s on %d threads (%s; affinity %d CPUs, cgroup quota "
+                       "%s); warm-up then %d repetition(s): factorization median %.2f s min %.2f s, solve median "
+                       "%.2f s (the oracle's sequential sweeps), time-to-solution median %.2f s min %.2f s"
+                       % (nrhs, ngrid, ngrid ** 3, fl, orc.blas_name(), threads, model, aff,
+                          quota if quota else "none", len(tf), float(np.median(tf)), min(tf),
+                          float(np.median(ts)), med, min(tts)),
+                fact_seconds=float(np.median(tf)), solve_seconds=float(np.median(ts)), tts_seconds=med,
+                tts_min_seconds=min(tts), thflops=fl, reps=len(tf))
+
+
+def solve_and_spmv_rates(px, run, reps=5):
+    """Device time of one solve (HIP events inside the engine) and of one
+    SpMV of all right-hand sides, with their algorithmic HBM bytes.
+
+    Solve bytes: each sweep reads every off-diagonal tile of the factor once
+    (sum over cblks of n_k (stride_k - n_k) entries) and the lower triangle of
+    each diagonal block's inverse (n_k (n_k + 1) / 2 entries), in the factor
+    precision; the vectors (n x nrhs, read and written a few times) are below
+    1 % of it and left out.  SpMV bytes: the stored CSC once (8-byte values
+    and row indices, n + 1 column pointers) plus x read and y written once."""
+    s = px.solver_export(run.pd)
+    cb = s["cblk"].reshape(-1, 6)[:s["cblknbr"]]
+    w = cb[:, 1] - cb[:, 0] + 1
+    offd = float(np.sum(w * (cb[:, 3] - w)))
+    tri = float(np.sum(w * (w + 1) / 2))
+    es = 4 if run.iparm[px.IPARM_MIXED] else 8
+    solve_bytes = 2.0 * es * (offd + tri)
+    times = []
+    for _ in range(reps):
+        px.lib.pxd_vec_copy(run.dev, run.N, run.nrhs, run.db, run.dx)
+        px._check("solve", px.lib.pastix_amd_solve_dev(run.pd, run.nrhs, run.dx, run.N))
+        times.append(px.get_stats(run.pd)["solve_seconds"])
+    t_solve = float(np.median(times))
+    dy = ctypes.c_void_p()
+    px.lib.pxd_malloc(run.dev, run.nbytes, ctypes.byref(dy))
+    nsp = 20
+    px.lib.pxd_sync(run.dev)
+    t0 = time.perf_counter()
+    for _ in range(nsp):
+        px.lib.pxd_spmv(run.dev, 1.0, run.dx, 0.0, dy, run.nrhs, run.N)
+    px.lib.pxd_sync(run.dev)
+    t_spmv = (time.perf_counter() - t0) / nsp
+    px.lib.pxd_free(run.dev, dy)
+    bc_n, bc_nnz = px.pastix_int(), px.pastix_int()
+    px.lib.pastix_amd_bcsc_sizes(run.pd, ctypes.byref(bc_n), ctypes.byref(bc_nnz))
+    spmv_bytes = bc_nnz.value * 16.0 + (run.N + 1) * 8.0 + 2.0 * run.N * run.nrhs * 8.0
+    return {
+        "solve": {"bound": "hbm", "achieved": solve_bytes / t_solve / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": solve_bytes / t_solve / 1e9 / HBM_PEAK_GBS, "seconds": t_solve,
+                  "algorithmic_bytes": solve_bytes,
+                  "kernel": "forward + backward sweeps of one %d-RHS solve (event-timed)" % run.nrhs},
+        "spmv": {"bound": "hbm", "achieved": spmv_bytes / t_spmv / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": spmv_bytes / t_spmv / 1e9 / HBM_PEAK_GBS, "seconds": t_spmv,
+                 "algorithmic_bytes": spmv_bytes,
+                 "kernel": "k_spmv<double, double>, %d right-hand sides (mean of %d launches)" % (run.nrhs, nsp)},
+    }
 
 
 # The other configurations of the benchmark suite (the headline is c2):
@@ -281,7 +389,7 @@ def main():
     log("step: generate %d^3 Laplacian" % n)
     spm = px.Spm.laplacian7(n, n, n)
 
-    results = {}
+    results, xs = {}, {}
     for label, mixed in (("fp32_ir", 1), ("fp64", 0)):
         if label == "fp64" and args.no_fp64:
             continue
@@ -295,15 +403,16 @@ def main():
         st = px.get_stats(run.pd)
         r["levels"] = int(st["levels"])
         r["launches"] = int(st["launches"])
-        if args.profile_stats or label == "fp32_ir":
-            px.set_option(run.pd, "timing", 1)
-            run.step()
-            st = px.get_stats(run.pd)
-            px.set_option(run.pd, "timing", 0)
-            r["kernel_seconds"] = {k: st[k] for k in ("fill_seconds", "panel_seconds", "trsm_seconds",
-                                                      "update_seconds")}
-            r["update_flops"] = update_flops(px, run.pd)
-        del r["x"]
+        # per-phase timing pass (one stream, events around each phase)
+        px.set_option(run.pd, "timing", 1)
+        run.step()
+        st = px.get_stats(run.pd)
+        px.set_option(run.pd, "timing", 0)
+        r["kernel_seconds"] = {k: st[k] for k in ("fill_seconds", "panel_seconds", "trsm_seconds",
+                                                  "update_seconds")}
+        r["update_flops"] = update_flops(px, run.pd)
+        r["bandwidth"] = solve_and_spmv_rates(px, run)
+        xs[label] = r.pop("x")
         results[label] = r
         log("%s: fact %.3f s solve %.3f s refine %.3f s (%d it) tts %.3f s berr %.2e"
             % (label, r["fact"], r["solve"], r["refine"], r["iters"], r["tts"], r["berr"]))
@@ -330,15 +439,22 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("step: CPU baseline sample")
-        cpu = cpu_baseline(px, args.cpu_sample, args.nrhs)
-        log("cpu: %.1f GFLOP/s on %d cores" % (cpu["value"], cpu["cores"]))
+        g = args.cpu_grid if args.cpu_grid else n
+        log("step: CPU baseline (%d^3, %d repetition(s))" % (g, args.cpu_reps))
+        cpu = cpu_baseline(px, g, args.nrhs, args.cpu_reps)
+        log("cpu: %.1f GFLOP/s on %d threads, time-to-solution %.2f s" % (cpu["value"], cpu["cores"],
+                                                                        cpu["tts_seconds"]))
 
     head = results["fp32_ir"]
     fl = head["thflops"]
-    upd = head.get("update_flops", 0.0)
-    upd_t = head.get("kernel_seconds", {}).get("update_seconds", 0.0)
-    achieved = upd / upd_t / 1e12 if upd_t > 0 else 0.0
+
+    def update_roofline(r, prec):
+        upd = r.get("update_flops", 0.0)
+        upd_t = r.get("kernel_seconds", {}).get("update_seconds", 0.0)
+        ach = upd / upd_t / 1e12 if upd_t > 0 else 0.0
+        return ach, upd
+
+    achieved, upd = update_roofline(head, "f32")
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
@@ -368,14 +484,27 @@ def main():
                                "of one factorization)", "algorithmic_flops": upd},
         "cpu_baseline": None,
     }
+    out["roofline_solve_fp32"] = head["bandwidth"]["solve"]
+    out["roofline_spmv"] = head["bandwidth"]["spmv"]
     if "fp64" in results:
         r64 = results["fp64"]
         out["fp64"] = r64
         out["speedup_fp32ir_vs_fp64"] = r64["tts"] / head["tts"]
+        a64, u64 = update_roofline(r64, "f64")
+        out["roofline_fp64"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                                "frac": a64 / MFMA_PEAK_TFLOPS["f64"], "traffic": None,
+                                "kernel": "k_update_big<double> + k_update_mfma<double> (all launches of one "
+                                          "factorization)", "algorithmic_flops": u64}
+        out["roofline_solve_fp64"] = r64["bandwidth"]["solve"]
+        # forward error of the FP32+IR solution against the FP64 factorization's
+        x32, x64 = xs["fp32_ir"], xs["fp64"]
+        out["ferr_vs_fp64"] = float(max(np.abs(x32[:, c] - x64[:, c]).max() / np.abs(x64[:, c]).max()
+                                        for c in range(x64.shape[1])))
     if cpu is not None:
         out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        cpu_tts = fl / (cpu["value"] * 1e9) + cpu["solve_seconds"] * (n ** 3) / (args.cpu_sample ** 3)
-        out["speedup_vs_cpu_projected"] = cpu_tts / head["tts"]
+        if cpu["reps"] and (args.cpu_grid in (None, n)):
+            out["cpu_tts_seconds"] = cpu["tts_seconds"]
+            out["speedup_vs_cpu"] = cpu["tts_seconds"] / head["tts"]
     print(json.dumps(out), flush=True)
 
 

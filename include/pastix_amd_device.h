@@ -92,10 +92,10 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "upd_small"     1: one-wavefront small updates (default)
  *   "upd_nh"        k-groups per stage of the tall-tile update (1 or 2)
  *   "prio"          1: raised wave priority on the critical path (default)
- *   "solve_sf"      1: one-launch solve sweeps above the cut level (default)
- *   "sf_from_level" first level of the one-launch part (-1: automatic)
+ *   "solve_graph"   1: each solve pass replays the sweeps' launches from a
+ *                   graph captured on first use (default); 0: launched one by one
  *   "solve_wide_max" level-solve launches with at least this many tiles load
- *                   16 columns per batch
+ *                   16 columns per batch (default 128)
  *   "solve_vec_double" 1: a single factor solves double vectors directly;
  *                   0: vectors rounded to single and back (default)
  *   "schur_solv_mode" 0: local, 1: interface, 2: whole matrix (see pxd_set_schur)

@@ -24,7 +24,7 @@
 #include <vector>
 #include "kernels_solve.hpp"
 #include "kernels_mfma.hpp"
-#include "kernels_sfsolve.hpp"
+#include "kernels_svsolve.hpp"
 #include "pastix/kernels_enums.h"
 
 namespace {
@@ -58,7 +58,19 @@ struct upd_group {
 }  // namespace
 
 // solve launches with fewer workgroups than this load every column at once
-#define PXD_SOLVE_WIDE_MAX 2048
+#define PXD_SOLVE_WIDE_MAX 128
+
+// A captured solve pass (option solve_graph): the launch sequence of both
+// sweeps on the instance's own vectors for nq columns of one factor type,
+// replayed as one graph.  It stays valid while everything the capture baked
+// in is unchanged (sig).
+struct solve_graph {
+    int            nq, tag;
+    const void    *sig[6];   // vectors, Y, L, U and the two inverse sets
+    uint64_t       gen;      // task-list generation
+    int64_t        wide;     // solve_wide_max at capture
+    hipGraphExec_t exec;
+};
 
 struct pxd_ctx_s {
     int         device = 0;
@@ -91,6 +103,9 @@ struct pxd_ctx_s {
     upd_task *dupdb = nullptr;
     upd_task *dupds = nullptr;
     row_task *dsolv = nullptr;
+    sv_tile  *dsvt  = nullptr;   // the solv tasks with their cblk sizes
+    sv_row   *dsvr  = nullptr;   // records of the off-diagonal stacked rows
+    sv_chunk *dsvc  = nullptr;   // the diag tasks with their cblk sizes
     row_task *ddiag = nullptr;
     inv_task *dinvt = nullptr;
     int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
@@ -109,6 +124,9 @@ struct pxd_ctx_s {
     size_t coef_bytes = 0;
     int   *dscratch = nullptr;     // counters (pivots, overflow)
     void  *dvecs = nullptr;        // solve scratch in the factor precision
+    std::vector<solve_graph> sgraphs;
+    uint64_t sched_gen   = 0;      // bumped when the task lists are rebuilt
+    int      solve_graph = 1;
     size_t dvecs_bytes = 0;
     void  *dsolvy = nullptr;       // the sweeps' second vector (n x PXD_MAXRHS, factor precision)
     size_t dsolvy_bytes = 0;
@@ -137,11 +155,6 @@ struct pxd_ctx_s {
     std::vector<inv_task>  hinvt;
     std::vector<upd_group> groups;
 
-    // one-launch solve sweeps (k_sf_forward / k_sf_backward): task lists,
-    // the cblks each tile's rows face, the tiles facing each cblk, and the
-    // dependency counters (forward cnt, rdy; backward cnt, rdy), which grow
-    // by one pass's worth per pass (sf_epoch)
-    int       solve_sf = 1;
     // 1: a single factor solves double vectors directly (entries widened as
     // they are read); 0 (default): the vectors are converted to single and
     // back, as in the reference.  On the 100^3 Laplacian the double vectors
@@ -152,13 +165,6 @@ struct pxd_ctx_s {
     // level solve launches with at least this many tiles load 16 columns
     // per batch (higher occupancy); smaller ones load every column at once
     int64_t   solve_wide_max = PXD_SOLVE_WIDE_MAX;
-    int32_t   sf_from_level = -1;   // option: first level of the one-launch part (-1: automatic)
-    int32_t   sf_lc = 0;            // the cut in use
-    sf_task  *dsf_fw = nullptr, *dsf_bw = nullptr;
-    int32_t  *dsf_facing = nullptr, *dsf_need = nullptr;
-    uint32_t *dsf_cnt = nullptr;
-    int64_t   nsf_fw = 0, nsf_bw = 0;
-    uint32_t  sf_epoch = 0, sf_epoch_bw = 0;   // passes of each sweep (their counters grow by one pass's worth)
 
     // Schur complement: the cblks from column schurfcol on (the last ones)
     // hold the Schur unknowns.  schur_fact_local: they receive the updates of
@@ -269,135 +275,8 @@ compute_levels( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
     }
 }
 
-// Task lists of the one-launch solve sweeps, for the cblks of dependency
-// level sf_lc and above (the levels below run level by level).  The cut is
-// the lowest level from which every level has at most PXD_SF_AUTO_TASKS
-// tasks (the latency-bound top of the tree), unless the option sets it.
-// A cblk with one diagonal chunk and at most one off-diagonal tile is one
-// SF_LEAF task; others are SF_DIAG chunks and SF_TILE tiles.  Forward: level
-// by level from sf_lc up, the chunks (and leaves) of a level before its
-// tiles; backward: from the top level down, a level's tiles (and leaves)
-// before its chunks.  So workgroups are dispatched as the level launches
-// would run, without the barriers.  A tile lists the distinct cblks its
-// rows face (the bloks of a cblk face rows in order, so equal faces are
-// adjacent); need[f] counts the tiles of the list that update f.
-#define PXD_SF_AUTO_TASKS 1024
-int
-build_sf_schedule( pxd_ctx_t *ctx )
-{
-    const int64_t nc   = ctx->cblknbr;
-    const int32_t nlev = ctx->nlev;
-    // per cblk: chunk count, tiles (r0, facing range) -- facing built once
-    std::vector<int32_t> facing, need( std::max<int64_t>( 1, nc ), 0 );
-    std::vector<int32_t> tile0( nc + 1, 0 );
-    std::vector<sf_task> tiles;
-    for ( int64_t k = 0; k < nc; k++ ) {
-        const dcblk  &c  = ctx->hcb[k];
-        const int32_t n  = c.lcolnum - c.fcolnum + 1;
-        const int32_t lb = ctx->hcb[k + 1].fblok;
-        int32_t       b  = c.fblok + 1;
-        tile0[k]         = (int32_t)tiles.size();
-        for ( int32_t r0 = n; r0 < c.stride; r0 += PXD_SF_TILE ) {
-            const int32_t r1 = std::min( r0 + PXD_SF_TILE, c.stride );
-            while ( b < lb && ctx->hbk[b].rowoff + ( ctx->hbk[b].lrownum - ctx->hbk[b].frownum + 1 ) <= r0 ) {
-                b++;
-            }
-            const int32_t f0 = (int32_t)facing.size();
-            for ( int32_t bb = b; bb < lb && ctx->hbk[bb].rowoff < r1; bb++ ) {
-                const int32_t f = ctx->hbk[bb].fcblknm;
-                if ( facing.size() == (size_t)f0 || facing.back() != f ) {
-                    facing.push_back( f );
-                }
-            }
-            tiles.push_back( sf_task{ SF_TILE, (int32_t)k, r0, f0, (int32_t)facing.size(), 0 } );
-        }
-    }
-    tile0[nc] = (int32_t)tiles.size();
-    auto nchunk = [&]( int64_t k ) {
-        return ( ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1 + PXD_SF_ROWS - 1 ) / PXD_SF_ROWS;
-    };
-    auto leaf = [&]( int64_t k ) { return nchunk( k ) == 1 && tile0[k + 1] - tile0[k] <= 1; };
-    std::vector<std::vector<int32_t>> bylev( std::max( 1, nlev ) );
-    std::vector<int64_t>              ntask( std::max( 1, nlev ), 0 );
-    for ( int64_t k = 0; k < nc; k++ ) {
-        bylev[ctx->level[k]].push_back( (int32_t)k );
-        ntask[ctx->level[k]] += leaf( k ) ? 1 : nchunk( k ) + ( tile0[k + 1] - tile0[k] );
-    }
-    int32_t lc = ctx->sf_from_level;
-    if ( lc < 0 ) {
-        lc = nlev;
-        while ( lc > 0 && ntask[lc - 1] <= PXD_SF_AUTO_TASKS ) {
-            lc--;
-        }
-    }
-    lc         = std::min( std::max( lc, 0 ), nlev );
-    ctx->sf_lc = lc;
 
-    std::vector<sf_task> fw, bw;
-    for ( int32_t L = lc; L < nlev; L++ ) {
-        for ( int32_t k : bylev[L] ) {
-            if ( leaf( k ) ) {
-                sf_task t{ SF_LEAF, k, 0, 0, 0, 0 };
-                if ( tile0[k + 1] > tile0[k] ) {
-                    t.f0 = tiles[tile0[k]].f0;
-                    t.f1 = tiles[tile0[k]].f1;
-                }
-                fw.push_back( t );
-            }
-            else {
-                for ( int32_t R0 = 0; R0 < ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1; R0 += PXD_SF_ROWS ) {
-                    fw.push_back( sf_task{ SF_DIAG, k, R0, 0, 0, 0 } );
-                }
-            }
-        }
-        for ( int32_t k : bylev[L] ) {
-            if ( !leaf( k ) ) {
-                fw.insert( fw.end(), tiles.begin() + tile0[k], tiles.begin() + tile0[k + 1] );
-            }
-            for ( int32_t t = tile0[k]; t < tile0[k + 1]; t++ ) {
-                for ( int32_t i = tiles[t].f0; i < tiles[t].f1; i++ ) need[facing[i]]++;
-            }
-        }
-    }
-    for ( int32_t L = nlev - 1; L >= lc; L-- ) {
-        for ( int32_t k : bylev[L] ) {
-            if ( leaf( k ) ) {
-                sf_task t{ SF_LEAF, k, 0, 0, 0, 0 };
-                if ( tile0[k + 1] > tile0[k] ) {
-                    t.f0 = tiles[tile0[k]].f0;
-                    t.f1 = tiles[tile0[k]].f1;
-                }
-                bw.push_back( t );
-            }
-            else {
-                bw.insert( bw.end(), tiles.begin() + tile0[k], tiles.begin() + tile0[k + 1] );
-            }
-        }
-        for ( int32_t k : bylev[L] ) {
-            if ( !leaf( k ) ) {
-                for ( int32_t R0 = 0; R0 < ctx->hcb[k].lcolnum - ctx->hcb[k].fcolnum + 1; R0 += PXD_SF_ROWS ) {
-                    bw.push_back( sf_task{ SF_DIAG, k, R0, 0, 0, 0 } );
-                }
-            }
-        }
-    }
-    if ( upload( &ctx->dsf_fw, fw ) || upload( &ctx->dsf_bw, bw ) || upload( &ctx->dsf_facing, facing ) ||
-         upload( &ctx->dsf_need, need ) ) {
-        return 6;
-    }
-    ctx->nsf_fw = (int64_t)fw.size();
-    ctx->nsf_bw = (int64_t)bw.size();
-    if ( ctx->dsf_cnt ) {
-        PXD_CHECK( hipFree( ctx->dsf_cnt ) );
-    }
-    PXD_CHECK( hipMalloc( (void **)&ctx->dsf_cnt, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
-    PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, nc ) ) );
-    PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
-    // the counters start from zero again: so do the passes of both sweeps
-    ctx->sf_epoch    = 0;
-    ctx->sf_epoch_bw = 0;
-    return 0;
-}
+void drop_solve_graphs( pxd_ctx_t *ctx );
 
 // Build the task lists of the factorization and the solve, level by level.
 int
@@ -621,7 +500,45 @@ build_schedule( pxd_ctx_t *ctx )
          upload( &ctx->dsolv, solv ) || upload( &ctx->dinvt, invt ) || upload( &ctx->ddiag, diag ) ) {
         return 6;
     }
-    return build_sf_schedule( ctx );
+    {
+        // the solve's row records and tasks with the sizes of their cblk
+        std::vector<int64_t> rowstart( nc + 1, 0 );
+        for ( int64_t k = 0; k < nc; k++ ) {
+            const dcblk &c = ctx->hcb[k];
+            rowstart[k + 1] = rowstart[k] + ( c.stride - ( c.lcolnum - c.fcolnum + 1 ) );
+        }
+        std::vector<sv_row> rows( rowstart[nc] );
+        for ( int64_t k = 0; k < nc; k++ ) {
+            const dcblk  &c  = ctx->hcb[k];
+            const int32_t n  = c.lcolnum - c.fcolnum + 1;
+            const int32_t lb = ctx->hcb[k + 1].fblok;
+            for ( int32_t b = c.fblok + 1; b < lb; b++ ) {
+                const dblok  &bb = ctx->hbk[b];
+                const int32_t nr = bb.lrownum - bb.frownum + 1;
+                for ( int32_t i = 0; i < nr; i++ ) {
+                    rows[rowstart[k] + bb.rowoff + i - n] =
+                        sv_row{ c.coefoff + (int64_t)bb.rowoff * n + i, nr, bb.frownum + i };
+                }
+            }
+        }
+        std::vector<sv_tile>  svt( solv.size() );
+        std::vector<sv_chunk> svc( diag.size() );
+        for ( size_t i = 0; i < solv.size(); i++ ) {
+            const dcblk  &c = ctx->hcb[solv[i].cblk];
+            const int32_t n = c.lcolnum - c.fcolnum + 1;
+            svt[i] = sv_tile{ n, c.fcolnum, solv[i].r0, c.stride, rowstart[solv[i].cblk] + solv[i].r0 - n };
+        }
+        for ( size_t i = 0; i < diag.size(); i++ ) {
+            const dcblk &c = ctx->hcb[diag[i].cblk];
+            svc[i] = sv_chunk{ c.lcolnum - c.fcolnum + 1, c.fcolnum, diag[i].r0, 0, c.dinvoff };
+        }
+        if ( upload( &ctx->dsvt, svt ) || upload( &ctx->dsvr, rows ) || upload( &ctx->dsvc, svc ) ) {
+            return 6;
+        }
+    }
+    ctx->sched_gen++;
+    drop_solve_graphs( ctx );
+    return 0;
 }
 
 // Elapsed milliseconds between two events (the stream must be synchronized).
@@ -1089,65 +1006,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     return 0;
 }
 
-// ---------------------------------------------------------------------------
-// Right-hand sides of a pass that the one-launch sweeps take: with more, the
-// accumulators and LDS of a task lower the occupancy below what the level
-// kernels reach (measured: a complex64 solve of 16 RHS in passes of 8 took
-// 26 ms one-launch against 19 ms level by level), so larger passes run
-// level by level throughout.
-#define PXD_SF_NR 4
-
-// The one-launch part of a sweep (the levels from sf_lc up) for nrhs <= NR
-// columns of X (vector precision V, leading dimension ldx), Y the second
-// vector (ld n); factor entries T.  The forward launch opens a pass (epoch).
-template <typename T, typename V, int NR>
-int
-sf_launch( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y, bool swap )
-{
-    constexpr int B  = 128 / (int)sizeof( T );   // columns per load batch of a forward tile
-    const bool    lu = ( ctx->facto == FACT_GETRF );
-    // forward: L (or U^T when swap), backward: U via U^T for LU (or L^T when swap)
-    const T      *L  = (const T *)( swap ? ctx->dU : ctx->dL );
-    const T      *DI = (const T *)( swap ? ctx->dudinv : ctx->ddinv );
-    const T      *LB = (const T *)( lu && !swap ? ctx->dU : ctx->dL );
-    const T      *DB = (const T *)( lu && !swap ? ctx->dudinv : ctx->ddinv );
-    const size_t  sh = sf_lds_bytes<V, NR>( ctx->colmax );
-    if ( ctx->colmax > PXD_TRSM_NT * 16 || sh > lds_room( (const void *)k_sf_forward<T, V, NR, B>, ctx->lds_max ) ||
-         sh > lds_room( (const void *)k_sf_backward<T, V, NR>, ctx->lds_max ) ) {
-        return 3;
-    }
-    if ( ( forward ? ctx->nsf_fw : ctx->nsf_bw ) == 0 ) {
-        return 0;
-    }
-    if ( forward ) {
-        ctx->sf_epoch++;
-    }
-    else {
-        ctx->sf_epoch_bw++;
-    }
-    const int64_t nc  = ctx->cblknbr;
-    uint32_t     *cnt = ctx->dsf_cnt;
-    int          *err = ctx->dscratch + 8;
-    if ( forward ) {
-        hipLaunchKernelGGL( ( k_sf_forward<T, V, NR, B> ), dim3( (unsigned)ctx->nsf_fw ), dim3( PXD_THREADS ), sh,
-                            ctx->stream, ctx->dsf_fw, ctx->dsf_facing, ctx->dsf_need, ctx->dcb, ctx->dbk, L, DI, X,
-                            ldx, Y, ctx->n, nrhs, ctx->sf_epoch, cnt, cnt + nc, err );
-    }
-    else {
-        hipLaunchKernelGGL( ( k_sf_backward<T, V, NR> ), dim3( (unsigned)ctx->nsf_bw ), dim3( PXD_THREADS ), sh,
-                            ctx->stream, ctx->dsf_bw, ctx->dsf_facing, ctx->dcb, ctx->dbk, LB, DB, X, ldx, Y, ctx->n,
-                            nrhs, ctx->sf_epoch_bw, cnt + 2 * nc, cnt + 3 * nc, err );
-    }
-    PXD_CHECK( hipGetLastError() );
-    return 0;
-}
-
-template <typename T, typename V>
-int
-sf_sweep( pxd_ctx_t *ctx, bool forward, V *X, int64_t ldx, int nrhs, V *Y, bool swap )
-{
-    return sf_launch<T, V, PXD_SF_NR>( ctx, forward, X, ldx, nrhs, Y, swap );
-}
 
 // Right-hand sides per solve pass.
 template <typename V>
@@ -1157,24 +1015,6 @@ solve_pass_rhs( const pxd_ctx_t * )
     return PXD_MAXRHS;
 }
 
-// After the passes of a solve: a timed-out wait leaves the counters
-// inconsistent, so they are cleared and the solve reports the error.
-int
-sf_check( pxd_ctx_t *ctx )
-{
-    int e = 0;
-    PXD_CHECK( hipMemcpyAsync( &e, ctx->dscratch + 8, sizeof( int ), hipMemcpyDeviceToHost, ctx->stream ) );
-    PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
-    if ( e != 0 ) {
-        PXD_CHECK( hipMemset( ctx->dsf_cnt, 0, 4 * sizeof( uint32_t ) * std::max<int64_t>( 1, ctx->cblknbr ) ) );
-        PXD_CHECK( hipMemset( ctx->dscratch + 8, 0, sizeof( int ) ) );
-        ctx->sf_epoch    = 0;
-        ctx->sf_epoch_bw = 0;
-        fprintf( stderr, "pastix_amd device: a solve task timed out waiting for its dependencies\n" );
-        return 6;
-    }
-    return 0;
-}
 
 // Steps of a solve (bit mask): the forward sweep, the diagonal D^{-1} of an
 // LDL^T factor, the backward sweep.
@@ -1186,6 +1026,49 @@ enum { SOLVE_FWD = 1, SOLVE_DIAG = 2, SOLVE_BWD = 4, SOLVE_ALL = 7 };
 // kernels only).  swap exchanges the two sides of an LU factor: the forward
 // sweep runs on U^T and the backward sweep on L^T (the solve with A^T,
 // reference sopalin/pastix_task_solve.c:597-601).
+// One level of the level-by-level sweeps with the sv kernels, NR columns
+// per launch (nrhs <= NR): forward, the diagonal chunks (X -> Y) then the
+// tiles (Y -> atomics into X); backward, the tiles (X -> atomics into Y)
+// then the chunks (Y -> X).  nd chunks from lv.d0, nf tiles from lv.f0.
+template <typename T, typename V, int NR>
+void
+sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned nf, const T *L, const T *DI,
+          V *X, int64_t ldx, V *Y, int64_t ldy, int nrhs )
+{
+    constexpr int WB = solve_wide<T>::value;   // column batch of launches with few workgroups
+    hipStream_t   s  = ctx->stream;
+    const size_t  sh = (size_t)lv.maxn * NR * sizeof( V );
+    const bool    many = (int64_t)nf >= ctx->solve_wide_max;
+    if ( forward ) {
+        if ( nd ) {
+            hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
+                                DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
+        }
+        if ( nf && many ) {
+            hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs );
+        }
+        else if ( nf ) {
+            hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, WB> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs );
+        }
+    }
+    else {
+        if ( nf && many ) {
+            hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs );
+        }
+        else if ( nf ) {
+            hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, WB / 16> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs );
+        }
+        if ( nd ) {
+            hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
+                                DI, (const V *)Y, ldy, X, ldx, nrhs, 1 );
+        }
+    }
+}
+
 template <typename T, typename V>
 int
 solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL, bool swap = false,
@@ -1255,16 +1138,15 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
         }
         V            *Y   = Yscr ? Yscr : (V *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
-        // levels [0, nlv) run level by level, the others in one launch per
-        // sweep (passes of at most PXD_SF_NR right-hand sides)
-        const bool    use_sf = ctx->solve_sf && nrhs <= PXD_SF_NR && !schur;
-        const int     nlv    = use_sf ? std::min( nl, (int)ctx->sf_lc ) : nl;
+        const int     nlv = nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
-        const size_t shmax = (size_t)ctx->colmax * nrhs * sizeof( V );
-        if ( nlv > 0 && ( nrhs > PXD_MAXRHS || shmax > lds_room( (const void *)k_solve_diag_chunk<T, V>, ctx->lds_max ) ||
-                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, V, 16>, ctx->lds_max ) ||
-                          shmax > lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, ctx->lds_max ) ) ) {
+        const size_t shmax = (size_t)ctx->colmax * PXD_MAXRHS * sizeof( V );
+        if ( nlv > 0 && ( nrhs > PXD_MAXRHS || ctx->colmax > PXD_SV_MAXN ||
+                          shmax > lds_room( (const void *)k_sv_diag<T, V, PXD_MAXRHS>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_sv_fwd_tile<T, V, PXD_MAXRHS, 16>, ctx->lds_max ) ||
+                          shmax > lds_room( (const void *)k_sv_fwd_tile<T, V, PXD_MAXRHS, solve_wide<T>::value>,
+                                            ctx->lds_max ) ) ) {
             return 3;
         }
         if ( !( what & SOLVE_FWD ) ) {
@@ -1279,24 +1161,12 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
             unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
-            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
-            if ( nd ) {
-                hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
-                                    ctx->ddiag + lv.d0, ctx->dcb, DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
+            if ( nrhs <= 4 ) {
+                sv_level<T, V, 4>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs );
             }
-            if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
-                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, V, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const V *)Y, ldy );
+            else {
+                sv_level<T, V, PXD_MAXRHS>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs );
             }
-            else if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_fwd_upd<T, V, solve_wide<T>::value> ), dim3( nf ), dim3( PXD_THREADS ),
-                                    sh, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, L, X, ldx, nrhs, (const V *)Y,
-                                    ldy );
-            }
-        }
-        int rc = 0;
-        if ( ( what & SOLVE_FWD ) && use_sf && ( rc = sf_sweep<T, V>( ctx, true, X, ldx, nrhs, Y, swap ) ) != 0 ) {
-            return rc;
         }
         if ( ( what & SOLVE_FWD ) && schur && iface ) {
             schur_copy( X, ldx, Y, ldy );   // interface: b22 - L21 x11
@@ -1321,26 +1191,15 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
                 schur_zero( X, ldx );
             }
         }
-        if ( use_sf && ( rc = sf_sweep<T, V>( ctx, false, X, ldx, nrhs, Y, swap ) ) != 0 ) {
-            return rc;
-        }
         for ( int li = nlv - 1; li >= 0; li-- ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
             unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
-            size_t         sh = (size_t)lv.maxn * nrhs * sizeof( V );
-            if ( nf && (int64_t)nf >= ctx->solve_wide_max ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, V, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                    ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB, (const V *)X, ldx, Y, ldy, nrhs );
+            if ( nrhs <= 4 ) {
+                sv_level<T, V, 4>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs );
             }
-            else if ( nf ) {
-                hipLaunchKernelGGL( ( k_solve_bwd_gather2<T, V, solve_wide<T>::value / 16> ), dim3( nf ),
-                                    dim3( PXD_THREADS ), 0, s, ctx->dsolv + lv.f0, ctx->dcb, ctx->dbk, LB,
-                                    (const V *)X, ldx, Y, ldy, nrhs );
-            }
-            if ( nd ) {
-                hipLaunchKernelGGL( ( k_solve_diag_chunk<T, V> ), dim3( nd ), dim3( PXD_THREADS ), sh, s,
-                                    ctx->ddiag + lv.d0, ctx->dcb, DB, (const V *)Y, ldy, X, ldx, nrhs, 1 );
+            else {
+                sv_level<T, V, PXD_MAXRHS>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs );
             }
         }
         if ( schur && !iface ) {
@@ -1397,11 +1256,12 @@ template <typename T, typename V>
 void
 set_lds_attributes_solve( size_t lds )
 {
-    set_lds_room( (const void *)k_solve_fwd_upd<T, V, 16>, lds );
-    set_lds_room( (const void *)k_solve_fwd_upd<T, V, solve_wide<T>::value>, lds );
-    set_lds_room( (const void *)k_solve_diag_chunk<T, V>, lds );
-    set_lds_room( (const void *)k_sf_forward<T, V, PXD_SF_NR, 128 / sizeof( T )>, lds );
-    set_lds_room( (const void *)k_sf_backward<T, V, PXD_SF_NR>, lds );
+    set_lds_room( (const void *)k_sv_diag<T, V, 4>, lds );
+    set_lds_room( (const void *)k_sv_diag<T, V, PXD_MAXRHS>, lds );
+    set_lds_room( (const void *)k_sv_fwd_tile<T, V, 4, 16>, lds );
+    set_lds_room( (const void *)k_sv_fwd_tile<T, V, PXD_MAXRHS, 16>, lds );
+    set_lds_room( (const void *)k_sv_fwd_tile<T, V, 4, solve_wide<T>::value>, lds );
+    set_lds_room( (const void *)k_sv_fwd_tile<T, V, PXD_MAXRHS, solve_wide<T>::value>, lds );
 }
 
 // the matrix-core kernels (all four arithmetics)
@@ -1433,6 +1293,7 @@ set_lds_attributes( size_t lds )
     set_lds_room( (const void *)k_trsm<T, 32>, lds );
     set_lds_room( (const void *)k_trsm<T, 16>, lds );
     set_lds_room( (const void *)k_solve_diag<T>, lds );
+    set_lds_room( (const void *)k_solve_fwd_upd<T, T, 16>, lds );
 }
 
 // Solve of nrhs vectors of type TV (double or complex double) with a factor
@@ -1442,17 +1303,129 @@ set_lds_attributes( size_t lds )
 // the factor applied without rounding of the vectors); otherwise through
 // vectors converted to the factor precision and back, as the reference's
 // mixed solve does.
+template <typename T> struct type_tag;
+template <> struct type_tag<double> { static constexpr int value = 0; };
+template <> struct type_tag<float> { static constexpr int value = 1; };
+template <> struct type_tag<pxcd> { static constexpr int value = 2; };
+template <> struct type_tag<pxcf> { static constexpr int value = 3; };
+
+void
+drop_solve_graphs( pxd_ctx_t *ctx )
+{
+    for ( solve_graph &g : ctx->sgraphs ) {
+        (void)hipGraphExecDestroy( g.exec );
+    }
+    ctx->sgraphs.clear();
+}
+
+// Whether the sweeps of a pass run as a captured graph: the full solve of
+// the matrix-core path without the one-launch sweeps (their pass counters
+// change from call to call) and without a Schur part.
+bool
+graph_ok( const pxd_ctx_t *ctx, int what, bool swap )
+{
+    return ctx->solve_graph && ctx->kernels && what == SOLVE_ALL && !swap &&
+           ctx->schurfcol >= ctx->n;
+}
+
+// Instance buffers of a graph pass: the vectors (n x PXD_MAXRHS of T, ld n)
+// and the sweeps' Y, allocated before any capture.
+template <typename T>
+int
+graph_buffers( pxd_ctx_t *ctx )
+{
+    const size_t need = (size_t)ctx->n * PXD_MAXRHS * sizeof( T );
+    if ( ctx->dvecs_bytes < need ) {
+        if ( ctx->dvecs ) PXD_CHECK( hipFree( ctx->dvecs ) );
+        PXD_CHECK( hipMalloc( &ctx->dvecs, need ) );
+        ctx->dvecs_bytes = need;
+    }
+    if ( ctx->dsolvy_bytes < need ) {
+        if ( ctx->dsolvy ) PXD_CHECK( hipFree( ctx->dsolvy ) );
+        PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
+        ctx->dsolvy_bytes = need;
+    }
+    return 0;
+}
+
+// Both sweeps of nq columns held in ctx->dvecs, replayed from a graph that
+// is captured on first use (and again when what it baked in has changed).
+template <typename T>
+int
+graph_pass( pxd_ctx_t *ctx, int nq )
+{
+    const void *sig[6] = { ctx->dvecs, ctx->dsolvy, ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv };
+    solve_graph *hit  = nullptr;
+    for ( size_t i = 0; i < ctx->sgraphs.size(); i++ ) {
+        solve_graph &g = ctx->sgraphs[i];
+        if ( g.nq == nq && g.tag == type_tag<T>::value ) {
+            if ( g.gen == ctx->sched_gen && g.wide == ctx->solve_wide_max && !memcmp( g.sig, sig, sizeof( sig ) ) ) {
+                hit = &g;
+            }
+            else {
+                (void)hipGraphExecDestroy( g.exec );
+                ctx->sgraphs.erase( ctx->sgraphs.begin() + i );
+            }
+            break;
+        }
+    }
+    if ( hit == nullptr ) {
+        hipGraph_t     g  = nullptr;
+        hipGraphExec_t ex = nullptr;
+        PXD_CHECK( hipStreamBeginCapture( ctx->stream, hipStreamCaptureModeThreadLocal ) );
+        const int  rc = solve_sweeps<T, T>( ctx, (T *)ctx->dvecs, ctx->n, nq, SOLVE_ALL, false, nullptr );
+        hipError_t e  = hipStreamEndCapture( ctx->stream, &g );
+        if ( rc != 0 || e != hipSuccess || g == nullptr ) {
+            if ( g ) (void)hipGraphDestroy( g );
+            (void)hipGetLastError();
+            return rc ? rc : 6;
+        }
+        e = hipGraphInstantiate( &ex, g, nullptr, nullptr, 0 );
+        (void)hipGraphDestroy( g );
+        if ( e != hipSuccess ) {
+            (void)hipGetLastError();
+            return 6;
+        }
+        solve_graph ng;
+        ng.nq   = nq;
+        ng.tag  = type_tag<T>::value;
+        memcpy( ng.sig, sig, sizeof( sig ) );
+        ng.gen  = ctx->sched_gen;
+        ng.wide = ctx->solve_wide_max;
+        ng.exec = ex;
+        ctx->sgraphs.push_back( ng );
+        hit = &ctx->sgraphs.back();
+    }
+    PXD_CHECK( hipGraphLaunch( hit->exec, ctx->stream ) );
+    return 0;
+}
+
 template <typename TV, typename T>
 int
 solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = SOLVE_ALL, bool swap = false,
                void *scr = nullptr )
 {
     int rc = 0;
+    const bool graph = scr == nullptr && graph_ok( ctx, what, swap );
+    if ( graph && ( rc = graph_buffers<T>( ctx ) ) != 0 ) {
+        return rc;
+    }
     if constexpr ( std::is_same<TV, T>::value ) {
         const int pass = solve_pass_rhs<T>( ctx );
         for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
             int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq, what, swap, (T *)scr );
+            if ( graph ) {
+                // through the instance's vectors, which the graph reads
+                T *xs = (T *)ctx->dvecs;
+                hipLaunchKernelGGL( ( k_copy_cols<T> ), dim3( nblocks( ctx->n, 256 ), nq ), dim3( 256 ), 0,
+                                    ctx->stream, ctx->n, (const T *)dx + q0 * ldx, ldx, xs, ctx->n );
+                rc = graph_pass<T>( ctx, nq );
+                hipLaunchKernelGGL( ( k_copy_cols<T> ), dim3( nblocks( ctx->n, 256 ), nq ), dim3( 256 ), 0,
+                                    ctx->stream, ctx->n, (const T *)xs, ctx->n, (T *)dx + q0 * ldx, ldx );
+            }
+            else {
+                rc = solve_sweeps<T, T>( ctx, (T *)dx + q0 * ldx, ldx, nq, what, swap, (T *)scr );
+            }
         }
         return rc;
     }
@@ -1488,7 +1461,7 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = S
                 hipLaunchKernelGGL( ( k_convert<T, TV> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, (const TV *)dx + ( q0 + q ) * ldx, xs + q * ctx->n );
             }
-            rc = solve_sweeps<T, T>( ctx, xs, ctx->n, nq, what, swap, ys );
+            rc = graph ? graph_pass<T>( ctx, nq ) : solve_sweeps<T, T>( ctx, xs, ctx->n, nq, what, swap, ys );
             for ( int q = 0; q < nq; q++ ) {
                 hipLaunchKernelGGL( ( k_convert<TV, T> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
                                     ctx->n, xs + q * ctx->n, (TV *)dx + ( q0 + q ) * ldx );
@@ -1538,10 +1511,6 @@ solve_steps( pxd_ctx_t *ctx, int what, bool swap, int64_t nrhs, void *dx, int64_
     }
     (void)hipEventDestroy( e0 );
     (void)hipEventDestroy( e1 );
-    if ( ctx->kernels && ctx->solve_sf ) {
-        const int rc2 = sf_check( ctx );
-        rc = rc ? rc : rc2;
-    }
     return rc;
 }
 
@@ -1672,10 +1641,11 @@ pxd_destroy( pxd_ctx_t *ctx )
     }
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
+    drop_solve_graphs( ctx );
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred,
-                     ctx->dsf_fw, ctx->dsf_bw, ctx->dsf_facing, ctx->dsf_need, ctx->dsf_cnt };
+                     };
     for ( void *p : ptrs ) {
         if ( p ) {
             (void)hipFree( p );
@@ -1730,8 +1700,8 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
         ctx->tasks_dirty = 1;
         return 0;
     }
-    if ( k == "solve_sf" ) {
-        ctx->solve_sf = ( value != 0. );
+    if ( k == "solve_graph" ) {
+        ctx->solve_graph = ( value != 0. );
         return 0;
     }
     if ( k == "solve_vec_double" ) {
@@ -1744,12 +1714,6 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     }
     if ( k == "solve_wide_max" ) {
         ctx->solve_wide_max = (int64_t)std::max( 0., std::min( value, 1e15 ) );
-        return 0;
-    }
-    if ( k == "sf_from_level" ) {
-        // takes effect at the next factorization
-        ctx->sf_from_level = ( value < 0. ) ? -1 : (int32_t)std::min( value, 1e9 );
-        ctx->tasks_dirty   = 1;
         return 0;
     }
     if ( k == "panel_reg" ) {
@@ -2123,11 +2087,7 @@ pxd_solve_noalloc( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx )
     case FT_CPLX64: rc = solve_vectors<pxcd, pxcd>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
     default: rc = solve_vectors<pxcd, pxcf>( ctx, nrhs, dx, ldx, SOLVE_ALL, false, scr ); break;
     }
-    if ( ctx->kernels && ctx->solve_sf ) {
-        const int rc2 = sf_check( ctx );   // also waits for the solve
-        rc = rc ? rc : rc2;
-    }
-    else if ( hipStreamSynchronize( ctx->stream ) != hipSuccess ) {
+    if ( hipStreamSynchronize( ctx->stream ) != hipSuccess ) {
         rc = rc ? rc : 6;
     }
     return rc;
@@ -2186,6 +2146,22 @@ pxd_get_diag( pxd_ctx_t *ctx, void *dd )
     return 0;
 }
 
+extern "C++" {
+template <typename T>
+static void
+spmv_launch( int G, dim3 grid, hipStream_t st, int64_t n, const int64_t *cp, const int64_t *rt, const T *v,
+             const T *x, T *y, int64_t ld, int nrhs, double alpha, double beta )
+{
+    switch ( G ) {
+    case 4: hipLaunchKernelGGL( ( k_spmv_grp<T, T, 4> ), grid, dim3( 256 ), 0, st, n, cp, rt, v, x, y, ld, nrhs, alpha, beta ); break;
+    case 8: hipLaunchKernelGGL( ( k_spmv_grp<T, T, 8> ), grid, dim3( 256 ), 0, st, n, cp, rt, v, x, y, ld, nrhs, alpha, beta ); break;
+    case 16: hipLaunchKernelGGL( ( k_spmv_grp<T, T, 16> ), grid, dim3( 256 ), 0, st, n, cp, rt, v, x, y, ld, nrhs, alpha, beta ); break;
+    case 32: hipLaunchKernelGGL( ( k_spmv_grp<T, T, 32> ), grid, dim3( 256 ), 0, st, n, cp, rt, v, x, y, ld, nrhs, alpha, beta ); break;
+    default: hipLaunchKernelGGL( ( k_spmv_grp<T, T, 64> ), grid, dim3( 256 ), 0, st, n, cp, rt, v, x, y, ld, nrhs, alpha, beta ); break;
+    }
+}
+}
+
 int
 pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy, int64_t nrhs, int64_t ld )
 {
@@ -2197,15 +2173,17 @@ pxd_spmv( pxd_ctx_t *ctx, double alpha, const void *dx, double beta, void *dy, i
         rt = ctx->dtrowtab;
         vv = ctx->dtval;
     }
+    // lanes per row: the power of two at or above the mean entries per row
+    const double avg = ctx->n > 0 ? (double)ctx->nnz / (double)ctx->n : 1.;
+    const int    G   = avg <= 4. ? 4 : avg <= 8. ? 8 : avg <= 16. ? 16 : avg <= 32. ? 32 : 64;
+    const dim3   grid( (unsigned)( ( ctx->n * G + 255 ) / 256 ) );
     if ( ctx->mflt == FT_DOUBLE ) {
-        hipLaunchKernelGGL( ( k_spmv<double, double> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
-                            ctx->n, cp, rt, (const double *)vv, (const double *)dx, (double *)dy, ld, (int)nrhs,
-                            alpha, beta );
+        spmv_launch<double>( G, grid, ctx->stream, ctx->n, cp, rt, (const double *)vv, (const double *)dx,
+                             (double *)dy, ld, (int)nrhs, alpha, beta );
     }
     else {
-        hipLaunchKernelGGL( ( k_spmv<pxcd, pxcd> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, ctx->stream,
-                            ctx->n, cp, rt, (const pxcd *)vv, (const pxcd *)dx, (pxcd *)dy, ld, (int)nrhs,
-                            alpha, beta );
+        spmv_launch<pxcd>( G, grid, ctx->stream, ctx->n, cp, rt, (const pxcd *)vv, (const pxcd *)dx, (pxcd *)dy,
+                           ld, (int)nrhs, alpha, beta );
     }
     PXD_CHECK( hipGetLastError() );
     return 0;

@@ -2281,163 +2281,3 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     stamp_end( stamps, ti, t_stamp );
 }
 
-// ---------------------------------------------------------------------------
-// Diagonal solve of one PXD_DIAG_ROWS-row chunk of a cblk, out of place:
-// Y[R0..R0+15] <- (Linv or Linv^T)[R0.., :] Xin_k, so that the chunks of a
-// cblk run in parallel workgroups (tasks {cblk, R0}).  Thread (r, g) sums
-// the columns g, g+16, ... of output row R0+r; every load of the workgroup
-// is issued at once (a cblk is at most PXD_TRSM_NT * 16 wide), and the
-// lanes of a 16-thread group read 16 consecutive elements (rows of Linv
-// forward, columns of Linv, i.e. rows of Linv^T, backward).
-#define PXD_DIAG_ROWS 16
-template <typename T, typename V>
-__global__ void __launch_bounds__( PXD_THREADS )
-k_solve_diag_chunk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const T *__restrict__ Linv,
-                    const V *__restrict__ Xin, int64_t ldx, V *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
-{
-    constexpr int CR   = PXD_DIAG_ROWS;
-    constexpr int NG   = PXD_THREADS / CR;          // column phases
-    constexpr int MAXU = ( PXD_TRSM_NT * 16 + NG - 1 ) / NG;
-    extern __shared__ unsigned char smem_raw[];
-    V *sb = reinterpret_cast<V *>( smem_raw );      // [n][nrhs]
-    __shared__ V sred[NG][CR][PXD_MAXRHS + 1];
-
-    const row_task tk  = tasks[xcd_task( blockIdx.x, gridDim.x )];
-    const dcblk    c   = cb[tk.cblk];
-    const int32_t  n   = c.lcolnum - c.fcolnum + 1;
-    const int32_t  R0  = tk.r0;
-    const int      tid = threadIdx.x;
-    const T       *LI  = Linv + c.dinvoff;
-    const V       *x   = Xin + c.fcolnum;
-    const int      r   = trans ? tid / NG : tid % CR;
-    const int      g   = trans ? tid % NG : tid / CR;
-    const int32_t  i   = R0 + r;
-
-    T a[MAXU];
-#pragma unroll
-    for ( int u = 0; u < MAXU; u++ ) {
-        const int32_t j  = g + NG * u;
-        const bool    ok = i < n && j < n && ( trans ? j >= i : j <= i );
-        a[u] = ok ? LI[trans ? j + (int64_t)i * n : i + (int64_t)j * n] : (T)0;
-    }
-    for ( int idx = tid; idx < n * nrhs; idx += PXD_THREADS ) {
-        const int rr = idx % n, q = idx / n;
-        sb[rr * nrhs + q] = x[rr + q * ldx];
-    }
-    __syncthreads();
-
-    V acc[PXD_MAXRHS];
-#pragma unroll
-    for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (V)0;
-#pragma unroll
-    for ( int u = 0; u < MAXU; u++ ) {
-        const int32_t j = g + NG * u;
-        if ( j < n ) {
-            const V av = px_cvt<V>( a[u] );
-#pragma unroll
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) {
-                if ( q < nrhs ) acc[q] += av * sb[j * nrhs + q];
-            }
-        }
-    }
-#pragma unroll
-    for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[g][r][q] = acc[q];
-    __syncthreads();
-    if ( tid < CR * nrhs ) {
-        const int r2 = tid % CR, q = tid / CR;
-        V         v  = (V)0;
-#pragma unroll
-        for ( int p2 = 0; p2 < NG; p2++ ) v += sred[p2][r2][q];
-        if ( R0 + r2 < n ) Yout[c.fcolnum + R0 + r2 + q * ldy] = v;
-    }
-}
-
-// Backward gather of a 64-row tile: x_k -= A(rows,:)^T X[grow].  The tile is
-// staged through LDS by 64-column chunks (coalesced loads), four threads per
-// column reduce 16 rows each, one atomic per column and RHS.
-template <typename T, typename V, int NCH>
-__global__ void __launch_bounds__( PXD_THREADS )
-k_solve_bwd_gather2( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
-                     const dblok *__restrict__ bk, const T *__restrict__ L, const V *X,
-                     int64_t ldx, V *Yt, int64_t ldt, int nrhs )
-{
-    __shared__ T       sA[64][64 + 1];           // sA[col][row]
-    __shared__ V       sx[64][PXD_MAXRHS];
-    __shared__ V       sred[4][64][PXD_MAXRHS + 1];
-    __shared__ int64_t sBase[64];
-    __shared__ int32_t sLd[64];
-
-    const row_task t   = tasks[xcd_task( blockIdx.x, gridDim.x )];
-    const dcblk    c   = cb[t.cblk];
-    const int32_t  n   = c.lcolnum - c.fcolnum + 1;
-    const int32_t  lb  = cb[t.cblk + 1].fblok;
-    const int      tid = threadIdx.x;
-    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
-    stage_row_keys( skeys, bk, c.fblok, lb );
-    __syncthreads();
-
-    if ( tid < 64 ) {
-        int32_t s = t.r0 + tid;
-        if ( s < c.stride ) {
-            int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
-            dblok   bb = bk[b];
-            sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
-            sLd[tid]   = bb.lrownum - bb.frownum + 1;
-            int32_t g  = bb.frownum + ( s - bb.rowoff );
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = ( q < nrhs ) ? X[g + q * ldx] : (V)0;
-        }
-        else {
-            sBase[tid] = -1;
-            sLd[tid]   = 0;
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) sx[tid][q] = (V)0;
-        }
-    }
-    __syncthreads();
-    const int col = tid & 63, part = tid >> 6;
-    // NCH 64-column chunks are loaded at once, then staged through LDS one
-    // after the other (NCH = 1 for launches with many workgroups)
-    constexpr int NU  = 64 * 64 / PXD_THREADS;
-    const int     rr0 = tid & 63;
-    const int64_t rb  = sBase[rr0];
-    const int32_t rl  = sLd[rr0];
-    T             v[NCH][NU];
-    for ( int32_t c0 = 0; c0 < n; c0 += 64 ) {
-        const int ch = ( c0 / 64 ) % NCH;
-        if ( ch == 0 ) {
-#pragma unroll
-            for ( int h = 0; h < NCH; h++ )
-#pragma unroll
-                for ( int u = 0; u < NU; u++ ) {
-                    const int32_t cc = c0 + 64 * h + ( tid >> 6 ) + 4 * u;
-                    v[h][u] = ( rb >= 0 && cc < n ) ? L[rb + (int64_t)cc * rl] : (T)0;
-                }
-        }
-#pragma unroll
-        for ( int h = 0; h < NCH; h++ ) {
-            if ( h == ch ) {
-#pragma unroll
-                for ( int u = 0; u < NU; u++ ) sA[( tid >> 6 ) + 4 * u][rr0] = v[h][u];
-            }
-        }
-        __syncthreads();
-        V acc[PXD_MAXRHS];
-#pragma unroll
-        for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] = (V)0;
-#pragma unroll
-        for ( int r = 16 * part; r < 16 * part + 16; r++ ) {
-            const V a = px_cvt<V>( sA[col][r] );
-#pragma unroll
-            for ( int q = 0; q < PXD_MAXRHS; q++ ) acc[q] += a * sx[r][q];
-        }
-#pragma unroll
-        for ( int q = 0; q < PXD_MAXRHS; q++ ) sred[part][col][q] = acc[q];
-        __syncthreads();
-        if ( part == 0 && c0 + col < n ) {
-            for ( int q = 0; q < nrhs; q++ ) {
-                V v = sred[0][col][q] + sred[1][col][q] + sred[2][col][q] + sred[3][col][q];
-                px_atomic_add( Yt + c.fcolnum + c0 + col + q * ldt, -v );
-            }
-        }
-        __syncthreads();
-    }
-}

@@ -222,27 +222,39 @@ k_solve_bwd_gather( const row_task *__restrict__ tasks, const dcblk *__restrict_
 // ---------------------------------------------------------------------------
 // SpMV on the full-pattern CSC: column j of the stored matrix gives row j of
 // the operator (symmetric: A itself; general: the transposed CSC is passed).
-// y = alpha * A x + beta * y for nrhs columns (real alpha / beta).  One
-// thread per column of the stored CSC (a row of A); the column's entries
-// are read once for up to PXD_SPMV_NR right-hand sides.
+// y = alpha * A x + beta * y for nrhs columns (real alpha / beta); the
+// column's entries are read once for up to PXD_SPMV_NR right-hand sides.
 #define PXD_SPMV_NR 8
-template <typename T, typename TV>
+// G lanes per row (G a power of two up to 64, about the average entries per
+// stored column): the lanes of a group read consecutive entries of the
+// column, so a wave's loads of values and row indices are contiguous, and
+// the G partial sums meet by shuffles.
+__device__ inline double spmv_shfl_xor( double v, int m ) { return __shfl_xor( v, m, 64 ); }
+__device__ inline pxcd   spmv_shfl_xor( pxcd v, int m )
+{
+    return pxcd{ __shfl_xor( v.re, m, 64 ), __shfl_xor( v.im, m, 64 ) };
+}
+
+template <typename T, typename TV, int G>
 __global__ void __launch_bounds__( 256 )
-k_spmv( int64_t n, const int64_t *__restrict__ colptr, const int64_t *__restrict__ rowtab,
-        const T *__restrict__ val, const TV *__restrict__ x, TV *__restrict__ y, int64_t ld,
-        int nrhs, double alpha, double beta )
+k_spmv_grp( int64_t n, const int64_t *__restrict__ colptr, const int64_t *__restrict__ rowtab,
+            const T *__restrict__ val, const TV *__restrict__ x, TV *__restrict__ y, int64_t ld,
+            int nrhs, double alpha, double beta )
 {
     typedef typename px_real<TV>::type RV;
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if ( j >= n ) {
-        return;
+    const int64_t j   = ( (int64_t)blockIdx.x * blockDim.x + threadIdx.x ) / G;
+    const int     sub = threadIdx.x % G;
+    // rows past n take part in the shuffles with no entries
+    int64_t e0 = 0, e1 = 0;
+    if ( j < n ) {
+        e0 = colptr[j];
+        e1 = colptr[j + 1];
     }
-    const int64_t e0 = colptr[j], e1 = colptr[j + 1];
     for ( int q0 = 0; q0 < nrhs; q0 += PXD_SPMV_NR ) {
         TV s[PXD_SPMV_NR];
 #pragma unroll
         for ( int q = 0; q < PXD_SPMV_NR; q++ ) s[q] = px_zero( TV() );
-        for ( int64_t e = e0; e < e1; e++ ) {
+        for ( int64_t e = e0 + sub; e < e1; e += G ) {
             const T       v = val[e];
             const int64_t r = rowtab[e];
 #pragma unroll
@@ -252,9 +264,16 @@ k_spmv( int64_t n, const int64_t *__restrict__ colptr, const int64_t *__restrict
         }
 #pragma unroll
         for ( int q = 0; q < PXD_SPMV_NR; q++ ) {
-            if ( q0 + q < nrhs ) {
-                TV *yy = y + j + ( q0 + q ) * ld;
-                *yy = ( beta == 0. ) ? s[q] * (RV)alpha : s[q] * (RV)alpha + *yy * (RV)beta;
+#pragma unroll
+            for ( int m = G / 2; m > 0; m >>= 1 ) s[q] += spmv_shfl_xor( s[q], m );
+        }
+        if ( j < n && sub == 0 ) {
+#pragma unroll
+            for ( int q = 0; q < PXD_SPMV_NR; q++ ) {
+                if ( q0 + q < nrhs ) {
+                    TV *yy = y + j + ( q0 + q ) * ld;
+                    *yy = ( beta == 0. ) ? s[q] * (RV)alpha : s[q] * (RV)alpha + *yy * (RV)beta;
+                }
             }
         }
     }

@@ -1,0 +1,256 @@
+// kernels_svsolve.hpp -- level-by-level supernodal solve kernels with few
+// dependent memory round trips (reference kernels/solve_ztrsmsp.c:300-424
+// forward, :449-625 backward).
+//
+// A workgroup of the older kernels looked its rows up at run time: it read
+// the task, then the cblk, then staged the cblk's blok keys and searched
+// them, then read the blok, and only then loaded the factor, about ten
+// dependent trips to memory (~20 us per workgroup measured, which bounded
+// every level of the upper tree).  Here the tasks carry the cblk's sizes,
+// each off-diagonal row has a precomputed record (factor offset, column
+// stride, global row), and the loads that do not depend on each other are
+// issued together:
+//   trip 1  the task;
+//   trip 2  the row records and the right-hand-side block (staged to LDS);
+//   trip 3  the factor entries (all columns of a batch in flight);
+// then the products, one reduction through LDS and the atomics.
+#pragma once
+#include "kernels_mfma.hpp"
+
+// A 64-row tile of the stacked off-diagonal rows of a cblk: rows r0 .. r0+63
+// (below the diagonal block, r0 >= n) have their records at row0 ...
+struct sv_tile {
+    int32_t n;         // columns of the cblk
+    int32_t fcolnum;   // its first column
+    int32_t r0;        // first stacked row of the tile
+    int32_t stride;    // stacked rows of the cblk
+    int64_t row0;      // record of stacked row r0
+};
+// ... one record per off-diagonal stacked row
+struct sv_row {
+    int64_t base;   // offset of the row's first column in the factor
+    int32_t ld;     // column stride of its blok
+    int32_t g;      // its global row (column of the unknowns)
+};
+// A PXD_DIAG_ROWS-row chunk of a diagonal block's inverse
+#define PXD_DIAG_ROWS 16
+struct sv_chunk {
+    int32_t n;
+    int32_t fcolnum;
+    int32_t R0;
+    int32_t pad;
+    int64_t dinvoff;
+};
+
+#define PXD_SV_MAXN ( PXD_TRSM_NT * 16 )   // widest cblk (352 columns)
+
+// Stage columns [0, n) of the NR right-hand sides of v (rows fcolnum..,
+// leading dimension ld) as s[j * NR + q] (q >= nrhs read 0).  Every load is
+// issued before the first LDS store.
+template <typename V, int NR>
+__device__ inline void
+sv_stage( V *s, const V *__restrict__ v, int64_t ld, int32_t n, int nrhs, int tid )
+{
+    constexpr int MAXE = ( PXD_SV_MAXN * NR + PXD_THREADS - 1 ) / PXD_THREADS;
+    V             t[MAXE];
+#pragma unroll
+    for ( int e = 0; e < MAXE; e++ ) {
+        const int idx = tid + PXD_THREADS * e;
+        const int j = idx % n, q = idx / n;
+        t[e] = ( idx < n * NR && q < nrhs ) ? v[j + q * ld] : (V)0;
+    }
+#pragma unroll
+    for ( int e = 0; e < MAXE; e++ ) {
+        const int idx = tid + PXD_THREADS * e;
+        if ( idx < n * NR ) s[( idx % n ) * NR + idx / n] = t[e];
+    }
+}
+
+// Forward tile: X[g(s)] -= A(s, :) y_k for its 64 rows, y_k = Y[fcolnum ..].
+// Lane r of each wave takes row r0 + r; wave w the columns [n w / 4,
+// n (w + 1) / 4), B at a time.  Dynamic LDS: n * NR values of V.
+template <typename T, typename V, int NR, int B>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ rows, const T *__restrict__ L,
+               V *X, int64_t ldx, const V *__restrict__ Y, int64_t ldy, int nrhs )
+{
+    extern __shared__ unsigned char smem_raw[];
+    V *sy = reinterpret_cast<V *>( smem_raw );
+    __shared__ V red[4][64][NR];
+
+    const sv_tile t   = tiles[xcd_task( blockIdx.x, gridDim.x )];
+    const int     tid = threadIdx.x;
+    const int     r = tid & 63, part = tid >> 6;
+    const bool    valid = t.r0 + r < t.stride;
+    sv_row        ri{ 0, 0, 0 };
+    if ( valid ) {
+        ri = rows[t.row0 + r];
+    }
+    sv_stage<V, NR>( sy, Y + t.fcolnum, ldy, t.n, nrhs, tid );
+    V acc[NR];
+#pragma unroll
+    for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
+    const int32_t c0 = ( t.n * part ) / 4, c1 = ( t.n * ( part + 1 ) ) / 4;
+    __syncthreads();
+    if ( valid ) {
+        for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
+            T a[B];
+#pragma unroll
+            for ( int u = 0; u < B; u++ ) a[u] = ( cb0 + u < c1 ) ? L[ri.base + (int64_t)( cb0 + u ) * ri.ld] : (T)0;
+#pragma unroll
+            for ( int u = 0; u < B; u++ ) {
+                if ( cb0 + u < c1 ) {
+                    const V av = px_cvt<V>( a[u] );
+#pragma unroll
+                    for ( int q = 0; q < NR; q++ ) acc[q] += av * sy[( cb0 + u ) * NR + q];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for ( int q = 0; q < NR; q++ ) red[part][r][q] = acc[q];
+    __syncthreads();
+    if ( part == 0 && valid ) {
+#pragma unroll
+        for ( int q = 0; q < NR; q++ ) {
+            if ( q < nrhs ) {
+                const V v = red[0][r][q] + red[1][r][q] + red[2][r][q] + red[3][r][q];
+                px_atomic_add( X + ri.g + q * ldx, -v );
+            }
+        }
+    }
+}
+
+// Backward tile: Y[fcolnum + j] -= sum over its rows s of A(s, j) X[g(s)].
+// The tile passes through LDS in 64-column chunks (coalesced loads along the
+// rows); four threads per column reduce 16 rows each; NCH chunks are loaded
+// at once (all of them for launches with few workgroups).
+template <typename T, typename V, int NR, int NCH>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ rows, const T *__restrict__ L,
+               const V *__restrict__ X, int64_t ldx, V *Y, int64_t ldy, int nrhs )
+{
+    __shared__ T sA[64][64 + 1];   // sA[col][row]
+    __shared__ V sx[64][NR];
+    __shared__ V red[4][64][NR + 1];
+
+    const sv_tile t   = tiles[xcd_task( blockIdx.x, gridDim.x )];
+    const int     tid = threadIdx.x;
+    const int     rr = tid & 63, part = tid >> 6;
+    const bool    valid = t.r0 + rr < t.stride;
+    sv_row        ri{ 0, 0, 0 };
+    if ( valid ) {
+        ri = rows[t.row0 + rr];
+    }
+    // X of the tile's rows (wave 0) and the first NCH chunks of the tile
+    if ( part == 0 ) {
+        V xv[NR];
+#pragma unroll
+        for ( int q = 0; q < NR; q++ ) xv[q] = ( valid && q < nrhs ) ? X[ri.g + q * ldx] : (V)0;
+#pragma unroll
+        for ( int q = 0; q < NR; q++ ) sx[rr][q] = xv[q];
+    }
+    constexpr int NU = 64 * 64 / PXD_THREADS;
+    T             v[NCH][NU];
+    for ( int32_t c0 = 0; c0 < t.n; c0 += 64 ) {
+        const int ch = ( c0 / 64 ) % NCH;
+        if ( ch == 0 ) {
+#pragma unroll
+            for ( int h = 0; h < NCH; h++ )
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) {
+                    const int32_t cc = c0 + 64 * h + part + 4 * u;
+                    v[h][u] = ( valid && cc < t.n ) ? L[ri.base + (int64_t)cc * ri.ld] : (T)0;
+                }
+        }
+#pragma unroll
+        for ( int h = 0; h < NCH; h++ ) {
+            if ( h == ch ) {
+#pragma unroll
+                for ( int u = 0; u < NU; u++ ) sA[part + 4 * u][rr] = v[h][u];
+            }
+        }
+        __syncthreads();
+        const int col = rr;
+        V         acc[NR];
+#pragma unroll
+        for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
+#pragma unroll
+        for ( int r = 16 * part; r < 16 * part + 16; r++ ) {
+            const V a = px_cvt<V>( sA[col][r] );
+#pragma unroll
+            for ( int q = 0; q < NR; q++ ) acc[q] += a * sx[r][q];
+        }
+#pragma unroll
+        for ( int q = 0; q < NR; q++ ) red[part][col][q] = acc[q];
+        __syncthreads();
+        if ( part == 0 && c0 + col < t.n ) {
+#pragma unroll
+            for ( int q = 0; q < NR; q++ ) {
+                if ( q < nrhs ) {
+                    const V s = red[0][col][q] + red[1][col][q] + red[2][col][q] + red[3][col][q];
+                    px_atomic_add( Y + t.fcolnum + c0 + col + q * ldy, -s );
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Rows R0 .. R0 + 15 of out = Linv in (forward, lower) or Linv^T in
+// (backward, upper), in taken from Xin (rows fcolnum ..).  Thread (r, g) sums
+// columns g, g + 16, ... of row R0 + r.  The inverse's entries and the
+// right-hand sides are loaded together.  Dynamic LDS: n * NR values of V.
+template <typename T, typename V, int NR>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_sv_diag( const sv_chunk *__restrict__ chunks, const T *__restrict__ Linv, const V *__restrict__ Xin,
+           int64_t ldx, V *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
+{
+    constexpr int CR   = PXD_DIAG_ROWS;
+    constexpr int NG   = PXD_THREADS / CR;   // column phases
+    constexpr int MAXU = ( PXD_SV_MAXN + NG - 1 ) / NG;
+    extern __shared__ unsigned char smem_raw[];
+    V *sb = reinterpret_cast<V *>( smem_raw );
+    __shared__ V sred[NG][CR][NR + 1];
+
+    const sv_chunk ch  = chunks[xcd_task( blockIdx.x, gridDim.x )];
+    const int32_t  n   = ch.n;
+    const int      tid = threadIdx.x;
+    const T       *LI  = Linv + ch.dinvoff;
+    const int      r   = trans ? tid / NG : tid % CR;
+    const int      g   = trans ? tid % NG : tid / CR;
+    const int32_t  i   = ch.R0 + r;
+
+    T a[MAXU];
+#pragma unroll
+    for ( int u = 0; u < MAXU; u++ ) {
+        const int32_t j  = g + NG * u;
+        const bool    ok = i < n && j < n && ( trans ? j >= i : j <= i );
+        a[u] = ok ? LI[trans ? j + (int64_t)i * n : i + (int64_t)j * n] : (T)0;
+    }
+    sv_stage<V, NR>( sb, Xin + ch.fcolnum, ldx, n, nrhs, tid );
+    __syncthreads();
+
+    V acc[NR];
+#pragma unroll
+    for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
+#pragma unroll
+    for ( int u = 0; u < MAXU; u++ ) {
+        const int32_t j = g + NG * u;
+        if ( j < n ) {
+            const V av = px_cvt<V>( a[u] );
+#pragma unroll
+            for ( int q = 0; q < NR; q++ ) acc[q] += av * sb[j * NR + q];
+        }
+    }
+#pragma unroll
+    for ( int q = 0; q < NR; q++ ) sred[g][r][q] = acc[q];
+    __syncthreads();
+    if ( tid < CR * NR ) {
+        const int r2 = tid % CR, q = tid / CR;
+        V         v  = (V)0;
+#pragma unroll
+        for ( int p2 = 0; p2 < NG; p2++ ) v += sred[p2][r2][q];
+        if ( ch.R0 + r2 < n && q < nrhs ) Yout[ch.fcolnum + ch.R0 + r2 + q * ldy] = v;
+    }
+}
