@@ -68,7 +68,7 @@ def build_library(verbose=False):
     objs = [_obj(s) for s in host_srcs + dev_srcs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-         ["-lpthread", "-lm"])
+         ["-lpthread", "-lm", "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

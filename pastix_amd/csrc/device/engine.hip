@@ -12,6 +12,7 @@
 // with plain read-modify-write, which is what the reference's per-cblk lock
 // serializes (kernels/core_zgemmsp.c:462,470).
 #include <algorithm>
+#include <dlfcn.h>
 #include <cfloat>
 #include <chrono>
 #include <cstdio>
@@ -127,6 +128,7 @@ struct pxd_ctx_s {
     std::vector<solve_graph> sgraphs;
     uint64_t sched_gen   = 0;      // bumped when the task lists are rebuilt
     int      solve_graph = 1;
+    std::string tuning_file;       // the tuning file applied at creation, if any
     size_t dvecs_bytes = 0;
     void  *dsolvy = nullptr;       // the sweeps' second vector (n x PXD_MAXRHS, factor precision)
     size_t dsolvy_bytes = 0;
@@ -1592,6 +1594,47 @@ pxd_device_count( void )
     return n;
 }
 
+// Engine options chosen by the autotuner (pastix_amd/autotune.py): a text
+// file of "key value" lines ('#' starts a comment), read at every
+// pxd_create from $PXD_TUNING_FILE, or else from tuning_mi355x.txt next to
+// this library.  Each line is applied with pxd_set_option; a missing file
+// leaves the built-in defaults, an unknown key is reported and skipped.
+static void
+load_tuning( pxd_ctx_t *ctx )
+{
+    std::string path;
+    if ( const char *env = getenv( "PXD_TUNING_FILE" ) ) {
+        path = env;
+    }
+    else {
+        Dl_info info;
+        if ( dladdr( (void *)&load_tuning, &info ) && info.dli_fname ) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind( '/' );
+            path = ( slash == std::string::npos ? std::string( "." ) : path.substr( 0, slash ) ) +
+                   "/tuning_mi355x.txt";
+        }
+    }
+    FILE *f = path.empty() ? nullptr : fopen( path.c_str(), "r" );
+    if ( f == nullptr ) {
+        return;
+    }
+    char line[256];
+    while ( fgets( line, sizeof( line ), f ) ) {
+        char  *hash = strchr( line, '#' );
+        char   key[128];
+        double value;
+        if ( hash ) {
+            *hash = '\0';
+        }
+        if ( sscanf( line, "%127s %lf", key, &value ) == 2 && pxd_set_option( ctx, key, value ) != 0 ) {
+            fprintf( stderr, "pastix_amd device: %s: unknown option %s\n", path.c_str(), key );
+        }
+    }
+    fclose( f );
+    ctx->tuning_file = path;
+}
+
 int
 pxd_create( pxd_ctx_t **pctx, int device )
 {
@@ -1625,9 +1668,11 @@ pxd_create( pxd_ctx_t **pctx, int device )
     set_lds_attributes_mfma<pxcd>( ctx->lds_max );
     set_lds_attributes_mfma<pxcf>( ctx->lds_max );
     (void)hipGetLastError();   // an attribute refused above must not be reported later
+    load_tuning( ctx );
     if ( getenv( "PXD_VERBOSE" ) ) {
-        fprintf( stderr, "pastix_amd device: %s, %d CUs, LDS/block %zu, L2 %d\n", prop.gcnArchName,
-                 prop.multiProcessorCount, ctx->lds_max, prop.l2CacheSize );
+        fprintf( stderr, "pastix_amd device: %s, %d CUs, LDS/block %zu, L2 %d, tuning %s\n", prop.gcnArchName,
+                 prop.multiProcessorCount, ctx->lds_max, prop.l2CacheSize,
+                 ctx->tuning_file.empty() ? "(defaults)" : ctx->tuning_file.c_str() );
     }
     *pctx = ctx;
     return 0;
