@@ -85,25 +85,21 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "model_log"     1: kernel timing log of the next factorization
  *   "kernels"       1: matrix-core kernels (default); 0: scalar kernels
  *   "panel_mw"      1: multi-workgroup Cholesky panels (default)
- *   "panel_reg"     single precision: levels with cblks this wide or wider
- *                   factor each diagonal tile in one workgroup's registers
- *                   (0: never, the default)
  *   "upd_big_rows"  stacked rows from which updates use the tall tiles
  *   "upd_small"     1: one-wavefront small updates (default)
- *   "upd_nh"        k-groups per stage of the tall-tile update (1 or 2)
  *   "prio"          1: raised wave priority on the critical path (default)
  *   "solve_graph"   1: each solve pass replays the sweeps' launches from a
  *                   graph captured on first use (default); 0: launched one by one
  *   "solve_wide_max" level-solve launches with at least this many tiles load
  *                   16 columns per batch (default 128)
- *   "solve_vec_double" 1: a single factor solves double vectors directly;
- *                   0: vectors rounded to single and back (default)
+ *   "wait_ms"       bound of the multi-workgroup panel's waits, in ms (2000);
+ *                   a wait that reaches it gives up and the factorization
+ *                   returns PASTIX_ERR_INTERNAL.  It is one value per device
+ *                   (all its contexts)
  *   "schur_solv_mode" 0: local, 1: interface, 2: whole matrix (see pxd_set_schur)
- *   "trsm_blk"      real Cholesky, N > 0: levels with at most N TRSM slabs
- *                   of 64 rows do the off-diagonal TRSM by blocked
- *                   substitution on the 32 x 32 diagonal-block inverses,
- *                   the full-tile inversion (read by the solve only)
- *                   launched after it; 0: TRSM with the full-tile inverse
+ * The file tuning_mi355x.txt next to the library (or $PXD_TUNING_FILE) sets
+ * options at context creation ("key value" lines, written by
+ * pastix_amd/autotune.py).
  */
 int  pxd_set_option( pxd_ctx_t *ctx, const char *key, double value );
 
@@ -256,7 +252,22 @@ int pxd_matrix_norm_inf( pxd_ctx_t *ctx, double *norm );
 typedef struct pxd_model_entry_s {
     int32_t ktype, m, n, k;
     double  time;
+    int32_t variant;   /* blok updates: the kernel class that ran them (0: 64 x 64
+                          tiles, 1: tall tiles, 2: one wavefront); -1 otherwise */
+    int32_t pad;
 } pxd_model_entry_t;
+
+/*
+ * Kernel class of each blok update, chosen by the caller (for instance from
+ * predicted times, pastix_amd_choose_update_kernels): cls[2 * b + side] for
+ * the update from blok b (side 1: the U side of an LU factor), with the
+ * values of pxd_model_entry_t.variant, or -1 to keep the built-in rule
+ * (the cut-offs upd_small / upd_big_rows).  A class the update does not fit
+ * (the one-wavefront kernel takes at most 64 rows, 16 columns and 32 k
+ * values) falls back to the rule.  count must be 2 * bloknbr, or 0 to drop
+ * the choices.  Takes effect at the next factorization.
+ */
+int pxd_set_update_classes( pxd_ctx_t *ctx, const int8_t *cls, int64_t count );
 int pxd_model_log( const pxd_ctx_t *ctx, pxd_model_entry_t *entries, int64_t max, int64_t *count );
 
 /*

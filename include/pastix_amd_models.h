@@ -84,9 +84,11 @@ int pastix_amd_models_write( const pastix_model_t *model, const char *filename )
 /*
  * Learned model: the GPU multilayer perceptron of pastix_amd_device.h
  * (pxd_mlp_*) fitted to the natural log of the times of timing-log records.
- * Its 8 inputs, standardised over the training records, are the kernel
- * family (diagonal block, TRSM, update) as three 0/1 values, ln(1+m),
- * ln(1+n), ln(1+k), ln(1+flops) of the record's sizes and the arithmetic.
+ * Its 10 inputs, standardised over the training records, are the kernel
+ * family as five 0/1 values (diagonal block, TRSM, and the update on each of
+ * its three kernel classes: 64 x 64 tiles, tall tiles, one wavefront, from
+ * the record's variant), ln(1+m), ln(1+n), ln(1+k), ln(1+flops) of the
+ * record's sizes and the arithmetic.
  * Training runs steps Adam steps, the rate lr divided by 3 after each
  * quarter; *loss gets the final mean squared error of the standardised log
  * times.
@@ -99,6 +101,22 @@ int  pastix_amd_learned_train  ( pastix_amd_learned_t **lm, int device, int arit
 int  pastix_amd_learned_predict( pastix_amd_learned_t *lm, int arith, const pxd_model_entry_t *entries,
                                  pastix_int_t count, double *time );
 void pastix_amd_learned_free   ( pastix_amd_learned_t *lm );
+
+/*
+ * Kernel class of every blok update chosen by the smallest predicted cost:
+ * with the learned model lm, else with poly[0..2], the polynomial models
+ * (GEMMCblk2d2d entries) of the three classes (64 x 64 tiles, tall tiles,
+ * one wavefront); with neither, the engine's cut-offs are restored.  The
+ * cost is the predicted time alone on the device times the fraction of the
+ * device's CUs the class's workgroups fill (the updates of a level share
+ * the device).  The
+ * reference's schedulers consult the same kind of prediction per update
+ * (modelsGetCost3Param, sopalin/starpu/codelet_cblk_zgemmsp.c:137-170).
+ * *predicted (optional) gets the predicted total cost of the choice.
+ * Call after pastix_subtask_bcsc2ctab; applies from the next factorization.
+ */
+int  pastix_amd_choose_update_kernels( pastix_data_t *pd, pastix_amd_learned_t *lm, pastix_model_t *const *poly,
+                                       int arith, double *predicted );
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,7 @@ struct pxd_ctx_s {
     uint64_t sched_gen   = 0;      // bumped when the task lists are rebuilt
     int      solve_graph = 1;
     std::string tuning_file;       // the tuning file applied at creation, if any
+    int         clock_khz = 0;     // rate of the device's constant clock
     size_t dvecs_bytes = 0;
     void  *dsolvy = nullptr;       // the sweeps' second vector (n x PXD_MAXRHS, factor precision)
     size_t dsolvy_bytes = 0;
@@ -141,11 +142,10 @@ struct pxd_ctx_s {
     int          kernels = 1;
     // blok updates with at least this many stacked rows use the tall tiles
     int          upd_big_rows = 192;
+    std::vector<int8_t> upd_cls;   // per (blok, side): kernel class chosen by the caller, -1: the rule
     // 1: updates of at most 64 rows, PXD_SMALL_N columns and PXD_SMALL_K
     // source columns take one wavefront each (k_update_small)
     int          upd_small = 1;
-    // k values per stage of the tall-tile update: 4 * KV * upd_nh (1 or 2)
-    int          upd_nh = 1;
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
 
@@ -157,13 +157,6 @@ struct pxd_ctx_s {
     std::vector<inv_task>  hinvt;
     std::vector<upd_group> groups;
 
-    // 1: a single factor solves double vectors directly (entries widened as
-    // they are read); 0 (default): the vectors are converted to single and
-    // back, as in the reference.  On the 100^3 Laplacian the double vectors
-    // lower the first solve's backward error from 4.5e-7 to 2.7e-7 but the
-    // refinement still takes two iterations, and the solve is twice as slow.
-    int       solve_vec_double = 0;
-    int       trsm_blk = 0;     // real Cholesky: levels with at most this many TRSM slabs use the blocked TRSM
     // level solve launches with at least this many tiles load 16 columns
     // per batch (higher occupancy); smaller ones load every column at once
     int64_t   solve_wide_max = PXD_SOLVE_WIDE_MAX;
@@ -186,11 +179,6 @@ struct pxd_ctx_s {
     // 1: Cholesky diagonal tiles factored by one workgroup per 32-row block
     // (k_potrf_mw); 0: one workgroup per cblk (k_potrf2)
     int          panel_mw = 1;
-    // single-precision Cholesky levels whose widest cblk has at least this
-    // many columns factor each diagonal tile in one workgroup's registers
-    // (k_potrf_reg); 0: never (default: a 330-column tile takes ~245 us on
-    // one CU against ~240 us over eleven workgroups)
-    int          panel_reg = 0;
     // 1: the factorization runs one stream, launches every blok update on
     // its own and records per-workgroup times into model_entries
     int          model_log = 0;
@@ -411,7 +399,14 @@ build_schedule( pxd_ctx_t *ctx )
                         }
                         const int32_t fl = side ? 2 : 0;
                         const int32_t K  = c.lcolnum - c.fcolnum + 1;
-                        if ( big_ok && ctx->upd_small && M <= 64 && N <= PXD_SMALL_N && K <= PXD_SMALL_K ) {
+                        // kernel class: the caller's choice when it fits the
+                        // update, else the cut-offs
+                        const bool small_fits = big_ok && M <= 64 && N <= PXD_SMALL_N && K <= PXD_SMALL_K;
+                        int        cls        = ctx->upd_cls.empty() ? -1 : ctx->upd_cls[2 * (size_t)b + side];
+                        if ( ( cls == 2 && !small_fits ) || ( cls == 1 && !big_ok ) || cls < 0 || cls > 2 ) {
+                            cls = ( small_fits && ctx->upd_small ) ? 2 : ( big_ok && M >= ctx->upd_big_rows ) ? 1 : 0;
+                        }
+                        if ( cls == 2 ) {
                             // one wavefront for the whole update
                             int64_t first = upds.size();
                             upds.push_back( upd_task{ k, b, 0, 0, fl, 0 } );
@@ -419,7 +414,7 @@ build_schedule( pxd_ctx_t *ctx )
                                                (int64_t)upds.size() } );
                             ctx->groups.push_back( upd_group{ k, b, M, N, K, 2, side, first, (int64_t)upds.size() } );
                         }
-                        else if ( big_ok && M >= ctx->upd_big_rows ) {
+                        else if ( cls == 1 ) {
                             int64_t first = updb.size();
                             // column tiles fastest: the tiles that share a row
                             // panel run together and share its L2 lines
@@ -622,7 +617,7 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
             const int32_t n = c.lcolnum - c.fcolnum + 1;
             double        t = 0.;
             if ( mw ? span( sp, qb[k], qe[k], t ) : span( sp, i, i + 1, t ) ) {
-                out.push_back( pxd_model_entry_t{ kpanel, n, 0, 0, t } );
+                out.push_back( pxd_model_entry_t{ kpanel, n, 0, 0, t, -1, 0 } );
             }
             if ( c.stride == n ) {
                 continue;
@@ -631,7 +626,7 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
                 double tt = 0.;
                 span( si + 2 * side * n_inv, ib[k], ie[k], tt );
                 if ( span( sm + 2 * side * n_trsm, tb[k], te[k], tt ) ) {
-                    out.push_back( pxd_model_entry_t{ PastixKernelTRSMCblk2d, c.stride - n, n, 0, tt } );
+                    out.push_back( pxd_model_entry_t{ PastixKernelTRSMCblk2d, c.stride - n, n, 0, tt, -1, 0 } );
                 }
             }
         }
@@ -640,7 +635,9 @@ model_log_entries( pxd_ctx_t *ctx, const std::vector<uint64_t> &hs, int64_t n_li
             double           t  = 0.;
             const bool ok = ( gr.big == 2 ) ? span( ss, gr.t0, gr.t1, t ) : span( gr.big ? sb : su, gr.t0, gr.t1, t );
             if ( ok ) {
-                out.push_back( pxd_model_entry_t{ PastixKernelGEMMCblk2d2d, gr.M, gr.N, gr.K, t } );
+                // variant: 0 64 x 64 tiles, 1 tall tiles, 2 one wavefront
+                const int32_t var = ( gr.big == 2 ) ? 2 : gr.big;
+                out.push_back( pxd_model_entry_t{ PastixKernelGEMMCblk2d2d, gr.M, gr.N, gr.K, t, var, 0 } );
             }
         }
     }
@@ -674,12 +671,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
     // strict upper triangle, transposed, into the U tiles)
     PXD_CHECK( hipMemsetAsync( L, 0, ctx->coefnbr * sizeof( T ), s ) );
     typedef typename px_real<T>::type R;
-    // TRSM by blocks on the panel's block inverses: the full-tile inversion
-    // is needed by the solve only and leaves the chain panel -> TRSM
-    // (levels with at most trsm_blk slabs of 64 rows: the top of the tree,
-    // where the chain is the critical path)
-    const bool blk_ok = ctx->trsm_blk > 0 && ctx->kernels && !lu && !ldlt && std::is_same<T, R>::value;
-    auto       blk    = [&]( const level_t &lv ) { return blk_ok && lv.t1 - lv.t0 <= ctx->trsm_blk; };
     typedef typename std::conditional<std::is_same<T, R>::value, double, pxcd>::type TI;   // matrix values
     const double ovf = (double)std::numeric_limits<R>::max();
     hipLaunchKernelGGL( ( k_fill<T, TI> ), dim3( nblocks( ctx->n, 256 ) ), dim3( 256 ), 0, s,
@@ -786,17 +777,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                                     ctx->dcb, L, DI, thresh, ctx->dscratch, stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
             }
             else if constexpr ( std::is_same<T, R>::value ) {
-                bool reg = false;
-                if constexpr ( std::is_same<T, float>::value ) {
-                    // single precision: the whole diagonal tile in one workgroup's registers
-                    reg = !mlog && ctx->panel_reg > 0 && lv.maxn >= ctx->panel_reg && lv.maxn <= PXD_REG_NMAX;
-                    if ( reg ) {
-                        hipLaunchKernelGGL( k_potrf_reg, dim3( np ), dim3( 64 * PXD_REG_WAVES ), 0, st,
-                                            ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
-                                            (uint64_t *)nullptr, ctx->prio );
-                    }
-                }
-                if ( !reg && ctx->panel_mw ) {
+                if ( ctx->panel_mw ) {
                     // one workgroup per 32-row block of each diagonal tile
                     const unsigned nq  = (unsigned)( lv.q1 - lv.q0 );
                     const size_t   shq = (size_t)( ( std::max( 1, lv.maxn ) + PXD_NB - 1 ) / PXD_NB ) * PXD_NB *
@@ -805,14 +786,14 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
                                         ctx->dpan + lv.q0, ctx->dcb, L, DI, thresh, ctx->dscratch, ctx->dflags,
                                         ctx->dscratch + 2, stp( st_panel, lv.q0, 0, 0 ), ctx->prio );
                 }
-                else if ( !reg ) {
+                else {
                     hipLaunchKernelGGL( ( k_potrf2<T, false> ), dim3( np ), dim3( PXD_THREADS ), sh, st,
                                         ctx->dlist + lv.c0, ctx->dcb, L, DI, thresh, ctx->dscratch,
                                         stp( st_panel, lv.c0, 0, 0 ), ctx->prio, pcap );
                 }
             }
             launches++;
-            if ( ni && !blk( lv ) ) {
+            if ( ni ) {
                 hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st,
                                     ctx->dinvt + lv.i0, ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), ctx->prio, rcap );
                 launches++;
@@ -821,16 +802,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         else if constexpr ( std::is_same<T, typename px_real<T>::type>::value ) {
             hipLaunchKernelGGL( ( k_potrf<T> ), dim3( np ), dim3( PXD_THREADS ), 0, st, ctx->dlist + lv.c0,
                                 ctx->dcb, L, DI, thresh, ctx->dscratch );
-            launches++;
-        }
-    };
-    // the full-tile inversion when the TRSM works by blocks (launched after it)
-    auto inv = [&]( const level_t &lv, hipStream_t st ) {
-        const unsigned ni = (unsigned)( lv.i1 - lv.i0 );
-        if ( blk( lv ) && ni ) {
-            const size_t trtri_lds = (size_t)std::max( 1, std::min( lv.maxn - PXD_NB, rcap ) ) * lds_row;
-            hipLaunchKernelGGL( ( k_trtri<T> ), dim3( ni ), dim3( PXD_THREADS ), trtri_lds, st, ctx->dinvt + lv.i0,
-                                ctx->dcb, (const T *)L, DI, stp( st_inv, lv.i0, 0, n_inv ), 0, rcap );
             launches++;
         }
     };
@@ -846,12 +817,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
                                 ctx->dcb, ctx->dbk, U, (const T *)DI, 0, stp( st_trsm, lv.t0, 1, n_trsm ), ctx->prio );
             launches++;
-        }
-        else if ( blk( lv ) ) {
-            if constexpr ( std::is_same<T, R>::value ) {
-                hipLaunchKernelGGL( ( k_trsm_blk<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
-                                    ctx->dcb, ctx->dbk, L, (const T *)DI, stp( st_trsm, lv.t0, 0, n_trsm ), ctx->prio );
-            }
         }
         else if ( ctx->kernels ) {
             hipLaunchKernelGGL( ( k_trsm_mfma<T> ), dim3( nt ), dim3( PXD_THREADS ), 0, st, ctx->dtrsm + lv.t0,
@@ -883,14 +848,8 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             launches++;
         }
         if ( nb ) {
-            if ( ctx->upd_nh == 2 ) {
-                hipLaunchKernelGGL( ( k_update_big<T, 2> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                    (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
-            }
-            else {
-                hipLaunchKernelGGL( ( k_update_big<T, 1> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                    (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
-            }
+            hipLaunchKernelGGL( ( k_update_big<T, 1> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
+                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
             launches++;
         }
         if ( nu && ctx->kernels ) {
@@ -913,7 +872,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         for ( const level_t &lv : ctx->levels ) {
             panel( lv, s );
             trsm( lv, s );
-            inv( lv, s );
             for ( int64_t g = lv.g0; g < lv.g1; g++ ) {
                 const upd_group &gr = ctx->groups[g];
                 if ( gr.big == 2 ) {
@@ -938,7 +896,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             t_panel += 1e-3 * ev_ms( ea, eb );
             PXD_CHECK( hipEventRecord( ea, s ) );
             trsm( lv, s );
-            inv( lv, s );
             PXD_CHECK( hipEventRecord( eb, s ) );
             PXD_CHECK( hipEventSynchronize( eb ) );
             t_trsm += 1e-3 * ev_ms( ea, eb );
@@ -963,7 +920,6 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             panel( lv, s2 );
             trsm( lv, s2 );
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
-            inv( lv, s2 );   // beside the updates of the level: only the solve reads it
             PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
             update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, su, ctx->prio );
             PXD_CHECK( hipEventRecord( ready, su ) );
@@ -988,9 +944,7 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         std::vector<uint64_t> hs( 2 * n_st );
         PXD_CHECK( hipMemcpy( hs.data(), stamps_all, hs.size() * sizeof( uint64_t ), hipMemcpyDeviceToHost ) );
         PXD_CHECK( hipFree( stamps_all ) );
-        int rate_khz = 0;
-        PXD_CHECK( hipDeviceGetAttribute( &rate_khz, hipDeviceAttributeWallClockRate, ctx->device ) );
-        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, n_updb, 1. / ( 1e3 * std::max( 1, rate_khz ) ),
+        model_log_entries( ctx, hs, n_list, n_inv, n_trsm, n_upd, n_updb, 1. / ( 1e3 * std::max( 1, ctx->clock_khz ) ),
                            use_mw );
     }
     (void)hipEventDestroy( e0 );
@@ -1022,10 +976,8 @@ solve_pass_rhs( const pxd_ctx_t * )
 // LDL^T factor, the backward sweep.
 enum { SOLVE_FWD = 1, SOLVE_DIAG = 2, SOLVE_BWD = 4, SOLVE_ALL = 7 };
 
-// The sweeps of `what` for nrhs columns of X (vector precision V, leading
-// dimension ldx) with the factor of type T: V is T, or its double precision
-// counterpart for a single factor (option solve_vec_double; the matrix-core
-// kernels only).  swap exchanges the two sides of an LU factor: the forward
+// The sweeps of `what` for nrhs columns of X (leading dimension ldx) with
+// the factor of type T, vectors of type V (= T).  swap exchanges the two sides of an LU factor: the forward
 // sweep runs on U^T and the backward sweep on L^T (the solve with A^T,
 // reference sopalin/pastix_task_solve.c:597-601).
 // One level of the level-by-level sweeps with the sv kernels, NR columns
@@ -1278,12 +1230,6 @@ set_lds_attributes_mfma( size_t lds )
     set_lds_room( (const void *)k_potrf2<T, true>, lds );
     set_lds_room( (const void *)k_trtri<T>, lds );
     set_lds_attributes_solve<T, T>( lds );
-    if constexpr ( std::is_same<T, float>::value ) {
-        set_lds_attributes_solve<float, double>( lds );
-    }
-    if constexpr ( std::is_same<T, pxcf>::value ) {
-        set_lds_attributes_solve<pxcf, pxcd>( lds );
-    }
 }
 
 // and the scalar kernels (real arithmetics)
@@ -1299,12 +1245,9 @@ set_lds_attributes( size_t lds )
 }
 
 // Solve of nrhs vectors of type TV (double or complex double) with a factor
-// of type T: directly when the types agree or, for a single factor on the
-// matrix-core kernels with option solve_vec_double, with the factor entries
-// widened as they are read (the preconditioner of the refinement is then
-// the factor applied without rounding of the vectors); otherwise through
-// vectors converted to the factor precision and back, as the reference's
-// mixed solve does.
+// of type T: directly when the types agree, otherwise through vectors
+// converted to the factor precision and back, as the reference's mixed
+// solve does.
 template <typename T> struct type_tag;
 template <> struct type_tag<double> { static constexpr int value = 0; };
 template <> struct type_tag<float> { static constexpr int value = 1; };
@@ -1431,14 +1374,6 @@ solve_vectors( pxd_ctx_t *ctx, int64_t nrhs, void *dx, int64_t ldx, int what = S
         }
         return rc;
     }
-    else if ( ctx->kernels && ctx->solve_vec_double ) {
-        const int pass = solve_pass_rhs<TV>( ctx );
-        for ( int64_t q0 = 0; q0 < nrhs && !rc; q0 += pass ) {
-            int nq = (int)std::min<int64_t>( pass, nrhs - q0 );
-            rc = solve_sweeps<T, TV>( ctx, (TV *)dx + q0 * ldx, ldx, nq, what, swap, (TV *)scr );
-        }
-        return rc;
-    }
     else {
         const int pass = solve_pass_rhs<T>( ctx );
         size_t need = (size_t)ctx->n * std::min<int64_t>( nrhs, pass ) * sizeof( T );
@@ -1562,20 +1497,19 @@ scratch_address( int device )
 }
 
 // Bytes of scratch a solve of nrhs columns takes: Y of one pass (at most
-// PXD_MAXRHS columns) in the sweeps' vector type, plus for a single factor
-// on double vectors the factor-precision copies (256-byte aligned).
+// PXD_MAXRHS columns) in the factor precision, plus for a single factor the
+// factor-precision copies of the double vectors (256-byte aligned).
 size_t
 noalloc_need( const pxd_ctx_t *ctx, int64_t nrhs )
 {
     const size_t cols = (size_t)std::min<int64_t>( nrhs, PXD_MAXRHS );
     const size_t n    = (size_t)ctx->n;
     auto         al   = []( size_t b ) { return ( b + 255 ) & ~(size_t)255; };
-    const bool   wide = ctx->kernels && ctx->solve_vec_double;
     switch ( ctx->fflt ) {
     case FT_DOUBLE: return n * cols * sizeof( double );
     case FT_CPLX64: return n * cols * sizeof( pxcd );
-    case FT_FLOAT: return wide ? n * cols * sizeof( double ) : al( n * cols * sizeof( float ) ) + n * cols * sizeof( float );
-    default: return wide ? n * cols * sizeof( pxcd ) : al( n * cols * sizeof( pxcf ) ) + n * cols * sizeof( pxcf );
+    case FT_FLOAT: return al( n * cols * sizeof( float ) ) + n * cols * sizeof( float );
+    default: return al( n * cols * sizeof( pxcf ) ) + n * cols * sizeof( pxcf );
     }
 }
 
@@ -1592,6 +1526,17 @@ pxd_device_count( void )
         return 0;
     }
     return n;
+}
+
+// The bound of the panel's waits on the context's device, in ms of the
+// constant clock (whose rate the device reports).
+static int
+set_wait_ms( pxd_ctx_t *ctx, double ms )
+{
+    const uint64_t ticks = (uint64_t)( std::max( 0., ms ) * std::max( 1, ctx->clock_khz ) );
+    PXD_CHECK( hipSetDevice( ctx->device ) );
+    PXD_CHECK( hipMemcpyToSymbol( HIP_SYMBOL( pxd_wait_ticks ), &ticks, sizeof( ticks ) ) );
+    return 0;
 }
 
 // Engine options chosen by the autotuner (pastix_amd/autotune.py): a text
@@ -1668,6 +1613,10 @@ pxd_create( pxd_ctx_t **pctx, int device )
     set_lds_attributes_mfma<pxcd>( ctx->lds_max );
     set_lds_attributes_mfma<pxcf>( ctx->lds_max );
     (void)hipGetLastError();   // an attribute refused above must not be reported later
+    PXD_CHECK( hipDeviceGetAttribute( &ctx->clock_khz, hipDeviceAttributeWallClockRate, device ) );
+    if ( set_wait_ms( ctx, 2000. ) != 0 ) {
+        return 6;
+    }
     load_tuning( ctx );
     if ( getenv( "PXD_VERBOSE" ) ) {
         fprintf( stderr, "pastix_amd device: %s, %d CUs, LDS/block %zu, L2 %d, tuning %s\n", prop.gcnArchName,
@@ -1736,33 +1685,21 @@ int
 pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
 {
     std::string k( key );
-    if ( k == "upd_nh" ) {
-        ctx->upd_nh = ( value >= 2. ) ? 2 : 1;
-        return 0;
-    }
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;
         return 0;
     }
+    if ( k == "wait_ms" ) {
+        // for every context on this device: the bound lives in device memory
+        return set_wait_ms( ctx, value );
+    }
     if ( k == "solve_graph" ) {
         ctx->solve_graph = ( value != 0. );
         return 0;
     }
-    if ( k == "solve_vec_double" ) {
-        ctx->solve_vec_double = ( value != 0. );
-        return 0;
-    }
-    if ( k == "trsm_blk" ) {
-        ctx->trsm_blk = (int)value;
-        return 0;
-    }
     if ( k == "solve_wide_max" ) {
         ctx->solve_wide_max = (int64_t)std::max( 0., std::min( value, 1e15 ) );
-        return 0;
-    }
-    if ( k == "panel_reg" ) {
-        ctx->panel_reg = (int)std::max( 0., std::min( value, 1e9 ) );
         return 0;
     }
     if ( k == "panel_mw" ) {
@@ -1841,6 +1778,17 @@ pxd_d2h( pxd_ctx_t *ctx, void *dst, const void *src, size_t bytes )
 {
     PXD_CHECK( hipMemcpyAsync( dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream ) );
     PXD_CHECK( hipStreamSynchronize( ctx->stream ) );
+    return 0;
+}
+
+int
+pxd_set_update_classes( pxd_ctx_t *ctx, const int8_t *cls, int64_t count )
+{
+    if ( count != 0 && ( cls == nullptr || count != 2 * ctx->bloknbr ) ) {
+        return 7;
+    }
+    ctx->upd_cls.assign( cls, cls + count );
+    ctx->tasks_dirty = 1;
     return 0;
 }
 

@@ -669,7 +669,10 @@ struct pan_task {
     int32_t flag0;
     int32_t nblk;    // blocks of the tile: the last one has no consumer
 };
-#define PXD_PANEL_WAIT_TICKS 200000000ull   // 2 s of the 100 MHz constant clock
+// Bound of one wait in ticks of the constant clock (wall_clock64), set on
+// every device from its clock rate when an engine context is created (2 s
+// by default, engine option "wait_ms").
+__constant__ uint64_t pxd_wait_ticks = 200000000ull;
 
 __device__ inline void
 panel_wait( const int *f, int target, int *err, int tid )
@@ -679,7 +682,7 @@ panel_wait( const int *f, int target, int *err, int tid )
         const uint64_t t0 = (uint64_t)wall_clock64();
         while ( __hip_atomic_load( f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) < target &&
                 __hip_atomic_load( err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == 0 ) {
-            if ( (uint64_t)wall_clock64() - t0 > PXD_PANEL_WAIT_TICKS ) {
+            if ( (uint64_t)wall_clock64() - t0 > pxd_wait_ticks ) {
                 __hip_atomic_store( err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
                 break;
             }
@@ -1256,222 +1259,6 @@ k_trtri( const inv_task *__restrict__ tasks, const dcblk *__restrict__ cb, const
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky of a single-precision diagonal tile held in registers (n <= 352):
-// one workgroup of PXD_REG_WAVES wavefronts per cblk.  The tile's 32 x 32
-// blocks T(i, j), i >= j, are spread over the waves: block t = i(i+1)/2 + j
-// lives in wave t % PXD_REG_WAVES, register slot t / PXD_REG_WAVES, as four
-// 16 x 16 accumulators.  Right-looking by 32-column block k:
-//   1. the owner of T(k, k) puts it in LDS, where the workgroup factors it
-//      with the elimination of k_potrf2 (static pivot, the block's inverse
-//      built alongside) and writes L(k, k) and its inverse;
-//   2. the owners of T(i, k), i > k, form L(i, k) = T(i, k) L(k, k)^{-T} on
-//      the matrix cores and publish it to the LDS panel and to HBM;
-//   3. the owners of T(i, j), k < j <= i, subtract L(i, k) L(j, k)^T.
-// The elimination takes a barrier per column, the products two per block
-// column; HBM sees one read and one write of the tile.
-#define PXD_REG_WAVES 8
-#define PXD_REG_SLOTS 9     // ceil(66 / 8) blocks of the 11 x 11 lower triangle per wave
-#define PXD_REG_NMAX ( 11 * PXD_NB )
-
-__global__ void __launch_bounds__( 64 * PXD_REG_WAVES )
-k_potrf_reg( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, float *__restrict__ L,
-             float *__restrict__ Linv, double thresh, int *__restrict__ npiv, uint64_t *__restrict__ stamps,
-             int prio )
-{
-    typedef mfma16<float>     M;
-    typedef typename M::acc_t acc_t;
-    __shared__ float sP[PXD_REG_NMAX * ( PXD_NB + 1 )];   // L(:, k) of the current block column, [row][col]
-    __shared__ float sA[PXD_NB * ( PXD_NB + 1 )];         // diagonal block, [col][row]
-    __shared__ float sI[PXD_NB * ( PXD_NB + 1 )];         // its inverse, [col][row]
-    __shared__ float sD[PXD_NB];
-
-    const int32_t  kc      = list[blockIdx.x];
-    const uint64_t t_stamp = stamp_begin( stamps );
-    wave_priority( prio );
-    const dcblk   c    = cb[kc];
-    const int32_t n    = c.lcolnum - c.fcolnum + 1;
-    const int32_t nb   = ( n + PXD_NB - 1 ) / PXD_NB;
-    const int     tid  = threadIdx.x;
-    const int     lane = tid & 63;
-    const int     w    = __builtin_amdgcn_readfirstlane( tid >> 6 );   // wave-uniform
-    float        *A    = L + c.coefoff;     // n x n, ld n
-    float        *LI   = Linv + c.dinvoff;  // n x n, ld n
-    const float   tau  = (float)thresh;
-    int           piv  = 0;
-
-    // the blocks of this wave and their registers
-    int   bi[PXD_REG_SLOTS], bj[PXD_REG_SLOTS];
-    acc_t acc[PXD_REG_SLOTS][2][2];
-#pragma unroll
-    for ( int s = 0; s < PXD_REG_SLOTS; s++ ) {
-        const int t = PXD_REG_WAVES * s + w;
-        int       i = (int)( ( sqrtf( 8.f * (float)t + 1.f ) - 1.f ) * 0.5f );
-        if ( ( i + 1 ) * ( i + 2 ) / 2 <= t ) i++;
-        if ( i * ( i + 1 ) / 2 > t ) i--;
-        bi[s] = i;
-        bj[s] = t - i * ( i + 1 ) / 2;
-        const bool valid = i < nb;
-#pragma unroll
-        for ( int a = 0; a < 2; a++ )
-#pragma unroll
-            for ( int b = 0; b < 2; b++ ) {
-                acc[s][a][b] = M::zero();
-#pragma unroll
-                for ( int v = 0; v < 4; v++ ) {
-                    const int32_t r  = PXD_NB * bi[s] + 16 * a + M::row( lane, v );
-                    const int32_t cc = PXD_NB * bj[s] + 16 * b + ( lane & 15 );
-                    const bool    ok = valid && r < n && cc < n && r >= cc;
-                    M::set( acc[s][a][b], v, ok ? A[r + (int64_t)cc * n] : 0.f );
-                }
-            }
-    }
-
-    for ( int32_t k = 0; k < nb; k++ ) {
-        const int32_t kb = PXD_NB * k;
-        const int32_t bs = min( PXD_NB, n - kb );
-        // 1. diagonal block: its owner wave puts it in LDS, the workgroup
-        //    eliminates it and writes L(k, k) and its inverse
-        const int tkk = k * ( k + 1 ) / 2 + k;
-        if ( w == tkk % PXD_REG_WAVES ) {
-#pragma unroll
-            for ( int s = 0; s < PXD_REG_SLOTS; s++ ) {
-                if ( s == tkk / PXD_REG_WAVES ) {
-#pragma unroll
-                    for ( int a = 0; a < 2; a++ )
-#pragma unroll
-                        for ( int b = 0; b < 2; b++ )
-#pragma unroll
-                            for ( int v = 0; v < 4; v++ ) {
-                                const int r = 16 * a + M::row( lane, v ), cc = 16 * b + ( lane & 15 );
-                                sA[cc * ( PXD_NB + 1 ) + r] = ( r < bs && cc < bs && r >= cc ) ? M::get( acc[s][a][b], v ) : 0.f;
-                                sI[cc * ( PXD_NB + 1 ) + r] = ( r == cc && r < bs ) ? 1.f : 0.f;
-                            }
-                }
-            }
-        }
-        __syncthreads();
-        for ( int cc = 0; cc < bs; cc++ ) {
-            float d = sA[cc * ( PXD_NB + 1 ) + cc];
-            if ( fabsf( d ) < tau ) {
-                d = tau;
-                if ( tid == 0 ) piv++;
-            }
-            const int   m   = bs - cc - 1;
-            const int   nsq = m * ( m + 1 ) / 2;
-            const float rd  = 1.f / d;
-            for ( int idx = tid; idx < nsq + m * ( cc + 1 ); idx += 64 * PXD_REG_WAVES ) {
-                if ( idx < nsq ) {
-                    int i, j;
-                    tri_index( idx, i, j );
-                    const int rr = cc + 1 + i, c2 = cc + 1 + j;
-                    sA[c2 * ( PXD_NB + 1 ) + rr] -= sA[cc * ( PXD_NB + 1 ) + rr] * sA[cc * ( PXD_NB + 1 ) + c2] * rd;
-                }
-                else {
-                    const int q = idx - nsq, rr = cc + 1 + q % m, j = q / m;
-                    sI[j * ( PXD_NB + 1 ) + rr] -= sA[cc * ( PXD_NB + 1 ) + rr] * rd * sI[j * ( PXD_NB + 1 ) + cc];
-                }
-            }
-            if ( tid == 0 ) sD[cc] = d;
-            __syncthreads();
-        }
-        // L = U D^{1/2}, L^{-1} = D^{-1/2} U^{-1}
-        for ( int idx = tid; idx < bs * bs; idx += 64 * PXD_REG_WAVES ) {
-            const int rr = idx % bs, cc = idx / bs;
-            if ( rr > cc ) sA[cc * ( PXD_NB + 1 ) + rr] = sA[cc * ( PXD_NB + 1 ) + rr] / sqrtf( sD[cc] );
-            if ( rr >= cc ) sI[cc * ( PXD_NB + 1 ) + rr] = sI[cc * ( PXD_NB + 1 ) + rr] / sqrtf( sD[rr] );
-        }
-        __syncthreads();
-        if ( tid < bs ) sA[tid * ( PXD_NB + 1 ) + tid] = sqrtf( sD[tid] );
-        __syncthreads();
-        for ( int idx = tid; idx < bs * bs; idx += 64 * PXD_REG_WAVES ) {
-            const int rr = idx % bs, cc = idx / bs;
-            if ( rr >= cc ) A[( kb + rr ) + (int64_t)( kb + cc ) * n] = sA[cc * ( PXD_NB + 1 ) + rr];
-            LI[( kb + rr ) + (int64_t)( kb + cc ) * n] = sI[cc * ( PXD_NB + 1 ) + rr];
-        }
-
-        // 2. L(i, k) = T(i, k) L(k, k)^{-T} for the blocks below, through the LDS panel
-#pragma unroll
-        for ( int s = 0; s < PXD_REG_SLOTS; s++ ) {
-            if ( bj[s] == k && bi[s] > k && bi[s] < nb ) {
-                const int32_t rb = PXD_NB * bi[s];
-#pragma unroll
-                for ( int a = 0; a < 2; a++ )
-#pragma unroll
-                    for ( int b = 0; b < 2; b++ )
-#pragma unroll
-                        for ( int v = 0; v < 4; v++ ) {
-                            const int r = 16 * a + M::row( lane, v ), cc = 16 * b + ( lane & 15 );
-                            sP[( rb + r ) * ( PXD_NB + 1 ) + cc] = M::get( acc[s][a][b], v );
-                        }
-                __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-                acc_t p[2][2];
-#pragma unroll
-                for ( int a = 0; a < 2; a++ )
-#pragma unroll
-                    for ( int b = 0; b < 2; b++ ) p[a][b] = M::zero();
-#pragma unroll
-                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                    const int m = 4 * s4 + ( lane >> 4 );
-#pragma unroll
-                    for ( int a = 0; a < 2; a++ ) {
-                        const float x = sP[( rb + 16 * a + ( lane & 15 ) ) * ( PXD_NB + 1 ) + m];
-#pragma unroll
-                        for ( int b = 0; b < 2; b++ ) {
-                            // (L^{-T})[m][c] = Linv[c][m]
-                            const float y = sI[m * ( PXD_NB + 1 ) + 16 * b + ( lane & 15 )];
-                            p[a][b] = M::mma( x, y, p[a][b] );
-                        }
-                    }
-                }
-                __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-#pragma unroll
-                for ( int a = 0; a < 2; a++ )
-#pragma unroll
-                    for ( int b = 0; b < 2; b++ )
-#pragma unroll
-                        for ( int v = 0; v < 4; v++ ) {
-                            const int32_t r = 16 * a + M::row( lane, v ), cc = 16 * b + ( lane & 15 );
-                            const float   x = ( cc < bs ) ? M::get( p[a][b], v ) : 0.f;
-                            sP[( rb + r ) * ( PXD_NB + 1 ) + cc] = x;
-                            if ( rb + r < n && cc < bs ) A[( rb + r ) + (int64_t)( kb + cc ) * n] = x;
-                        }
-            }
-        }
-        __syncthreads();
-
-        // 3. T(i, j) -= L(i, k) L(j, k)^T for k < j <= i
-#pragma unroll
-        for ( int s = 0; s < PXD_REG_SLOTS; s++ ) {
-            if ( bj[s] > k && bi[s] < nb ) {
-                const int32_t ri = PXD_NB * bi[s], rj = PXD_NB * bj[s];
-#pragma unroll
-                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                    const int m = 4 * s4 + ( lane >> 4 );
-                    float     x[2], y[2];
-#pragma unroll
-                    for ( int a = 0; a < 2; a++ ) {
-                        x[a] = -sP[( ri + 16 * a + ( lane & 15 ) ) * ( PXD_NB + 1 ) + m];
-                        y[a] = sP[( rj + 16 * a + ( lane & 15 ) ) * ( PXD_NB + 1 ) + m];
-                    }
-#pragma unroll
-                    for ( int a = 0; a < 2; a++ )
-#pragma unroll
-                        for ( int b = 0; b < 2; b++ ) acc[s][a][b] = M::mma( x[a], y[b], acc[s][a][b] );
-                }
-            }
-        }
-    }
-    if ( tid == 0 && piv > 0 ) {
-        atomicAdd( npiv, piv );
-    }
-    stamp_end( stamps, blockIdx.x, t_stamp );
-}
-
-// ---------------------------------------------------------------------------
 // Off-diagonal TRSM on the matrix cores: X = B * Linv^T for a tile of 64
 // off-diagonal rows and all n columns of the cblk, in place (the workgroup
 // owns complete rows, so every read of B precedes every write of X).
@@ -1632,128 +1419,6 @@ k_trsm_mfma( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb,
     stamp_end( stamps, ti, t_stamp );
 }
 
-// Off-diagonal TRSM of a real Cholesky cblk by blocked substitution on the
-// panel's 32 x 32 diagonal-block inverses, without the inverse of the whole
-// diagonal tile (engine option trsm_blk): for each 32-column block k,
-// Y = B_k - sum_{j<k} X_j L_kj^T and X_k = Y L_kk^{-T}, which is
-// X = B L_kk^{-T} of core_ztrsmsp.c:148-183 by blocks.  Wave w owns rows
-// 16w..16w+15 of the 64-row slab; X_j is stored in place and read back by
-// the later blocks.  The L_kj tiles of JB blocks j are read together.
-template <typename T>
-__global__ void __launch_bounds__( PXD_THREADS )
-k_trsm_blk( const row_task *__restrict__ tasks, const dcblk *__restrict__ cb, const dblok *__restrict__ bk,
-            T *__restrict__ L, const T *__restrict__ Linv, uint64_t *__restrict__ stamps, int prio )
-{
-    typedef mfma16<T>         M;
-    typedef typename M::acc_t acc_t;
-    constexpr int JB = ( sizeof( T ) <= 4 ) ? 4 : 2;
-    __shared__ int64_t sBase[64];
-    __shared__ int32_t sLd[64];
-    __shared__ T       sY[4][16][PXD_NB + 1];   // per wave: Y, [row][col]
-
-    const int32_t  ti   = xcd_task( blockIdx.x, gridDim.x );
-    const row_task t    = tasks[ti];
-    const uint64_t t_stamp = stamp_begin( stamps );
-    wave_priority( prio );
-    const dcblk   c    = cb[t.cblk];
-    const int32_t n    = c.lcolnum - c.fcolnum + 1;
-    const int32_t lb   = cb[t.cblk + 1].fblok;
-    const int     tid  = threadIdx.x;
-    const int     lane = tid & 63, w = tid >> 6;
-    const int     lr   = lane & 15, lg = lane >> 4;
-    const T      *A    = L + c.coefoff;        // diagonal tile, n x n, ld n
-    const T      *LI   = Linv + c.dinvoff;
-
-    __shared__ int32_t skeys[PXD_ROWKEY_CAP];
-    stage_row_keys( skeys, bk, c.fblok, lb );
-    __syncthreads();
-    if ( tid < 64 ) {
-        int32_t s = t.r0 + tid;
-        if ( s < c.stride ) {
-            int32_t b  = search_row_keys( skeys, bk, c.fblok, lb, s );
-            dblok   bb = bk[b];
-            sBase[tid] = c.coefoff + (int64_t)bb.rowoff * n + ( s - bb.rowoff );
-            sLd[tid]   = bb.lrownum - bb.frownum + 1;
-        }
-        else {
-            sBase[tid] = -1;
-            sLd[tid]   = 0;
-        }
-    }
-    __syncthreads();
-
-    // operand row of this lane, and the four accumulator rows
-    const int64_t abase = sBase[16 * w + lr];
-    const int32_t ald   = sLd[16 * w + lr];
-    int64_t       cbase[4];
-    int32_t       cld[4];
-#pragma unroll
-    for ( int v = 0; v < 4; v++ ) {
-        cbase[v] = sBase[16 * w + M::row( lane, v )];
-        cld[v]   = sLd[16 * w + M::row( lane, v )];
-    }
-
-    for ( int32_t k0 = 0; k0 < n; k0 += PXD_NB ) {
-        const int32_t bkk = min( PXD_NB, n - k0 );
-        const int32_t c0 = k0 + lr, c1 = k0 + 16 + lr;   // columns of the two 16-column halves
-        acc_t         acc[2];
-#pragma unroll
-        for ( int v = 0; v < 4; v++ ) {
-            M::set( acc[0], v, ( cbase[v] >= 0 && c0 < n ) ? L[cbase[v] + (int64_t)c0 * cld[v]] : (T)0 );
-            M::set( acc[1], v, ( cbase[v] >= 0 && c1 < n ) ? L[cbase[v] + (int64_t)c1 * cld[v]] : (T)0 );
-        }
-        // Y -= X_j L_kj^T; the blocks j < k are whole (32 columns)
-        for ( int32_t j0 = 0; j0 < k0; j0 += JB * PXD_NB ) {
-            T a[JB][PXD_NB / 4], b0[JB][PXD_NB / 4], b1[JB][PXD_NB / 4];
-#pragma unroll
-            for ( int jj = 0; jj < JB; jj++ ) {
-                const int32_t jc = j0 + jj * PXD_NB;
-                const bool    ok = jc < k0;
-#pragma unroll
-                for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                    const int32_t l = jc + 4 * s4 + lg;
-                    a[jj][s4]  = ( ok && abase >= 0 ) ? -L[abase + (int64_t)l * ald] : (T)0;
-                    b0[jj][s4] = ( ok && c0 < n ) ? A[c0 + (int64_t)l * n] : (T)0;
-                    b1[jj][s4] = ( ok && c1 < n ) ? A[c1 + (int64_t)l * n] : (T)0;
-                }
-            }
-#pragma unroll
-            for ( int jj = 0; jj < JB; jj++ ) {
-                if ( j0 + jj * PXD_NB < k0 ) {
-#pragma unroll
-                    for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-                        acc[0] = M::mma( a[jj][s4], b0[jj][s4], acc[0] );
-                        acc[1] = M::mma( a[jj][s4], b1[jj][s4], acc[1] );
-                    }
-                }
-            }
-        }
-        // X_k = Y L_kk^{-T}, Y through this wave's LDS rows
-#pragma unroll
-        for ( int v = 0; v < 4; v++ ) {
-            sY[w][M::row( lane, v )][lr]      = M::get( acc[0], v );
-            sY[w][M::row( lane, v )][16 + lr] = M::get( acc[1], v );
-        }
-        __syncthreads();
-        acc_t x0 = M::zero(), x1 = M::zero();
-#pragma unroll
-        for ( int s4 = 0; s4 < PXD_NB / 4; s4++ ) {
-            const int l  = 4 * s4 + lg;
-            const T   ya = sY[w][lr][l];
-            const T   i0 = ( c0 < n && l < bkk ) ? LI[c0 + (int64_t)( k0 + l ) * n] : (T)0;
-            const T   i1 = ( c1 < n && l < bkk ) ? LI[c1 + (int64_t)( k0 + l ) * n] : (T)0;
-            x0 = M::mma( ya, i0, x0 );
-            x1 = M::mma( ya, i1, x1 );
-        }
-#pragma unroll
-        for ( int v = 0; v < 4; v++ ) {
-            if ( cbase[v] >= 0 && c0 < n ) L[cbase[v] + (int64_t)c0 * cld[v]] = M::get( x0, v );
-            if ( cbase[v] >= 0 && c1 < n ) L[cbase[v] + (int64_t)c1 * cld[v]] = M::get( x1, v );
-        }
-        __syncthreads();   // X_k stored (read back by the next blocks) and sY free
-    }
-    stamp_end( stamps, ti, t_stamp );
-}
 
 // ---------------------------------------------------------------------------
 // Supernodal update on the matrix cores: C[f(i)] -= A_i * A_b^T for one

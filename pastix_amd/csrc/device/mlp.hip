@@ -20,7 +20,7 @@
 #include "pastix_amd_device.h"
 #include "px_device.hpp"
 
-#define PXD_MLP_F    8
+#define PXD_MLP_F    10
 #define PXD_MLP_H    32
 #define PXD_MLP_ROWS 64
 

@@ -572,7 +572,7 @@ pastix_amd_models_fit( pastix_model_t *model, const char *name, int arith, const
 /* ------------------------------------------------------------------------ */
 /* Learned model: standardised features and log times around the GPU MLP. */
 
-#define PX_LM_F 8
+#define PX_LM_F 10
 
 struct pastix_amd_learned_s {
     pxd_mlp_t *mlp;
@@ -584,10 +584,15 @@ static void
 px_lm_raw_features( int arith, const pxd_model_entry_t *e, double *f )
 {
     const double m = e->m, n = e->n, k = e->k;
+    const int    upd = ( e->ktype >= PastixKernelGEMMCblk1d1d );
     double       flops;
     f[0] = ( e->ktype <= PastixKernelSYTRF ) ? 1. : 0.;
     f[1] = ( e->ktype > PastixKernelSCALOBlok && e->ktype < PastixKernelGEMMCblk1d1d ) ? 1. : 0.;
-    f[2] = ( e->ktype >= PastixKernelGEMMCblk1d1d ) ? 1. : 0.;
+    /* updates: the kernel class that runs them (an unknown class counts as
+     * the 64 x 64 tiles) */
+    f[2] = ( upd && e->variant != 1 && e->variant != 2 ) ? 1. : 0.;
+    f[3] = ( upd && e->variant == 1 ) ? 1. : 0.;
+    f[4] = ( upd && e->variant == 2 ) ? 1. : 0.;
     if ( f[0] != 0. ) {
         flops = m * m * m / 3.;
     }
@@ -597,11 +602,11 @@ px_lm_raw_features( int arith, const pxd_model_entry_t *e, double *f )
     else {
         flops = 2. * m * n * k;
     }
-    f[3] = log1p( m );
-    f[4] = log1p( n );
-    f[5] = log1p( k );
-    f[6] = log1p( flops );
-    f[7] = arith;
+    f[5] = log1p( m );
+    f[6] = log1p( n );
+    f[7] = log1p( k );
+    f[8] = log1p( flops );
+    f[9] = arith;
 }
 
 static float *
@@ -711,6 +716,121 @@ pastix_amd_learned_predict( pastix_amd_learned_t *lm, int arith, const pxd_model
     }
     free( X );
     free( y );
+    return rc;
+}
+
+/*
+ * Kernel class of every blok update chosen by predicted cost: the learned
+ * model when lm is given, else the polynomial models poly[0..2] of the three
+ * classes (64 x 64 tiles, tall tiles, one wavefront); neither: the engine's
+ * cut-offs again.  The models predict the time of an update alone on the
+ * device; the updates of a level run side by side, so the cost compared is
+ * the share of the device an update holds over that time: the predicted
+ * time times the fraction of the CUs its workgroups fill (64 x 64 tiles:
+ * one workgroup per tile; tall tiles: one per PX_TALL_ROWS(arith) x 64
+ * tile; one wavefront: a quarter of a workgroup).  The updates and their sizes are those the engine
+ * schedules (kernels/core_zgemmsp.c:401-472 sizes: M the stacked rows from
+ * the facing blok on, less the blok itself on an LU factor's U side, N the
+ * blok's rows, K the cblk's width); the one-wavefront class is offered only
+ * where it fits (M <= 64, N <= 16, K <= 32).  *predicted, if given, gets the
+ * predicted total update time of the choice.  Needs the structure on the
+ * device (after pastix_subtask_bcsc2ctab); takes effect at the next
+ * factorization.
+ */
+#define PX_DEVICE_CUS 256.   /* MI355X */
+#define PX_TALL_ROWS( arith ) ( ( arith ) == 0 ? 256 : ( arith ) == 3 ? 64 : 128 )
+
+int
+pastix_amd_choose_update_kernels( pastix_data_t *pd, pastix_amd_learned_t *lm, pastix_model_t *const *poly,
+                                  int arith, double *predicted )
+{
+    const px_solver_t *s;
+    pxd_model_entry_t *rec;
+    double            *t;
+    int8_t            *cls;
+    pastix_int_t       nrec = 0, r, nb;
+    int                lu, rc = PASTIX_SUCCESS;
+
+    if ( pd == NULL || pd->solvmatr == NULL || pd->dev == NULL || arith < 0 || arith > 3 ) {
+        return PASTIX_ERR_BADPARAMETER;
+    }
+    if ( lm == NULL && poly == NULL ) {
+        if ( predicted ) {
+            *predicted = 0.;
+        }
+        return pxd_set_update_classes( pd->dev, NULL, 0 );
+    }
+    s   = pd->solvmatr;
+    nb  = s->bloknbr;
+    lu  = ( pd->iparm[IPARM_FACTORIZATION] == PastixFactGETRF || pd->iparm[IPARM_FACTORIZATION] == PastixFactLU );
+    rec = px_malloc( ( 6 * nb + 1 ) * sizeof( pxd_model_entry_t ) );
+    /* three candidate records per (blok, side); m = 0 marks a missing one */
+    for ( pastix_int_t k = 0; k < s->cblknbr; k++ ) {
+        const px_cblk_t   *c = s->cblktab + k;
+        const pastix_int_t K = c->lcolnum - c->fcolnum + 1;
+        for ( pastix_int_t b = c->fblok; b < c[1].fblok; b++ ) {
+            const px_blok_t   *bb = s->bloktab + b;
+            const pastix_int_t N  = bb->lrownum - bb->frownum + 1;
+            for ( int side = 0; side < 2; side++ ) {
+                pxd_model_entry_t *e = rec + 3 * ( 2 * b + side );
+                const pastix_int_t M = c->stride - bb->coefind / K - ( side ? N : 0 );
+                const int          ok = ( b > c->fblok ) && ( side == 0 || lu ) && M > 0;
+                for ( int v = 0; v < 3; v++ ) {
+                    const int fits = ok && ( v < 2 || ( M <= 64 && N <= 16 && K <= 32 ) );
+                    e[v] = (pxd_model_entry_t){ PastixKernelGEMMCblk2d2d, fits ? (int32_t)M : 0, (int32_t)N,
+                                                (int32_t)K, 0., v, 0 };
+                }
+            }
+        }
+    }
+    nrec = 6 * nb;
+    t    = px_calloc( nrec + 1, sizeof( double ) );
+    if ( lm != NULL ) {
+        rc = pastix_amd_learned_predict( lm, arith, rec, nrec, t );
+    }
+    else {
+        for ( r = 0; r < nrec; r++ ) {
+            t[r] = pastix_amd_model_cost( poly[rec[r].variant], arith, PastixKernelGEMMCblk2d2d, rec[r].m, rec[r].n,
+                                          rec[r].k );
+        }
+    }
+    /* device share: time x the fraction of the CUs the workgroups fill */
+    for ( r = 0; r < nrec; r++ ) {
+        const double M = rec[r].m, N = rec[r].n;
+        double       wg;
+        if ( rec[r].variant == 0 ) {
+            wg = ceil( M / 64. ) * ceil( N / 64. );
+        }
+        else if ( rec[r].variant == 1 ) {
+            wg = ceil( M / PX_TALL_ROWS( arith ) ) * ceil( N / 64. );
+        }
+        else {
+            wg = 0.25;
+        }
+        t[r] *= fmin( 1., wg / PX_DEVICE_CUS );
+    }
+    cls = px_malloc( 2 * nb + 1 );
+    if ( predicted ) {
+        *predicted = 0.;
+    }
+    for ( r = 0; r < 2 * nb && rc == PASTIX_SUCCESS; r++ ) {
+        int best = -1;
+        for ( int v = 0; v < 3; v++ ) {
+            if ( rec[3 * r + v].m > 0 && ( best < 0 || t[3 * r + v] < t[3 * r + best] ) ) {
+                best = v;
+            }
+        }
+        cls[r] = (int8_t)best;
+        if ( predicted && best >= 0 ) {
+            *predicted += t[3 * r + best];
+        }
+    }
+    if ( rc == PASTIX_SUCCESS ) {
+        rc = pxd_set_update_classes( pd->dev, cls, 2 * nb );
+    }
+    free( cls );
+    free( t );
+    free( rec );
     return rc;
 }
 

@@ -40,8 +40,11 @@ KTYPE = {name: i for i, name in enumerate(KTYPES)}
 NKTYPES = len(KTYPES)
 UNSET = 3735928559.0
 
+# pxd_model_entry_t; variant: kernel class of a blok update (0: 64 x 64
+# tiles, 1: tall tiles, 2: one wavefront), -1 for other records
 ENTRY = np.dtype([("ktype", np.int32), ("m", np.int32), ("n", np.int32), ("k", np.int32),
-                  ("time", np.float64)])
+                  ("time", np.float64), ("variant", np.int32), ("pad", np.int32)])
+MLP_INPUTS = 10   # PXD_MLP_F
 
 
 class pastix_model_t(ctypes.Structure):
@@ -80,6 +83,7 @@ _sig("pxd_mlp_set_data", _I, _p, _p, _p, ctypes.c_int64)
 _sig("pxd_mlp_loss_grad", _I, _p, ctypes.POINTER(_D), _p)
 _sig("pxd_mlp_train", _I, _p, _I, _D, ctypes.POINTER(_D))
 _sig("pxd_mlp_predict", _I, _p, _p, ctypes.c_int64, _p)
+_sig("pastix_amd_choose_update_kernels", _I, _p, _p, _p, _I, ctypes.POINTER(_D))
 
 
 def cost1(coefs, n):
@@ -198,7 +202,7 @@ def read_model_csv(filename):
                 continue
             p = line.strip().split(";")
             if len(p) == 5:
-                rows.append((int(p[0]), int(p[1]), int(p[2]), int(p[3]), float(p[4])))
+                rows.append((int(p[0]), int(p[1]), int(p[2]), int(p[3]), float(p[4]), -1, 0))
     return np.array(rows, dtype=ENTRY)
 
 
@@ -228,7 +232,7 @@ class Mlp:
     def set_data(self, X, t):
         X = np.ascontiguousarray(X, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
-        assert X.ndim == 2 and X.shape[1] == 8 and t.shape == (X.shape[0],)
+        assert X.ndim == 2 and X.shape[1] == MLP_INPUTS and t.shape == (X.shape[0],)
         self._X, self._t = X, t
         _check("pxd_mlp_set_data", lib.pxd_mlp_set_data(self.h, _ptr(X), _ptr(t), X.shape[0]))
 
@@ -275,6 +279,38 @@ class LearnedModel:
         return t
 
 
+VARIANTS = ("tiles64", "tall", "wavefront")   # kernel classes of a blok update
+
+
+def choose_update_kernels(pastix_data, arith, learned=None, poly=None):
+    """Kernel class of every blok update by the smallest predicted time
+    (pastix_amd_choose_update_kernels): with a LearnedModel, else with three
+    Models (one per class, fitted to that class's update records), else the
+    engine's cut-offs again.  Returns the predicted total update time of the
+    choice (0 for the cut-offs).  Applies from the next factorization."""
+    pred = _D()
+    arr = None
+    if learned is None and poly is not None:
+        arr = (_PM * 3)(*[m.ptr for m in poly])
+    _check("pastix_amd_choose_update_kernels", lib.pastix_amd_choose_update_kernels(
+        pastix_data, learned.h if learned is not None else None,
+        ctypes.cast(arr, _p) if arr is not None else None, arith, ctypes.byref(pred)))
+    return pred.value
+
+
+def class_models(records, arith):
+    """Three polynomial Models of the update time, one per kernel class, each
+    fitted to the update records that ran on that class."""
+    upd = records[records["ktype"] >= KTYPE["GEMMCblk1d1d"]]
+    out = []
+    for v in range(3):
+        sel = upd[upd["variant"] == v]
+        if len(sel) < 8:
+            raise ValueError("too few update records of class %s to fit its model" % VARIANTS[v])
+        out.append(Model().fit(sel, arith, name=VARIANTS[v]))
+    return out
+
+
 def error_summary(pred, meas):
     """Relative errors |pred - meas| / meas: median, 90th percentile, and the
     ratio of the predicted to the measured total."""
@@ -316,8 +352,9 @@ TRAIN_RUNS = [("20^3 Laplacian POTRF", "lap7", 20, 1, "POTRF", (1,)),
 TEST_RUN = ("60^3 x 3 dof vector Laplacian POTRF", "vlap7", 60, 3, "POTRF", (0, 1))
 
 
-def _logged_factorization(px, kind, nx, dof, facto, mixed):
-    """Timing-log records of one factorization (model_log = 1)."""
+def _logged_factorization(px, kind, nx, dof, facto, mixed, opts=None):
+    """Timing-log records of one factorization (model_log = 1), with the
+    engine options opts."""
     if kind == "lap7":
         spm = px.Spm.laplacian7(nx, nx, nx, 6.0, 1.0)
     elif kind == "convdiff27":
@@ -331,11 +368,83 @@ def _logged_factorization(px, kind, nx, dof, facto, mixed):
     iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
     pd = px.init(iparm, dparm)
     px.analyze_grid(pd, spm, nx, nx, nx, dof=dof)
+    for k, v in (opts or {}).items():
+        px.set_option(pd, k, float(v))
     px.set_option(pd, "model_log", 1)
     px.task_numfact(pd, spm)
     rec = timing_log(pd)
     px.finalize(pd)
     return rec[rec["time"] > 0]
+
+
+# Engine options under which every blok update runs on one kernel class
+# (where it fits): the built-in cut-offs, all tall tiles, all 64 x 64 tiles.
+REGIMES = {"cut-offs": {}, "tall": {"upd_small": 0, "upd_big_rows": 1},
+           "tiles64": {"upd_small": 0, "upd_big_rows": 2 ** 30}}
+
+
+def class_logs(px, runs, mixed, log=print):
+    """Timing-log records of the problems runs ((label, generator, grid edge,
+    dof, factorization) tuples) under each regime of REGIMES, so that the
+    update shapes are timed on every kernel class."""
+    out = []
+    for label, kind, nx, dof, facto in runs:
+        for name, opts in REGIMES.items():
+            rec = _logged_factorization(px, kind, nx, dof, facto, mixed, opts)
+            log("logged %s (%s): %d records" % (label, name, len(rec)))
+            out.append(rec)
+    return np.concatenate(out)
+
+
+def choice_experiment(px, train_runs, test_run, mixed=1, steps=3000, seed=7, reps=5, log=print):
+    """Kernel class of every update chosen three ways for the test problem:
+    the engine's cut-offs, the per-class polynomial models (the models.c
+    fallback) and the learned model, both fitted to the training problems'
+    logs.  Each choice is timed over reps factorizations of the test problem
+    (the median of the device's own factorization time) and its factor is
+    kept for the caller to check.  Also scores both models on a held-out
+    fifth of the training updates."""
+    arith = 0 if mixed else 1
+    rec = class_logs(px, train_runs, mixed, log=log)
+    upd = rec[rec["ktype"] >= KTYPE["GEMMCblk1d1d"]]
+    rng = np.random.default_rng(seed)
+    idx = rng.permutation(len(upd))
+    ntr = int(0.8 * len(upd))
+    tr, te = upd[idx[:ntr]], upd[idx[ntr:]]
+    poly = class_models(tr, arith)
+    learned = LearnedModel(tr, arith, steps=steps, seed=seed)
+    pp = np.array([poly[int(r["variant"])].cost(arith, int(r["ktype"]), int(r["m"]), int(r["n"]), int(r["k"]))
+                   for r in te])
+    held_out = dict(records=int(len(te)), polynomial=error_summary(pp, te["time"]),
+                    learned=error_summary(learned.predict(te), te["time"]))
+    poly = class_models(upd, arith)
+    learned = LearnedModel(upd, arith, steps=steps, seed=seed)
+    label, kind, nx, dof, facto = test_run
+    if kind == "lap7":
+        spm = px.Spm.laplacian7(nx, nx, nx, 6.0, 1.0)
+    elif kind == "convdiff27":
+        spm = px.Spm.convdiff27(nx, nx, nx, 1.0, 0.5, 0)
+    else:
+        spm = px.Spm.vlaplacian7(nx, nx, nx, dof, 1.0, 0)
+    iparm, dparm = px.initParam()
+    iparm[px.IPARM_ORDERING] = px.PastixOrderPersonal
+    iparm[px.IPARM_FACTORIZATION] = px.PastixFactGETRF if facto == "GETRF" else px.PastixFactPOTRF
+    iparm[px.IPARM_MIXED] = mixed
+    iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
+    pd = px.init(iparm, dparm)
+    px.analyze_grid(pd, spm, nx, nx, nx, dof=dof)
+    px.task_numfact(pd, spm)
+    out = dict(train=[r[0] for r in train_runs], train_update_records=int(len(upd)), test_problem=label,
+               held_out=held_out, choices={})
+    for name, kw in (("cut-offs", {}), ("polynomial", dict(poly=poly)), ("learned", dict(learned=learned))):
+        pred = choose_update_kernels(pd, arith, **kw)
+        times = []
+        for _ in range(reps):
+            px.task_numfact(pd, spm)
+            times.append(px.get_stats(pd)["fact_seconds"])
+        out["choices"][name] = dict(fact_seconds=float(np.median(times)), predicted_update_seconds=pred)
+        log("%s: factorization %.2f ms" % (name, 1e3 * float(np.median(times))))
+    return out, pd, spm, dparm
 
 
 def _families(rec):

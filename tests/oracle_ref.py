@@ -215,10 +215,10 @@ def dense_from_factor(solver, L):
 
 # ---------------------------------------------------------------------------
 # Learned cost model: float64 restatement of the perceptron whose forward and
-# backward passes run on the GPU (8 inputs, two tanh layers of 32 units, one
-# linear output; parameters W1[8][32], b1, W2[32][32], b2, w3, b3 in one array).
+# backward passes run on the GPU (10 inputs, two tanh layers of 32 units, one
+# linear output; parameters W1[10][32], b1, W2[32][32], b2, w3, b3 in one array).
 
-MLP_F, MLP_H = 8, 32
+MLP_F, MLP_H = 10, 32
 
 
 def mlp_unpack(p):

@@ -46,30 +46,6 @@ def test_potrf_fp32_tiles_match_oracle(nx, kernels):
     px.finalize(pd)
 
 
-@pytest.mark.parametrize("panel_reg", [1, 96, 0])
-@pytest.mark.parametrize("nx,kind", [(9, "lap7"), (17, "lap7"), (43, "lap7"), (13, "vlap7")])
-def test_register_panel_matches_oracle(nx, kind, panel_reg):
-    """Single-precision Cholesky with the diagonal tiles factored in one
-    workgroup's registers (k_potrf_reg) on every level (panel_reg = 1), on
-    the levels with cblks of 96 columns or more, or never (the default);
-    43^3 has a 331-column cblk (11 block columns, a partial last one), and
-    the 3-dof vector Laplacian has lognormal coefficients."""
-    spm = make_problem(kind, nx)
-    pd, iparm, dparm = analyzed(spm, nx, mixed=1, dof=3 if kind == "vlap7" else 1)
-    px.set_option(pd, "panel_reg", panel_reg)
-    px.task_numfact(pd, spm)
-    s = px.solver_export(pd)
-    cbk = s["cblk"].reshape(-1, 6)[:s["cblknbr"]]
-    L = px.get_coeftab(pd, s["coefnbr"], np.float32)
-    F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float32), np.float32,
-                         nthreads=8)
-    assert iparm[px.IPARM_STATIC_PIVOTING] == F.npiv
-    assert cblk_diff(s, L, F.L, np.float32) <= 10.0
-    if nx == 43:
-        assert (cbk[:, 1] - cbk[:, 0] + 1).max() > 320
-    px.finalize(pd)
-
-
 @pytest.mark.parametrize("kernels", [1, 0])
 @pytest.mark.parametrize("nx,nrhs", [(8, 1), (20, 1), (20, 3), (12, 9), (30, 2)])
 def test_solve_fp64_backward_error(nx, nrhs, kernels):
@@ -148,52 +124,3 @@ def test_fp64_gmres_reference_criterion():
     r = b - spm.matmat(x)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
     px.finalize(pd)
-
-
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_tall_tile_stage_depth_matches_oracle(dtype):
-    """Every blok update on the tall tiles with two MFMA k-groups per LDS
-    stage (upd_nh=2): the factor matches the oracle."""
-    nx = 18
-    spm = make_problem("lap7", nx)
-    pd, iparm, dparm = analyzed(spm, nx, mixed=int(dtype == np.float32))
-    px.set_option(pd, "upd_big_rows", 1)
-    px.set_option(pd, "upd_nh", 2)
-    px.task_numfact(pd, spm)
-    s = px.solver_export(pd)
-    L = px.get_coeftab(pd, s["coefnbr"], dtype)
-    F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype)
-    assert cblk_diff(s, L, F.L, dtype) <= 10.0
-    px.finalize(pd)
-
-
-@pytest.mark.parametrize("timing", [0, 1])
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("nx", [5, 12, 20, 28])
-def test_blocked_trsm_matches_oracle(nx, dtype, timing):
-    """TRSM by blocked substitution on the panel's 32 x 32 block inverses
-    (option trsm_blk = 10**6: on every level), the full-tile inversion
-    launched after it: the
-    factor matches the oracle tile by tile (28^3 has 320-column cblks of 10
-    blocks), and the solve, which reads the full-tile inverses, reaches the
-    same backward error as with the default TRSM."""
-    mixed = int(dtype == np.float32)
-    spm = make_problem("lap7", nx)
-    res = {}
-    for blk in (1, 0):
-        pd, iparm, dparm = analyzed(spm, nx, mixed=mixed)
-        px.set_option(pd, "trsm_blk", 10 ** 6 * blk)
-        px.set_option(pd, "timing", timing)
-        px.task_numfact(pd, spm)
-        s = px.solver_export(pd)
-        L = px.get_coeftab(pd, s["coefnbr"], dtype)
-        if blk:
-            F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, dtype), dtype, nthreads=8)
-            assert cblk_diff(s, L, F.L, dtype) <= 10.0
-        x0, b = spm.gen_rhs(3, seed=2)
-        x = np.array(b, order="F", copy=True)
-        px.task_solve(pd, spm, x)
-        res[blk] = spm.backward_error(b, x)[0]
-        px.finalize(pd)
-    eps = np.finfo(dtype).eps
-    assert res[1] <= 10 * max(res[0], eps)

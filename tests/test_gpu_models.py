@@ -86,7 +86,7 @@ def test_timing_log_records_and_factor(facto, dtype, tmp_path):
 
 def _mlp_case(n, seed):
     rng = np.random.default_rng(seed)
-    X = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    X = rng.normal(0, 1, (n, orc.MLP_F)).astype(np.float32)
     t = (np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 3]).astype(np.float32)
     return X, t
 

@@ -103,38 +103,6 @@ def test_solve_on_a_refactorized_instance():
     px.finalize(pd2)
 
 
-@pytest.mark.parametrize("kind,facto,nx", [("lap7", None, 12), ("convdiff27", px.PastixFactGETRF, 9),
-                                           ("helm", px.PastixFactLDLT, 9), ("helm", px.PastixFactGETRF, 8)])
-def test_single_factor_on_double_vectors(kind, facto, nx):
-    """A single-precision factor applied to double vectors (option
-    solve_vec_double = 1, the entries widened as they are read) against the
-    reference's way (vectors rounded to single and back): the same solution
-    to single-precision accuracy, a backward error no worse, and a mixed
-    GMRES / BiCGStab that needs no more iterations."""
-    spm = _problem(kind, nx)
-    extra = {px.IPARM_REFINEMENT: px.PastixRefineBiCGSTAB} if kind == "helm" else None
-    pd, iparm, dparm = analyzed(spm, nx, facto=facto, mixed=1, extra=extra)
-    px.task_numfact(pd, spm)
-    x0, b = spm.gen_rhs(3, seed=7)
-    xs, berr, iters = {}, {}, {}
-    for vd in (1, 0):
-        px.set_option(pd, "solve_vec_double", vd)
-        x = np.array(b, order="F", copy=True)
-        px.task_solve(pd, spm, x)
-        xs[vd] = x
-        berr[vd] = spm.backward_error(b, x)[0]
-        px.set_option(pd, "berr_target", 1e-12)
-        xr = np.array(x, order="F", copy=True)
-        px.task_refine(pd, spm, b, xr)
-        assert spm.backward_error(b, xr)[0] <= 1e-12
-        iters[vd] = iparm[px.IPARM_NBITER]
-    eps = np.finfo(np.float32).eps
-    assert np.abs(xs[1] - xs[0]).max() <= 1e3 * eps * np.abs(xs[0]).max()
-    assert berr[1] <= 1.5 * berr[0]
-    assert iters[1] <= iters[0]
-    px.finalize(pd)
-
-
 @pytest.mark.parametrize("kind,facto,mixed,nx", [("lap7", None, 0, 10), ("lap7", None, 1, 10),
                                                  ("helm", px.PastixFactLDLT, 0, 8),
                                                  ("convdiff27", px.PastixFactGETRF, 1, 8)])

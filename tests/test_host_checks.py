@@ -191,3 +191,24 @@ def test_matrix_of_another_arithmetic_sets_the_vector_type():
     assert px.rhs_array(Bp).dtype == np.float64
     px.rhsFinalize(Bp)
     px.finalize(pd)
+
+
+def test_steps_out_of_order_are_refused():
+    """The reference's order of the steps (sopalin/pastix_task_sopalin.c,
+    pastix_task_solve.c): a factorization before the analysis, and a solve or
+    a refinement before the factorization, return PASTIX_ERR_BADPARAMETER
+    before any work (none reaches the device)."""
+    spm = px.Spm.laplacian7(4, 4, 4)
+    iparm, dparm = px.initParam()
+    iparm[px.IPARM_VERBOSE] = px.PastixVerboseNot
+    pd = px.init(iparm, dparm)
+    assert px.lib.pastix_task_numfact(pd, ctypes.byref(spm.s)) == px.PASTIX_ERR_BADPARAMETER
+    n = spm.n
+    x = np.zeros((n, 1), order="F")
+    b = np.ones((n, 1), order="F")
+    assert px.lib.pastix_task_solve(pd, n, 1, px._ptr(x), n) == px.PASTIX_ERR_BADPARAMETER
+    assert px.lib.pastix_task_refine(pd, n, 1, px._ptr(b), n, px._ptr(x), n) == px.PASTIX_ERR_BADPARAMETER
+    # after the analysis, still no factorization
+    px.task_analyze(pd, spm)
+    assert px.lib.pastix_task_solve(pd, n, 1, px._ptr(x), n) == px.PASTIX_ERR_BADPARAMETER
+    px.finalize(pd)

@@ -137,9 +137,9 @@ def _synthetic_records(rng, n=400):
     trsm = np.array([3e-6, 1e-9, 2e-9, 3e-11, 4e-11, 5e-13])
     potrf = np.array([1e-6, 1e-8, 1e-10, 1e-12])
     for i in range(n):
-        rec[i] = (K["GEMMCblk2d2d"], ms[i], ns[i], ks[i], pm.cost3(gemm, ms[i], ns[i], ks[i]))
-        rec[n + i] = (K["TRSMCblk2d"], ms[i], ns[i], 0, pm.cost2(trsm, ms[i], ns[i]))
-        rec[2 * n + i] = (K["POTRF"], ns[i], 0, 0, pm.cost1(potrf, ns[i]))
+        rec[i] = (K["GEMMCblk2d2d"], ms[i], ns[i], ks[i], pm.cost3(gemm, ms[i], ns[i], ks[i]), 0, 0)
+        rec[n + i] = (K["TRSMCblk2d"], ms[i], ns[i], 0, pm.cost2(trsm, ms[i], ns[i]), -1, 0)
+        rec[2 * n + i] = (K["POTRF"], ns[i], 0, 0, pm.cost1(potrf, ns[i]), -1, 0)
     return rec, gemm, trsm, potrf
 
 
