@@ -65,6 +65,21 @@ struct upd_group {
 // sweeps on the instance's own vectors for nq columns of one factor type,
 // replayed as one graph.  It stays valid while everything the capture baked
 // in is unchanged (sig).
+// What a captured factorization baked in: it is replayed only while these
+// are unchanged.
+struct fact_graph_sig {
+    int         tag;
+    uint64_t    gen;
+    const void *L, *U, *DI, *DU;
+    double      thresh;
+    int         prio, panel_mw, ldlt;
+    bool operator==( const fact_graph_sig &o ) const
+    {
+        return tag == o.tag && gen == o.gen && L == o.L && U == o.U && DI == o.DI && DU == o.DU &&
+               thresh == o.thresh && prio == o.prio && panel_mw == o.panel_mw && ldlt == o.ldlt;
+    }
+};
+
 struct solve_graph {
     int            nq, tag;
     const void    *sig[6];   // vectors, Y, L, U and the two inverse sets
@@ -128,6 +143,12 @@ struct pxd_ctx_s {
     std::vector<solve_graph> sgraphs;
     uint64_t sched_gen   = 0;      // bumped when the task lists are rebuilt
     int      solve_graph = 1;
+    // the factorization's level loop as a captured graph (option fact_graph)
+    int            fact_graph = 1;
+    hipGraphExec_t fgraph     = nullptr;
+    fact_graph_sig fgraph_sig{};
+    int64_t        fgraph_launches = 0;
+    int64_t        stats_graph_captures = 0;
     std::string tuning_file;       // the tuning file applied at creation, if any
     int         clock_khz = 0;     // rate of the device's constant clock
     size_t dvecs_bytes = 0;
@@ -267,6 +288,13 @@ compute_levels( pxd_ctx_t *ctx, const pxd_cblk_t *cblks, const pxd_blok_t *bloks
 
 
 void drop_solve_graphs( pxd_ctx_t *ctx );
+
+// a number for each factor type (keys of the captured graphs)
+template <typename T> struct type_tag;
+template <> struct type_tag<double> { static constexpr int value = 0; };
+template <> struct type_tag<float> { static constexpr int value = 1; };
+template <> struct type_tag<pxcd> { static constexpr int value = 2; };
+template <> struct type_tag<pxcf> { static constexpr int value = 3; };
 
 // Build the task lists of the factorization and the solve, level by level.
 int
@@ -914,20 +942,59 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
         hipStream_t s2 = ctx->stream2;
         hipStream_t su = s;
         hipEvent_t  ready = ctx->ev_ready, paneldone = ctx->ev_panel;
-        PXD_CHECK( hipEventRecord( ready, su ) );
-        for ( const level_t &lv : ctx->levels ) {
-            PXD_CHECK( hipStreamWaitEvent( s2, ready, 0 ) );
-            panel( lv, s2 );
-            trsm( lv, s2 );
+        auto        issue = [&]() -> int {
+            PXD_CHECK( hipEventRecord( ready, su ) );
+            for ( const level_t &lv : ctx->levels ) {
+                PXD_CHECK( hipStreamWaitEvent( s2, ready, 0 ) );
+                panel( lv, s2 );
+                trsm( lv, s2 );
+                PXD_CHECK( hipEventRecord( paneldone, s2 ) );
+                PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
+                update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, su, ctx->prio );
+                PXD_CHECK( hipEventRecord( ready, su ) );
+                update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, su, 0 );
+            }
+            // the last inversions ran on s2 after the last handoff
             PXD_CHECK( hipEventRecord( paneldone, s2 ) );
             PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
-            update( lv.u0, lv.uc, lv.b0, lv.bc, lv.s0, lv.sc, su, ctx->prio );
-            PXD_CHECK( hipEventRecord( ready, su ) );
-            update( lv.uc, lv.u1, lv.bc, lv.b1, lv.sc, lv.s1, su, 0 );
+            return 0;
+        };
+        if ( ctx->fact_graph ) {
+            // the level loop (both streams) as one graph, captured once per
+            // structure and factor buffers; the clearing and the fill above
+            // stay outside it
+            const fact_graph_sig sig{ type_tag<T>::value, ctx->sched_gen, L, U, DI, DU, thresh, ctx->prio,
+                                      ctx->panel_mw, ldlt };
+            if ( ctx->fgraph == nullptr || !( ctx->fgraph_sig == sig ) ) {
+                if ( ctx->fgraph ) {
+                    (void)hipGraphExecDestroy( ctx->fgraph );
+                    ctx->fgraph = nullptr;
+                }
+                const int64_t l0 = launches;
+                hipGraph_t    g  = nullptr;
+                PXD_CHECK( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
+                const int  rc = issue();
+                hipError_t e  = hipStreamEndCapture( s, &g );
+                if ( rc != 0 || e != hipSuccess || g == nullptr ||
+                     hipGraphInstantiate( &ctx->fgraph, g, nullptr, nullptr, 0 ) != hipSuccess ) {
+                    if ( g ) (void)hipGraphDestroy( g );
+                    (void)hipGetLastError();
+                    ctx->fgraph = nullptr;
+                    return rc ? rc : 6;
+                }
+                (void)hipGraphDestroy( g );
+                ctx->fgraph_sig      = sig;
+                ctx->fgraph_launches = launches - l0;
+                ctx->stats_graph_captures++;
+            }
+            else {
+                launches += ctx->fgraph_launches;
+            }
+            PXD_CHECK( hipGraphLaunch( ctx->fgraph, s ) );
         }
-        // the last inversions ran on s2 after the last handoff
-        PXD_CHECK( hipEventRecord( paneldone, s2 ) );
-        PXD_CHECK( hipStreamWaitEvent( su, paneldone, 0 ) );
+        else if ( int rc = issue() ) {
+            return rc;
+        }
     }
     PXD_CHECK( hipGetLastError() );
     PXD_CHECK( hipEventRecord( e1, s ) );
@@ -1248,11 +1315,6 @@ set_lds_attributes( size_t lds )
 // of type T: directly when the types agree, otherwise through vectors
 // converted to the factor precision and back, as the reference's mixed
 // solve does.
-template <typename T> struct type_tag;
-template <> struct type_tag<double> { static constexpr int value = 0; };
-template <> struct type_tag<float> { static constexpr int value = 1; };
-template <> struct type_tag<pxcd> { static constexpr int value = 2; };
-template <> struct type_tag<pxcf> { static constexpr int value = 3; };
 
 void
 drop_solve_graphs( pxd_ctx_t *ctx )
@@ -1636,6 +1698,9 @@ pxd_destroy( pxd_ctx_t *ctx )
     (void)hipSetDevice( ctx->device );
     (void)hipStreamSynchronize( ctx->stream );
     drop_solve_graphs( ctx );
+    if ( ctx->fgraph ) {
+        (void)hipGraphExecDestroy( ctx->fgraph );
+    }
     void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred,
@@ -1693,6 +1758,10 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     if ( k == "wait_ms" ) {
         // for every context on this device: the bound lives in device memory
         return set_wait_ms( ctx, value );
+    }
+    if ( k == "fact_graph" ) {
+        ctx->fact_graph = ( value != 0. );
+        return 0;
     }
     if ( k == "solve_graph" ) {
         ctx->solve_graph = ( value != 0. );

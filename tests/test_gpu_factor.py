@@ -124,3 +124,38 @@ def test_fp64_gmres_reference_criterion():
     r = b - spm.matmat(x)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
     px.finalize(pd)
+
+
+@pytest.mark.parametrize("kind,facto,mixed,nx", [("lap7", None, 0, 12), ("lap7", None, 1, 14),
+                                                 ("convdiff27", px.PastixFactGETRF, 0, 8),
+                                                 ("lap7sd", px.PastixFactLDLT, 0, 9)])
+def test_factorization_graph_replays_match_oracle(kind, facto, mixed, nx):
+    """The level loop of the factorization captured as a graph (engine
+    option fact_graph = 1, the default) and replayed: every replay, with
+    solves and an eager factorization (fact_graph = 0) in between, gives a
+    factor that matches the oracle's tile by tile."""
+    spm = make_problem(kind, nx)
+    pd, iparm, dparm = analyzed(spm, nx, facto=facto, mixed=mixed)
+    dt = np.float32 if mixed else np.float64
+    for rep, graph in enumerate((1, 1, 0, 1, 1)):
+        px.set_option(pd, "fact_graph", graph)
+        px.task_numfact(pd, spm)
+        if rep == 0:
+            s = px.solver_export(pd)
+            bc = px.bcsc_export(pd)
+            tau = static_pivot_threshold(dparm, dt)
+            if facto == px.PastixFactGETRF:
+                F = orc.OracleLU(s, bc, tau, dt)
+            elif facto == px.PastixFactLDLT:
+                F = orc.OracleLDLT(s, bc, tau, dt)
+            else:
+                F = orc.OracleFactor(s, bc, tau, dt)
+        L = px.get_coeftab(pd, s["coefnbr"], dt)
+        assert cblk_diff(s, L, F.L, dt, full_diag=facto is not None) <= 10.0
+        if facto == px.PastixFactGETRF:
+            U = px.get_coeftab(pd, s["coefnbr"], dt, side=1)
+            assert cblk_diff(s, U, F.U, dt, full_diag=True) <= 10.0
+        x0, b = spm.gen_rhs(2, seed=rep)
+        x = np.array(b, order="F", copy=True)
+        px.task_solve(pd, spm, x)
+    px.finalize(pd)
