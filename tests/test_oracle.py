@@ -10,7 +10,7 @@ import scipy.sparse as sp
 
 import pastix_amd as px
 import oracle_ref as orc
-from helpers import analyzed, permute_rhs, static_pivot_threshold
+from helpers import analyzed, make_problem, permute_rhs, static_pivot_threshold
 
 
 def _dense(cp, rt, v, n):
@@ -314,4 +314,65 @@ def test_oracle_complex_lu_equals_dense():
     D = _dense_lu_nopiv(A.astype(np.complex128))
     packed = np.tril(orc.dense_from_factor(s, F.L), -1) + np.triu(orc.dense_from_factor(s, F.U).T)
     assert np.abs(packed - D).max() <= 1e-12 * np.abs(D).max()
+    px.finalize(pd)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (6, 5, 4)])
+def test_oracle_single_lu_equals_dense(dims):
+    """The single-precision LU instantiation against the dense no-pivot LU
+    of P A P^T in double: within the growth-free bound of a few hundred
+    eps_32 (a convection-diffusion matrix is diagonally dominant)."""
+    spm = px.Spm.convdiff27(*dims, 1.0, 0.5, 3)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLU)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd)
+    n = s["nodenbr"]
+    A = _dense(*bc, n)
+    F = orc.OracleLU(s, bc, static_pivot_threshold(dparm, np.float32), np.float32)
+    assert F.npiv == 0 and F.L.dtype == np.float32
+    D = _dense_lu_nopiv(A)
+    Lo = orc.dense_from_factor(s, F.L.astype(np.float64))
+    Uo = orc.dense_from_factor(s, F.U.astype(np.float64))
+    packed = np.tril(Lo, -1) + np.triu(Uo.T)
+    eps = np.finfo(np.float32).eps
+    assert np.abs(packed - D).max() <= 300 * eps * np.abs(D).max()
+    # a wrong sign or a transposed operand would be off by O(1), not eps
+    assert np.abs(packed - D).max() > 0
+    x0, b = spm.gen_rhs(2, seed=1)
+    x = F.solve(permute_rhs(pd, b))[px.order_export(pd)[0]]
+    assert np.abs(x - x0).max() <= 1e3 * eps * np.abs(x0).max()
+    px.finalize(pd)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (6, 5, 4)])
+def test_oracle_single_ldlt_equals_dense(dims):
+    """The real single-precision LDL^T instantiation against the dense LDL^T
+    in double of a symmetric indefinite, diagonally dominant stencil
+    (diagonal +7 / -7 on alternate vertices): its Schur complements stay
+    diagonally dominant, so the error stays at the level of eps_32."""
+    spm = make_problem("lap7sd", *dims)
+    pd, iparm, dparm = analyzed(spm, *dims, facto=px.PastixFactLDLT)
+    px._check("spm2bcsc", px.lib.pastix_subtask_spm2bcsc(pd, ctypes.byref(spm.s)))
+    s, bc = px.solver_export(pd), px.bcsc_export(pd)
+    n = s["nodenbr"]
+    A = _dense(*bc, n)
+    F = orc.OracleLDLT(s, bc, static_pivot_threshold(dparm, np.float32), np.float32)
+    assert F.npiv == 0 and F.L.dtype == np.float32
+    Ld, d = _dense_ldlt_nopiv(A)
+    Lo = orc.dense_from_factor(s, F.L.astype(np.float64))
+    # without pivoting an indefinite matrix amplifies rounding (small d's):
+    # the double factor measures the amplification, the single one must stay
+    # within it, scaled by the ratio of the epsilons
+    F64 = orc.OracleLDLT(s, bc, static_pivot_threshold(dparm, np.float64), np.float64)
+    L64 = orc.dense_from_factor(s, F64.L)
+    eps, eps64 = np.finfo(np.float32).eps, np.finfo(np.float64).eps
+    amp = max(np.abs(np.tril(L64, -1) - np.tril(Ld, -1)).max() / eps64, 1.0)
+    err = np.abs(np.tril(Lo, -1) - np.tril(Ld, -1)).max()
+    assert 0 < err <= 20 * amp * eps
+    ampd = max(np.abs(np.diag(L64) - d).max() / (eps64 * np.abs(d).max()), 1.0)
+    assert np.abs(np.diag(Lo) - d).max() <= 20 * ampd * eps * np.abs(d).max()
+    x0, b = spm.gen_rhs(2, seed=2)
+    x = F.solve(permute_rhs(pd, b))[px.order_export(pd)[0]]
+    berr, _ = spm.backward_error(b, x)
+    assert berr <= 20 * amp * eps
     px.finalize(pd)
