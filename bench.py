@@ -480,8 +480,8 @@ def main():
         "gflops_fact_binary": fl / head["fact"] / 1024 ** 3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / MFMA_PEAK_TFLOPS["f32"], "traffic": traffic,
-                     "kernel": "k_update_big<float> + k_update_mfma<float> (supernodal update, all launches "
-                               "of one factorization)", "algorithmic_flops": upd},
+                     "kernel": "k_update_dma<8,3> + k_update_mfma<float> + k_update_small<float> (supernodal "
+                               "update, all launches of one factorization)", "algorithmic_flops": upd},
         "cpu_baseline": None,
     }
     out["roofline_solve_fp32"] = head["bandwidth"]["solve"]
@@ -493,8 +493,8 @@ def main():
         a64, u64 = update_roofline(r64, "f64")
         out["roofline_fp64"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                                 "frac": a64 / MFMA_PEAK_TFLOPS["f64"], "traffic": None,
-                                "kernel": "k_update_big<double> + k_update_mfma<double> (all launches of one "
-                                          "factorization)", "algorithmic_flops": u64}
+                                "kernel": "k_update_big<double> + k_update_mfma<double> + k_update_small<double> "
+                                          "(all launches of one factorization)", "algorithmic_flops": u64}
         out["roofline_solve_fp64"] = r64["bandwidth"]["solve"]
         # forward error of the FP32+IR solution against the FP64 factorization's
         x32, x64 = xs["fp32_ir"], xs["fp64"]

@@ -87,7 +87,14 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "panel_mw"      1: multi-workgroup Cholesky panels (default)
  *   "upd_big_rows"  stacked rows from which updates use the tall tiles
  *   "upd_small"     1: one-wavefront small updates (default)
- *   "prio"          1: raised wave priority on the critical path (default)
+ *   "upd_dma"       1: single-precision LL^T / LU tall tiles load their
+ *                   operands straight into LDS (default); 0: register-staged
+ *   "upd_dma_pad"   extra LDS bytes per workgroup of that kernel (0..65536,
+ *                   default 4096: four workgroups per CU, room left for the
+ *                   lookahead stream's panels)
+ *   "fact_graph"    1: the factorization's level loop replays a graph
+ *                   captured once per structure (default)
+ *   "prio"         1: raised wave priority on the critical path (default)
  *   "solve_graph"   1: each solve pass replays the sweeps' launches from a
  *                   graph captured on first use (default); 0: launched one by one
  *   "solve_wide_max" level-solve launches with at least this many tiles load

@@ -12,7 +12,11 @@ rounding:
   16 columns and 32 k values;
 - ``solve_wide_max``: level-solve launches from which the tiles load 16
   columns per batch instead of all of them;
-- ``panel_mw``: the multi-workgroup Cholesky panel.
+- ``panel_mw``: the multi-workgroup Cholesky panel;
+- ``upd_dma``: single-precision tall tiles on the kernel that loads its
+  operands straight into LDS;
+- ``upd_dma_pad``: extra LDS per workgroup of that kernel, which sets how
+  many of its workgroups share a CU with the lookahead stream's panels.
 
 The search is coordinate descent from the current values (each option in
 turn over its candidates, the others fixed, two passes); the objective is
@@ -44,8 +48,11 @@ SPACE = {
     "upd_small": [0, 1],
     "solve_wide_max": [32, 64, 128, 256, 1024],
     "panel_mw": [0, 1],
+    "upd_dma": [0, 1],
+    "upd_dma_pad": [0, 2048, 4096, 6144, 8192],
 }
-DEFAULTS = {"upd_big_rows": 192, "upd_small": 1, "solve_wide_max": 128, "panel_mw": 1}
+DEFAULTS = {"upd_big_rows": 192, "upd_small": 1, "solve_wide_max": 128, "panel_mw": 1, "upd_dma": 1,
+            "upd_dma_pad": 4096}
 
 # problems: (label, generator, grid edge, factorization, mixed precision, RHS)
 PROBLEMS = {

@@ -169,6 +169,14 @@ struct pxd_ctx_s {
     int          upd_small = 1;
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
+    // 1: single-precision LL^T / LU tall tiles on the kernel that loads its
+    // operands straight into LDS (k_update_dma); 0: on k_update_big
+    int          upd_dma = 1;
+    // dynamic LDS bytes added to each k_update_dma workgroup: 4 KiB makes it
+    // 34 KiB, four workgroups per CU, and leaves 24 KiB of the CU's LDS to the
+    // panel and TRSM workgroups of the lookahead stream (without it five fit
+    // and the next level's panels wait for a CU)
+    unsigned     upd_dma_pad = 4096;
 
     // host copies of the panel, TRSM and inversion lists, and the blok
     // updates in launch order (kernel timing log)
@@ -876,8 +884,19 @@ factorize( pxd_ctx_t *ctx, double thresh, int64_t *nbpivots )
             launches++;
         }
         if ( nb ) {
-            hipLaunchKernelGGL( ( k_update_big<T, 1> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
-                                (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
+            bool done = false;
+            if constexpr ( std::is_same<T, float>::value ) {
+                if ( ctx->upd_dma && !ldlt ) {
+                    hipLaunchKernelGGL( ( k_update_dma<PXD_DMA_BK, PXD_DMA_NBUF> ), dim3( nb ), dim3( PXD_THREADS ),
+                                        ctx->upd_dma_pad, st, ctx->dupdb + b0, (int32_t)nb, ctx->dcb, ctx->dbk, L, U,
+                                        stp( st_updb, b0, 0, 0 ), prio );
+                    done = true;
+                }
+            }
+            if ( !done ) {
+                hipLaunchKernelGGL( ( k_update_big<T, 1> ), dim3( nb ), dim3( PXD_THREADS ), 0, st, ctx->dupdb + b0,
+                                    (int32_t)nb, ctx->dcb, ctx->dbk, L, U, ldlt, stp( st_updb, b0, 0, 0 ), prio );
+            }
             launches++;
         }
         if ( nu && ctx->kernels ) {
@@ -1752,6 +1771,19 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     std::string k( key );
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
+        ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "upd_dma_pad" ) {
+        if ( value < 0. || value > 65536. ) {
+            return 7;
+        }
+        ctx->upd_dma_pad = (unsigned)value;
+        ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "upd_dma" ) {
+        ctx->upd_dma     = ( value != 0. );
         ctx->tasks_dirty = 1;
         return 0;
     }

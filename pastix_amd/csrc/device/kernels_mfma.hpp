@@ -10,6 +10,7 @@
 #include "kernels_fact.hpp"
 #include "kernels_solve.hpp"
 #include "mfma.hpp"
+#include <type_traits>
 
 // tasks of the block-column inversion: cblk and first column of the block
 struct inv_task {
@@ -1946,3 +1947,234 @@ k_update_big( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *_
     stamp_end( stamps, ti, t_stamp );
 }
 
+
+// ---------------------------------------------------------------------------
+// Single-precision tall-tile update (the 256 x 64 tiles of k_update_big<float>,
+// real LL^T and LU) with its operands moved from HBM straight into LDS by
+// buffer loads that write LDS (no register staging), in a ring of NBUF
+// buffers of BK k values: while a stage's products run, the loads of the
+// next NBUF - 1 stages are in flight, and each stage ends with one barrier.  The staged tiles are k
+// major (sA[k][256 rows], sB[k][64 columns]), so a load instruction moves one
+// k value of 64 consecutive rows (one wave) and every lane writes LDS at its
+// own consecutive word.  A 16-byte LDS read then gives a lane one k value of
+// four rows (or columns): the wave's 64 x 64 block is cut into 16 x 16
+// matrix-core tiles by interleaving, tile i (j) taking the rows (columns)
+// 4 r + i of the block, so that the four values of one read feed the four
+// row (column) tiles of one k step.  The scatter into the facing cblk is the
+// one of k_update_big: a 16-column slab at a time, transposed through LDS so
+// that a lane owns one row and each memory instruction covers 64 consecutive
+// rows of one column.
+//
+// The row mapped by thread tid is row tid of the tile, for the loads and for
+// the scatter alike, so its offsets stay in registers.  The LDS holds the
+// ring and nothing else (NBUF x BK x 1280 bytes); the search keys live in the
+// last buffer until its first load, and the scatter's transposes in the A
+// buffers after the last stage.
+#define PXD_DMA_BM 256
+// measured on the 100^3 FP32 factorization: 8 k values in a ring of 3 (loads
+// of the next two stages in flight, 30 KiB) ahead of 16 x 2 (40 KiB), 12 x 2,
+// 12 x 3 and 8 x 4
+#define PXD_DMA_BK   8
+#define PXD_DMA_NBUF 3
+template <int BK, int NBUF>
+__global__ void __launch_bounds__( PXD_THREADS, 4 )
+k_update_dma( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
+              const dblok *__restrict__ bk, float *L, float *U, uint64_t *__restrict__ stamps, int prio )
+{
+    typedef mfma16<float> M;
+    typedef M::acc_t      acc_t;
+    typedef float f4 __attribute__( ( ext_vector_type( 4 ) ) );
+    constexpr int      BM = PXD_DMA_BM, BN = PXD_UPD_BN;
+    constexpr uint32_t ES = sizeof( float );
+    static_assert( BK % 4 == 0, "a stage is a whole number of 4-k matrix-core steps (one per wave for B)" );
+    static_assert( NBUF >= 2 && NBUF <= 4, "ring of 2 to 4 stages" );
+    static_assert( PXD_THREADS == BM, "thread tid maps row tid of the tile" );
+    static_assert( upd_rows<float>::value == BM, "the tall-tile list of a single factor has 256-row tiles" );
+    static_assert( sizeof( search_keys ) <= BK * BM * sizeof( float ), "search keys fit in one A buffer" );
+    static_assert( 4 * 64 * 17 <= NBUF * BK * BM, "scatter transposes fit in the A buffers" );
+    constexpr int LPS = BK + BK / 4;                    // loads per stage and wave
+    constexpr int VMN = LPS * ( NBUF - 2 );             // loads issued after the next stage's
+    static_assert( VMN < 64, "vmcnt holds 6 bits" );
+    constexpr int WAIT_NEXT = ( VMN & 15 ) | ( ( VMN >> 4 ) << 14 ) | 0x0F70;   // s_waitcnt vmcnt(VMN)
+    __shared__ __align__( 16 ) float smem[NBUF * BK * ( BM + BN )];
+    search_keys &sk = *reinterpret_cast<search_keys *>( smem + ( NBUF - 1 ) * BK * BM );
+
+    const int32_t  ti      = xcd_task( blockIdx.x, ntasks );
+    const upd_task t       = tasks[ti];
+    const uint64_t t_stamp = stamp_begin( stamps );
+    wave_priority( prio );
+    const int     tid = threadIdx.x;
+    const int     lane = tid & 63, w = tid >> 6;
+    const dcblk   c   = cb[t.cblk];
+    const int32_t K   = c.lcolnum - c.fcolnum + 1;
+    const int32_t lb  = cb[t.cblk + 1].fblok;
+    const dblok   bb  = bk[t.blok];
+    const int32_t N   = bb.lrownum - bb.frownum + 1;
+    // U-side tile (LU): A = U^T rows after blok b, B = L rows of blok b, C = U
+    const bool    uside = ( t.flags & 2 ) != 0;
+    const float  *LA    = uside ? U : L;
+    const float  *LB    = uside ? L : U;
+    float        *LC    = uside ? U : L;
+    const int32_t r0    = bb.rowoff + ( uside ? N : 0 );
+    const int32_t Mr    = c.stride - r0;
+    const int32_t F     = bb.fcblknm;
+    const dcblk   cf    = cb[F];
+    const int32_t Fn    = cf.lcolnum - cf.fcolnum + 1;
+    const int32_t flb   = cb[F + 1].fblok;
+    const int32_t colF0 = bb.frownum - cf.fcolnum + t.tn * BN;
+    if ( bb.frownum < cf.fcolnum || bb.lrownum > cf.lcolnum ) {
+        return;   // inconsistent structure: the host checks prevent this
+    }
+    const __amdgpu_buffer_rsrc_t rA = make_rsrc( LA + c.coefoff, (uint32_t)( (int64_t)c.stride * K * ES ) );
+    const __amdgpu_buffer_rsrc_t rB = make_rsrc( LB + c.coefoff, (uint32_t)( (int64_t)c.stride * K * ES ) );
+    const __amdgpu_buffer_rsrc_t rC = make_rsrc( LC + cf.coefoff, (uint32_t)( (int64_t)cf.stride * Fn * ES ) );
+
+    stage_search_keys( sk, bk, t.blok, lb, cf.fblok, flb, tid );
+    __syncthreads();
+
+    // row tid of the tile: byte offset of its first element and byte stride
+    // between its columns in the source, and its place in F
+    uint32_t abase = PXD_OOB, astep = 0, dst = PXD_OOB, dld = 0;
+    {
+        const int32_t r = t.tm * BM + tid;
+        if ( r < Mr ) {
+            const int32_t s  = r0 + r;
+            const dblok   ib = bk[search_row( sk, bk, t.blok, lb, s )];
+            const int32_t g  = ib.frownum + ( s - ib.rowoff );
+            abase = (uint32_t)( ib.rowoff * K + ( s - ib.rowoff ) ) * ES;
+            astep = (uint32_t)( ib.lrownum - ib.frownum + 1 ) * ES;
+            const dblok fb = bk[search_grow( sk, bk, cf.fblok, flb, g )];
+            if ( g >= fb.frownum && g <= fb.lrownum ) {
+                dst = (uint32_t)( fb.rowoff * Fn + ( g - fb.frownum ) ) * ES;
+                dld = (uint32_t)( fb.lrownum - fb.frownum + 1 ) * ES;
+            }
+        }
+    }
+    // facing column `lane` of the tile
+    const int32_t  nvalid = min( BN, N - t.tn * BN );
+    const uint32_t bbase  = ( lane < nvalid ) ? (uint32_t)( bb.rowoff * K + t.tn * BN + lane ) * ES : PXD_OOB;
+    const uint32_t bstep  = ( lane < nvalid ) ? (uint32_t)N * ES : 0;
+
+    // stage k0 into buffer b: wave w loads the BK k values of rows
+    // [64 w, 64 w + 64) and k values w, w + 4, ... of the 64 columns.  The
+    // offsets advance by one column per load (ao, bo carried from stage to
+    // stage), so that no per-k products stay live across the loop.
+    const int wu = __builtin_amdgcn_readfirstlane( w );
+    uint32_t  ao = abase, bo = ( bbase == PXD_OOB ) ? PXD_OOB : bbase + (uint32_t)wu * bstep;
+    auto dma = [&]( auto bc, int32_t k0 ) {
+        constexpr int b  = decltype( bc )::value;
+        float        *dA = smem + b * ( BK * BM ) + 64 * wu;
+        float        *dB = smem + NBUF * BK * BM + b * ( BK * BN ) + wu * BN;
+        const bool    full = ( k0 + BK <= K );
+#pragma unroll
+        for ( int q = 0; q < BK; q++ ) {
+            const uint32_t o = ( full || k0 + q < K ) ? ao : PXD_OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds( rA, (__attribute__( ( address_space( 3 ) ) ) void *)( dA + q * BM ),
+                                                      4, o, 0, 0, 0 );
+            ao += astep;
+        }
+#pragma unroll
+        for ( int q = 0; q < BK / 4; q++ ) {
+            const uint32_t o = ( full || k0 + 4 * q + wu < K ) ? bo : PXD_OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rB, (__attribute__( ( address_space( 3 ) ) ) void *)( dB + 4 * q * BN ), 4, o, 0, 0, 0 );
+            bo += 4 * bstep;
+        }
+    };
+
+    acc_t acc[4][4];
+#pragma unroll
+    for ( int i = 0; i < 4; i++ )
+#pragma unroll
+        for ( int j = 0; j < 4; j++ ) acc[i][j] = M::zero();
+
+    const int     lr = lane & 15, lg = lane >> 4;
+    const int32_t ns = ( K + BK - 1 ) / BK;
+    // stage s from buffer b = s % NBUF, with the loads of stage s + NBUF - 1
+    // into the buffer that stage s - 1 read (issued past the last stage too,
+    // as out-of-range loads, so that every stage leaves the same count)
+    auto stage = [&]( auto bc, int32_t s ) {
+        constexpr int b  = decltype( bc )::value;
+        const float  *SA = smem + b * ( BK * BM ) + 64 * w + 4 * lr;
+        const float  *SB = smem + NBUF * BK * BM + b * ( BK * BN ) + 4 * lr;
+        f4            av[BK / 4], bv[BK / 4];
+#pragma unroll
+        for ( int ks = 0; ks < BK / 4; ks++ ) {
+            av[ks] = *(const f4 *)&SA[( 4 * ks + lg ) * BM];
+            bv[ks] = *(const f4 *)&SB[( 4 * ks + lg ) * BN];
+        }
+        if ( NBUF > 2 || s + 1 < ns ) {
+            dma( std::integral_constant<int, ( b + NBUF - 1 ) % NBUF>{}, ( s + NBUF - 1 ) * BK );
+        }
+#pragma unroll
+        for ( int ks = 0; ks < BK / 4; ks++ )
+#pragma unroll
+            for ( int i = 0; i < 4; i++ )
+#pragma unroll
+                for ( int j = 0; j < 4; j++ ) acc[i][j] = M::mma( av[ks][i], bv[ks][j], acc[i][j] );
+        // the products stay ahead of the wait for the next stage's loads
+        __builtin_amdgcn_sched_barrier( 0 );
+        __builtin_amdgcn_s_waitcnt( WAIT_NEXT );   // this wave's loads of stage s + 1 are in LDS
+        __syncthreads();
+    };
+    dma( std::integral_constant<int, 0>{}, 0 );
+    if constexpr ( NBUF > 2 ) dma( std::integral_constant<int, 1>{}, BK );
+    if constexpr ( NBUF > 3 ) dma( std::integral_constant<int, 2>{}, 2 * BK );
+    __builtin_amdgcn_s_waitcnt( WAIT_NEXT );
+    __syncthreads();
+    for ( int32_t s = 0; s < ns; s += NBUF ) {
+        stage( std::integral_constant<int, 0>{}, s );
+        if ( s + 1 < ns ) stage( std::integral_constant<int, 1>{}, s + 1 );
+        if constexpr ( NBUF > 2 ) {
+            if ( s + 2 < ns ) stage( std::integral_constant<int, 2>{}, s + 2 );
+        }
+        if constexpr ( NBUF > 3 ) {
+            if ( s + 3 < ns ) stage( std::integral_constant<int, 3>{}, s + 3 );
+        }
+    }
+    if constexpr ( NBUF > 2 ) {
+        // the out-of-range loads past the last stage land before the scatter
+        // reuses the buffers
+        __builtin_amdgcn_s_waitcnt( 0x0F70 );
+        __syncthreads();
+    }
+
+    // scatter: slab j holds the columns 4 r + j of the wave's block
+    const bool atomic = ( t.flags & 1 );
+    float     *sE     = smem + w * ( 64 * 17 );
+#pragma unroll
+    for ( int j = 0; j < 4; j++ ) {
+#pragma unroll
+        for ( int i = 0; i < 4; i++ )
+#pragma unroll
+            for ( int v = 0; v < 4; v++ ) sE[( 4 * M::row( lane, v ) + i ) * 17 + lr] = M::get( acc[i][j], v );
+        __syncthreads();
+        // two halves of 8 columns (the other slabs' accumulators are live)
+#pragma unroll
+        for ( int h = 0; h < 16; h += 8 ) {
+            uint32_t off[8];
+            float    val[8];
+#pragma unroll
+            for ( int q = 0; q < 8; q++ ) {
+                const int cc = 4 * ( h + q ) + j;
+                off[q] = ( dst != PXD_OOB && cc < nvalid ) ? dst + (uint32_t)( colF0 + cc ) * dld : PXD_OOB;
+                val[q] = sE[lane * 17 + h + q];
+            }
+            if ( atomic ) {
+#pragma unroll
+                for ( int q = 0; q < 8; q++ ) {
+                    if ( off[q] != PXD_OOB ) px_atomic_add( LC + cf.coefoff + off[q] / ES, -val[q] );
+                }
+            }
+            else {
+                float old[8];
+#pragma unroll
+                for ( int q = 0; q < 8; q++ ) old[q] = bufio<float>::load( rC, off[q] );
+#pragma unroll
+                for ( int q = 0; q < 8; q++ ) bufio<float>::store( old[q] - val[q], rC, off[q] );
+            }
+        }
+        __syncthreads();
+    }
+    stamp_end( stamps, ti, t_stamp );
+}

@@ -84,6 +84,44 @@ def test_update_tilings_and_schedules_match_oracle(dtype, big_rows, timing, pane
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("dma,pad", [(1, 4096), (1, 0), (0, 0)])
+@pytest.mark.parametrize("kind,nx", [("lap7", 20), ("convdiff27", 13)])
+def test_single_tall_tiles_match_oracle(kind, nx, dma, pad):
+    """Every blok update of a single-precision factor on the 256-row tiles
+    (upd_big_rows=1), through the kernel that loads its operands straight
+    into LDS (upd_dma=1, with and without the LDS padding that limits the
+    workgroups per CU) or through the register-staged one (upd_dma=0), for
+    LL^T and for LU (the U-side tiles too): L (and U) match the oracle.  The 20^3 separators give cblks wider
+    than two stages and facing bloks narrower than the 64-column tile."""
+    lu = kind == "convdiff27"
+    spm = make_problem(kind, nx)
+    pd, iparm, dparm = analyzed(spm, nx, facto=px.PastixFactGETRF if lu else None, mixed=1)
+    px.set_option(pd, "upd_big_rows", 1)
+    px.set_option(pd, "upd_small", 0)
+    px.set_option(pd, "upd_dma", dma)
+    px.set_option(pd, "upd_dma_pad", pad)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    L = px.get_coeftab(pd, s["coefnbr"], np.float32)
+    if lu:
+        F = orc.OracleLU(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float32), np.float32)
+        U = px.get_coeftab(pd, s["coefnbr"], np.float32, side=1)
+        assert cblk_diff(s, L, F.L, np.float32, full_diag=True) <= 10.0
+        assert cblk_diff(s, U, F.U, np.float32, full_diag=True) <= 10.0
+    else:
+        F = orc.OracleFactor(s, px.bcsc_export(pd), static_pivot_threshold(dparm, np.float32), np.float32)
+        assert cblk_diff(s, L, F.L, np.float32) <= 10.0
+    px.finalize(pd)
+
+
+def test_tall_tile_kernel_option_values():
+    spm = make_problem("lap7", 6)
+    pd, iparm, dparm = analyzed(spm, 6, mixed=1)
+    with pytest.raises(px.PastixError):
+        px.set_option(pd, "upd_dma_pad", 1 << 20)
+    px.finalize(pd)
+
+
 def test_device_solve_matches_oracle_solve():
     nx = 12
     spm, pd, iparm, dparm = _factor_case(nx)
