@@ -455,10 +455,14 @@ def main():
         return ach, upd
 
     achieved, upd = update_roofline(head, "f32")
-    traffic = None
+    traffic = traffic64 = traffic_launch = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
-            traffic = json.load(f).get("hbm_bytes_per_factorization")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_factorization")
+        traffic64 = tj.get("fp64_hbm_bytes_per_factorization")
+        if tj.get("fetch_bytes_per_launch_k_update_dma"):
+            traffic_launch = tj["fetch_bytes_per_launch_k_update_dma"] + tj["write_bytes_per_launch_k_update_dma"]
     out = {
         "metric": "fact+solve s & GFLOP/s at bwd err<=1e-12: FP32+IR vs FP64, 100^3 Laplacian",
         "value": head["tts"],
@@ -480,6 +484,8 @@ def main():
         "gflops_fact_binary": fl / head["fact"] / 1024 ** 3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS["f32"],
                      "unit": "TFLOP/s", "frac": achieved / MFMA_PEAK_TFLOPS["f32"], "traffic": traffic,
+                     "traffic_scope": "HBM bytes of the update launches of one factorization (rocprofv3 FETCH_SIZE "
+                                      "+ WRITE_SIZE, copy-calibrated)", "traffic_per_launch": traffic_launch,
                      "kernel": "k_update_dma<8,3> + k_update_mfma<float> + k_update_small<float> (supernodal "
                                "update, all launches of one factorization)", "algorithmic_flops": upd},
         "cpu_baseline": None,
@@ -492,7 +498,7 @@ def main():
         out["speedup_fp32ir_vs_fp64"] = r64["tts"] / head["tts"]
         a64, u64 = update_roofline(r64, "f64")
         out["roofline_fp64"] = {"bound": "mfma", "achieved": a64, "peak": MFMA_PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                                "frac": a64 / MFMA_PEAK_TFLOPS["f64"], "traffic": None,
+                                "frac": a64 / MFMA_PEAK_TFLOPS["f64"], "traffic": traffic64,
                                 "kernel": "k_update_big<double> + k_update_mfma<double> + k_update_small<double> "
                                           "(all launches of one factorization)", "algorithmic_flops": u64}
         out["roofline_solve_fp64"] = r64["bandwidth"]["solve"]
