@@ -94,7 +94,7 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *                   lookahead stream's panels)
  *   "fact_graph"    1: the factorization's level loop replays a graph
  *                   captured once per structure (default)
- *   "prio"         1: raised wave priority on the critical path (default)
+ *   "prio"          1: raised wave priority on the critical path (default)
  *   "solve_graph"   1: each solve pass replays the sweeps' launches from a
  *                   graph captured on first use (default); 0: launched one by one
  *   "solve_wide_max" level-solve launches with at least this many tiles load

@@ -174,8 +174,8 @@ struct pxd_ctx_s {
     int          upd_dma = 1;
     // dynamic LDS bytes added to each k_update_dma workgroup: 4 KiB makes it
     // 34 KiB, four workgroups per CU, and leaves 24 KiB of the CU's LDS to the
-    // panel and TRSM workgroups of the lookahead stream (without it five fit
-    // and the next level's panels wait for a CU)
+    // panel and TRSM workgroups of the lookahead stream (without it, or with
+    // less free, the next level's panels wait for a CU)
     unsigned     upd_dma_pad = 4096;
 
     // host copies of the panel, TRSM and inversion lists, and the blok

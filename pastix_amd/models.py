@@ -502,6 +502,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--transfer", action="store_true",
                     help="fit on the 20^3 / 100^3 Laplacian and 64^3 LU logs, test on the 60^3 x 3 dof log")
+    ap.add_argument("--choice", action="store_true",
+                    help="fit the per-class models on the 20^3 / 100^3 Laplacian and 64^3 LU logs, then let "
+                         "the cut-offs, the polynomials and the perceptron choose the update kernels of the "
+                         "60^3 x 3 dof factorization, and time each choice")
+    ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--nx", type=int, default=100)
     ap.add_argument("--mixed", type=int, default=1, help="1: FP32 factor, 0: FP64")
     ap.add_argument("--steps", type=int, default=3000)
@@ -512,6 +517,16 @@ def main(argv=None):
         import sys
         out = transfer(steps=a.steps, log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(out, indent=1))
+        return
+    if a.choice:
+        import sys
+        train = [("20^3 Laplacian", "lap7", 20, 1, "POTRF"), ("100^3 Laplacian", "lap7", 100, 1, "POTRF"),
+                 ("64^3 convection-diffusion", "convdiff27", 64, 1, "GETRF")]
+        test = ("60^3 x 3 dof vector Laplacian", "vlap7", 60, 3, "POTRF")
+        out, pd, spm, dparm = choice_experiment(px, train, test, mixed=a.mixed, steps=a.steps, reps=a.reps,
+                                                log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(out, indent=1))
+        px.finalize(pd)
         return
     spm = px.Spm.laplacian7(a.nx, a.nx, a.nx, 6.0, 1.0)
     iparm, dparm = px.initParam()
