@@ -16,7 +16,9 @@ rounding:
 - ``upd_dma``: single-precision tall tiles on the kernel that loads its
   operands straight into LDS;
 - ``upd_dma_pad``: extra LDS per workgroup of that kernel, which sets how
-  many of its workgroups share a CU with the lookahead stream's panels.
+  many of its workgroups share a CU with the lookahead stream's panels;
+- ``solve_split_max``, ``solve_split_wg``: how many workgroups share a tile
+  of a level-solve launch with few tiles.
 
 The search is coordinate descent from the current values (each option in
 turn over its candidates, the others fixed, two passes); the objective is
@@ -50,9 +52,11 @@ SPACE = {
     "panel_mw": [0, 1],
     "upd_dma": [0, 1],
     "upd_dma_pad": [0, 2048, 4096, 6144, 8192],
+    "solve_split_max": [1, 2, 4, 8],
+    "solve_split_wg": [256, 512, 1024, 2048],
 }
 DEFAULTS = {"upd_big_rows": 192, "upd_small": 1, "solve_wide_max": 128, "panel_mw": 1, "upd_dma": 1,
-            "upd_dma_pad": 4096}
+            "upd_dma_pad": 4096, "solve_split_max": 4, "solve_split_wg": 512}
 
 # problems: (label, generator, grid edge, factorization, mixed precision, RHS)
 PROBLEMS = {

@@ -172,6 +172,10 @@ struct pxd_ctx_s {
     // 1: single-precision LL^T / LU tall tiles on the kernel that loads its
     // operands straight into LDS (k_update_dma); 0: on k_update_big
     int          upd_dma = 1;
+    // level-solve launches with fewer than solve_wide_max tiles share each
+    // tile among up to solve_split_max workgroups, solve_split_wg in all
+    int          solve_split_max = 4;
+    int64_t      solve_split_wg  = 512;
     // dynamic LDS bytes added to each k_update_dma workgroup: 4 KiB makes it
     // 34 KiB, four workgroups per CU, and leaves 24 KiB of the CU's LDS to the
     // panel and TRSM workgroups of the lookahead stream (without it, or with
@@ -1079,6 +1083,14 @@ sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned
     hipStream_t   s  = ctx->stream;
     const size_t  sh = (size_t)lv.maxn * NR * sizeof( V );
     const bool    many = (int64_t)nf >= ctx->solve_wide_max;
+    // launches with few tiles share each tile among `split` workgroups (the
+    // columns, or the 64-column chunks backward), up to solve_split_wg
+    // workgroups in all
+    int split = 1;
+    if ( !many && nf ) {
+        split = std::max( 1, std::min( ctx->solve_split_max, (int)( ctx->solve_split_wg / (int64_t)nf ) ) );
+    }
+    const int split_b = std::max( 1, std::min( split, ( lv.maxn + 63 ) / 64 ) );
     if ( forward ) {
         if ( nd ) {
             hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
@@ -1086,21 +1098,21 @@ sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned
         }
         if ( nf && many ) {
             hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs );
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs, 1 );
         }
         else if ( nf ) {
-            hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, WB> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
-                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs );
+            hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, WB> ), dim3( nf * split ), dim3( PXD_THREADS ), sh, s,
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, X, ldx, (const V *)Y, ldy, nrhs, split );
         }
     }
     else {
         if ( nf && many ) {
             hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs );
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, 1 );
         }
         else if ( nf ) {
-            hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, WB / 16> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs );
+            hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, WB / 16> ), dim3( nf * split_b ), dim3( PXD_THREADS ), 0,
+                                s, ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, split_b );
         }
         if ( nd ) {
             hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
@@ -1772,6 +1784,15 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "solve_split_max" || k == "solve_split_wg" ) {
+        if ( value < 1. || value > 1e6 ) {
+            return 7;
+        }
+        if ( k == "solve_split_max" ) ctx->solve_split_max = (int)value;
+        else ctx->solve_split_wg = (int64_t)value;
+        drop_solve_graphs( ctx );
         return 0;
     }
     if ( k == "upd_dma_pad" ) {

@@ -67,18 +67,22 @@ sv_stage( V *s, const V *__restrict__ v, int64_t ld, int32_t n, int nrhs, int ti
 }
 
 // Forward tile: X[g(s)] -= A(s, :) y_k for its 64 rows, y_k = Y[fcolnum ..].
-// Lane r of each wave takes row r0 + r; wave w the columns [n w / 4,
-// n (w + 1) / 4), B at a time.  Dynamic LDS: n * NR values of V.
+// `split` workgroups share a tile (launches with few tiles): workgroup part
+// ps takes the columns [n ps / split, n (ps + 1) / split) and adds its
+// partial sums.  Lane r of each wave takes row r0 + r; wave w a quarter of
+// the workgroup's columns, B at a time.  Dynamic LDS: n * NR values of V.
 template <typename T, typename V, int NR, int B>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ rows, const T *__restrict__ L,
-               V *X, int64_t ldx, const V *__restrict__ Y, int64_t ldy, int nrhs )
+               V *X, int64_t ldx, const V *__restrict__ Y, int64_t ldy, int nrhs, int split )
 {
     extern __shared__ unsigned char smem_raw[];
     V *sy = reinterpret_cast<V *>( smem_raw );
     __shared__ V red[4][64][NR];
 
-    const sv_tile t   = tiles[xcd_task( blockIdx.x, gridDim.x )];
+    const int     idx = xcd_task( blockIdx.x, gridDim.x );
+    const sv_tile t   = tiles[idx / split];
+    const int     ps  = idx % split;
     const int     tid = threadIdx.x;
     const int     r = tid & 63, part = tid >> 6;
     const bool    valid = t.r0 + r < t.stride;
@@ -90,7 +94,8 @@ k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
     V acc[NR];
 #pragma unroll
     for ( int q = 0; q < NR; q++ ) acc[q] = (V)0;
-    const int32_t c0 = ( t.n * part ) / 4, c1 = ( t.n * ( part + 1 ) ) / 4;
+    const int32_t cs0 = ( t.n * ps ) / split, cw = ( t.n * ( ps + 1 ) ) / split - cs0;
+    const int32_t c0 = cs0 + ( cw * part ) / 4, c1 = cs0 + ( cw * ( part + 1 ) ) / 4;
     __syncthreads();
     if ( valid ) {
         for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
@@ -124,17 +129,21 @@ k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
 // Backward tile: Y[fcolnum + j] -= sum over its rows s of A(s, j) X[g(s)].
 // The tile passes through LDS in 64-column chunks (coalesced loads along the
 // rows); four threads per column reduce 16 rows each; NCH chunks are loaded
-// at once (all of them for launches with few workgroups).
+// at once (all of them for launches with few workgroups).  `split`
+// workgroups share a tile: workgroup part ps takes the chunks ps, ps +
+// split, ...
 template <typename T, typename V, int NR, int NCH>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ rows, const T *__restrict__ L,
-               const V *__restrict__ X, int64_t ldx, V *Y, int64_t ldy, int nrhs )
+               const V *__restrict__ X, int64_t ldx, V *Y, int64_t ldy, int nrhs, int split )
 {
     __shared__ T sA[64][64 + 1];   // sA[col][row]
     __shared__ V sx[64][NR];
     __shared__ V red[4][64][NR + 1];
 
-    const sv_tile t   = tiles[xcd_task( blockIdx.x, gridDim.x )];
+    const int     idx = xcd_task( blockIdx.x, gridDim.x );
+    const sv_tile t   = tiles[idx / split];
+    const int     ps  = idx % split;
     const int     tid = threadIdx.x;
     const int     rr = tid & 63, part = tid >> 6;
     const bool    valid = t.r0 + rr < t.stride;
@@ -152,14 +161,15 @@ k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
     }
     constexpr int NU = 64 * 64 / PXD_THREADS;
     T             v[NCH][NU];
-    for ( int32_t c0 = 0; c0 < t.n; c0 += 64 ) {
-        const int ch = ( c0 / 64 ) % NCH;
+    int kk = 0;
+    for ( int32_t c0 = 64 * ps; c0 < t.n; c0 += 64 * split, kk++ ) {
+        const int ch = kk % NCH;
         if ( ch == 0 ) {
 #pragma unroll
             for ( int h = 0; h < NCH; h++ )
 #pragma unroll
                 for ( int u = 0; u < NU; u++ ) {
-                    const int32_t cc = c0 + 64 * h + part + 4 * u;
+                    const int32_t cc = c0 + 64 * split * h + part + 4 * u;
                     v[h][u] = ( valid && cc < t.n ) ? L[ri.base + (int64_t)cc * ri.ld] : (T)0;
                 }
         }
