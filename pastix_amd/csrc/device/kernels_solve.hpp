@@ -242,7 +242,9 @@ k_spmv_grp( int64_t n, const int64_t *__restrict__ colptr, const int64_t *__rest
             int nrhs, double alpha, double beta )
 {
     typedef typename px_real<TV>::type RV;
-    const int64_t j   = ( (int64_t)blockIdx.x * blockDim.x + threadIdx.x ) / G;
+    // the workgroups of one XCD take a contiguous range of columns, so that
+    // the rows of x that neighbouring columns gather share that XCD's L2
+    const int64_t j   = ( (int64_t)xcd_task( blockIdx.x, gridDim.x ) * blockDim.x + threadIdx.x ) / G;
     const int     sub = threadIdx.x % G;
     // rows past n take part in the shuffles with no entries
     int64_t e0 = 0, e1 = 0;
