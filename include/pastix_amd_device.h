@@ -99,6 +99,8 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *                   graph captured on first use (default); 0: launched one by one
  *   "solve_wide_max" level-solve launches with at least this many tiles load
  *                   16 columns per batch (default 128)
+ *   "solve_small"   1: levels of cblks of at most 16 columns solve with one
+ *                   wavefront per cblk, one launch per level (default)
  *   "solve_split_max", "solve_split_wg"  smaller launches share each tile
  *                   among up to solve_split_max workgroups (default 4),
  *                   solve_split_wg workgroups in all (default 512)

@@ -41,6 +41,7 @@ struct level_t {
     int64_t s0, sc, s1; // small updates (one wavefront each); [s0, sc) face the next level
     int64_t f0, fs, f1; // solve row tasks (64-row tiles of off-diagonal rows); [fs, f1): Schur cblks
     int64_t d0, ds, d1; // solve diagonal tasks (PXD_DIAG_ROWS-row chunks of the diagonal blocks); [ds, d1): Schur
+    int64_t m0, m1;     // one-wavefront solve tasks (levels whose cblks all fit one chunk, no Schur part)
     int64_t i0, i1;     // block-column inversion tasks
     int64_t g0, g1;     // blok updates (upd_group)
     int64_t q0, q1;     // multi-workgroup panel tasks (32-row blocks of the diagonal tiles)
@@ -122,6 +123,10 @@ struct pxd_ctx_s {
     sv_tile  *dsvt  = nullptr;   // the solv tasks with their cblk sizes
     sv_row   *dsvr  = nullptr;   // records of the off-diagonal stacked rows
     sv_chunk *dsvc  = nullptr;   // the diag tasks with their cblk sizes
+    sv_small *dsvs  = nullptr;   // the one-wavefront solve tasks
+    // 1: levels of cblks of at most PXD_DIAG_ROWS columns solve with one
+    // wavefront per cblk (diagonal and off-diagonal rows in one launch)
+    int       solve_small = 1;
     row_task *ddiag = nullptr;
     inv_task *dinvt = nullptr;
     int64_t   nupd = 0, ntrsm = 0, nsolv = 0;
@@ -358,6 +363,7 @@ build_schedule( pxd_ctx_t *ctx )
     struct iv { int32_t side, F, f, l; std::vector<upd_task> *list; int64_t first, last; };
     const bool lu = ( ctx->facto == FACT_GETRF );
     std::vector<iv> ivs;
+    std::vector<int32_t> smallk;
     ctx->groups.clear();
 
     for ( int32_t L = 0; L < nlev; L++ ) {
@@ -376,9 +382,11 @@ build_schedule( pxd_ctx_t *ctx )
         ivs.clear();
         lv.ds = -1;
         lv.fs = -1;
+        int32_t maxn_all = 0;
         for ( int32_t k : bylev[L] ) {
             const dcblk &c  = ctx->hcb[k];
             int32_t      nk = c.lcolnum - c.fcolnum + 1;
+            maxn_all        = std::max( maxn_all, nk );
             if ( lv.ds < 0 && c.fcolnum >= ctx->schurfcol ) {
                 lv.ds = diag.size();
                 lv.fs = solv.size();
@@ -411,6 +419,11 @@ build_schedule( pxd_ctx_t *ctx )
             lv.ds = diag.size();
             lv.fs = solv.size();
         }
+        lv.m0 = smallk.size();
+        if ( lv.ds == (int64_t)diag.size() && maxn_all <= PXD_DIAG_ROWS ) {
+            for ( int32_t k : bylev[L] ) smallk.push_back( k );
+        }
+        lv.m1 = smallk.size();
         // update tiles: first those into cblks of the next level (the
         // critical part that the next level's panels wait for), then the rest
         for ( int pass = 0; pass < 2; pass++ ) {
@@ -569,7 +582,14 @@ build_schedule( pxd_ctx_t *ctx )
             const dcblk &c = ctx->hcb[diag[i].cblk];
             svc[i] = sv_chunk{ c.lcolnum - c.fcolnum + 1, c.fcolnum, diag[i].r0, 0, c.dinvoff };
         }
-        if ( upload( &ctx->dsvt, svt ) || upload( &ctx->dsvr, rows ) || upload( &ctx->dsvc, svc ) ) {
+        std::vector<sv_small> svs( smallk.size() );
+        for ( size_t i = 0; i < smallk.size(); i++ ) {
+            const dcblk  &c = ctx->hcb[smallk[i]];
+            const int32_t n = c.lcolnum - c.fcolnum + 1;
+            svs[i] = sv_small{ n, c.fcolnum, c.stride - n, 0, c.dinvoff, rowstart[smallk[i]] };
+        }
+        if ( upload( &ctx->dsvt, svt ) || upload( &ctx->dsvr, rows ) || upload( &ctx->dsvc, svc ) ||
+             upload( &ctx->dsvs, svs ) ) {
             return 6;
         }
     }
@@ -1091,6 +1111,21 @@ sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned
         split = std::max( 1, std::min( ctx->solve_split_max, (int)( ctx->solve_split_wg / (int64_t)nf ) ) );
     }
     const int split_b = std::max( 1, std::min( split, ( lv.maxn + 63 ) / 64 ) );
+    const int64_t nm  = lv.m1 - lv.m0;
+    if constexpr ( NR == 4 && sizeof( V ) <= 8 ) {   // complex double keeps the chunk and tile launches
+        if ( ctx->solve_small && nm > 0 && nrhs <= 4 ) {
+            const unsigned nb = (unsigned)( ( nm + 3 ) / 4 );
+            if ( forward ) {
+                hipLaunchKernelGGL( ( k_sv_small_fwd<T, V> ), dim3( nb ), dim3( PXD_THREADS ), 0, s, ctx->dsvs + lv.m0,
+                                    nm, ctx->dsvr, L, DI, X, ldx, Y, ldy, nrhs );
+            }
+            else {
+                hipLaunchKernelGGL( ( k_sv_small_bwd<T, V> ), dim3( nb ), dim3( PXD_THREADS ), 0, s, ctx->dsvs + lv.m0,
+                                    nm, ctx->dsvr, L, DI, X, ldx, (const V *)Y, ldy, nrhs );
+            }
+            return;
+        }
+    }
     if ( forward ) {
         if ( nd ) {
             hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
@@ -1732,7 +1767,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     if ( ctx->fgraph ) {
         (void)hipGraphExecDestroy( ctx->fgraph );
     }
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->dsvs, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred,
                      };
@@ -1784,6 +1819,11 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "solve_small" ) {
+        ctx->solve_small = ( value != 0. );
+        drop_solve_graphs( ctx );
         return 0;
     }
     if ( k == "solve_split_max" || k == "solve_split_wg" ) {

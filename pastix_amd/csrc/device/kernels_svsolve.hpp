@@ -264,3 +264,146 @@ k_sv_diag( const sv_chunk *__restrict__ chunks, const T *__restrict__ Linv, cons
         if ( ch.R0 + r2 < n && q < nrhs ) Yout[ch.fcolnum + ch.R0 + r2 + q * ldy] = v;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Levels whose cblks all fit one diagonal chunk (at most PXD_DIAG_ROWS
+// columns: the leaves of the nested dissection, ~60k cblks of 1-16 columns
+// for 100^3) run the diagonal solve and the off-diagonal rows of a cblk in
+// one wavefront, four cblks per workgroup, in one launch per level.  The
+// wavefront exchanges its right-hand-side block through its own LDS area;
+// its LDS accesses complete in program order, so no workgroup barrier is
+// needed.  NR = 4 right-hand sides at most: lane (i, q) = (l & 15, l >> 4).
+struct sv_small {
+    int32_t n;         // columns of the cblk (<= PXD_DIAG_ROWS)
+    int32_t fcolnum;
+    int32_t nrows;     // off-diagonal stacked rows
+    int32_t pad;
+    int64_t dinvoff;   // its diagonal block's inverse
+    int64_t row0;      // record of its first off-diagonal row
+};
+
+__device__ inline void
+sv_wave_sync()
+{
+    __builtin_amdgcn_s_waitcnt( 0xC07F );   // lgkmcnt(0)
+    __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+}
+
+// Forward: y_k = Linv_k X[k rows] into Y, then X[g(s)] -= A(s, :) y_k for
+// the cblk's off-diagonal rows s (atomics: other cblks add to the same rows).
+template <typename T, typename V>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_sv_small_fwd( const sv_small *__restrict__ tasks, int64_t ntasks, const sv_row *__restrict__ rows,
+                const T *__restrict__ L, const T *__restrict__ Linv, V *X, int64_t ldx, V *__restrict__ Y,
+                int64_t ldy, int nrhs )
+{
+    constexpr int CR = PXD_DIAG_ROWS;
+    static_assert( CR * 4 == 64, "lane (i, q) covers 16 rows and 4 right-hand sides" );
+    __shared__ V sb[4][CR][4];
+    const int     tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int64_t ti  = (int64_t)xcd_task( blockIdx.x, gridDim.x ) * 4 + w;
+    if ( ti >= ntasks ) {
+        return;   // no workgroup barrier follows
+    }
+    const sv_small t = tasks[ti];
+    const int      i = lane & ( CR - 1 ), q = lane >> 4;
+    const T       *LI = Linv + t.dinvoff;
+    T              a[CR];
+#pragma unroll
+    for ( int j = 0; j < CR; j++ ) a[j] = ( i < t.n && j <= i ) ? LI[i + (int64_t)j * t.n] : (T)0;
+    const V b = ( i < t.n && q < nrhs ) ? X[t.fcolnum + i + q * ldx] : (V)0;
+    sb[w][i][q] = b;
+    sv_wave_sync();
+    V y = (V)0;
+#pragma unroll
+    for ( int j = 0; j < CR; j++ ) y += px_cvt<V>( a[j] ) * sb[w][j][q];
+    sv_wave_sync();
+    sb[w][i][q] = y;
+    if ( i < t.n && q < nrhs ) Y[t.fcolnum + i + q * ldy] = y;
+    sv_wave_sync();
+    for ( int32_t s = lane; s < t.nrows; s += 64 ) {
+        const sv_row ri = rows[t.row0 + s];
+        T            av[CR];
+#pragma unroll
+        for ( int j = 0; j < CR; j++ ) av[j] = ( j < t.n ) ? L[ri.base + (int64_t)j * ri.ld] : (T)0;
+#pragma unroll
+        for ( int qq = 0; qq < 4; qq++ ) {
+            if ( qq < nrhs ) {
+                V acc = (V)0;
+#pragma unroll
+                for ( int j = 0; j < CR; j++ ) acc += px_cvt<V>( av[j] ) * sb[w][j][qq];
+                px_atomic_add( X + ri.g + qq * ldx, -acc );
+            }
+        }
+    }
+}
+
+// Backward: y_k = Y[k rows] - sum over the off-diagonal rows s of A(s, :)^T
+// X[g(s)], then X[k rows] = Linv_k^T y_k.  The gather's partial sums of the
+// wavefront's lanes (one row each) meet through LDS, one right-hand side at
+// a time.
+template <typename T, typename V>
+__global__ void __launch_bounds__( PXD_THREADS )
+k_sv_small_bwd( const sv_small *__restrict__ tasks, int64_t ntasks, const sv_row *__restrict__ rows,
+                const T *__restrict__ L, const T *__restrict__ Linv, V *X, int64_t ldx, const V *__restrict__ Y,
+                int64_t ldy, int nrhs )
+{
+    constexpr int CR = PXD_DIAG_ROWS;
+    __shared__ V sb[4][CR][4];
+    __shared__ V sred[4][64][CR + 1];
+    const int     tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int64_t ti  = (int64_t)xcd_task( blockIdx.x, gridDim.x ) * 4 + w;
+    if ( ti >= ntasks ) {
+        return;   // no workgroup barrier follows
+    }
+    const sv_small t = tasks[ti];
+    const int      i = lane & ( CR - 1 ), q = lane >> 4;
+    // Linv^T row i (column i of Linv) and y_k, issued with the gather's loads
+    const T *LI = Linv + t.dinvoff;
+    T        a[CR];
+#pragma unroll
+    for ( int j = 0; j < CR; j++ ) a[j] = ( i < t.n && j >= i && j < t.n ) ? LI[j + (int64_t)i * t.n] : (T)0;
+    const V yk = ( i < t.n && q < nrhs ) ? Y[t.fcolnum + i + q * ldy] : (V)0;
+    V       p[4][CR];
+#pragma unroll
+    for ( int qq = 0; qq < 4; qq++ )
+#pragma unroll
+        for ( int j = 0; j < CR; j++ ) p[qq][j] = (V)0;
+    for ( int32_t s = lane; s < t.nrows; s += 64 ) {
+        const sv_row ri = rows[t.row0 + s];
+        T            av[CR];
+#pragma unroll
+        for ( int j = 0; j < CR; j++ ) av[j] = ( j < t.n ) ? L[ri.base + (int64_t)j * ri.ld] : (T)0;
+        V xs[4];
+#pragma unroll
+        for ( int qq = 0; qq < 4; qq++ ) xs[qq] = ( qq < nrhs ) ? X[ri.g + qq * ldx] : (V)0;
+#pragma unroll
+        for ( int qq = 0; qq < 4; qq++ )
+#pragma unroll
+            for ( int j = 0; j < CR; j++ ) p[qq][j] += px_cvt<V>( av[j] ) * xs[qq];
+    }
+    // gather sum of (j, qq): lane (j, part) adds the rows part * 16 .. + 15,
+    // then the four parts meet in LDS
+    V g = (V)0;
+#pragma unroll
+    for ( int qq = 0; qq < 4; qq++ ) {
+#pragma unroll
+        for ( int j = 0; j < CR; j++ ) sred[w][lane][j] = p[qq][j];
+        sv_wave_sync();
+        V part = (V)0;
+#pragma unroll
+        for ( int r = 0; r < 16; r++ ) part += sred[w][16 * q + r][i];
+        sv_wave_sync();
+        sred[w][lane][0] = part;
+        sv_wave_sync();
+        const V tot = sred[w][i][0] + sred[w][16 + i][0] + sred[w][32 + i][0] + sred[w][48 + i][0];
+        if ( q == qq ) g = tot;
+        sv_wave_sync();
+    }
+    sb[w][i][q] = yk - g;
+    sv_wave_sync();
+    V x = (V)0;
+#pragma unroll
+    for ( int j = 0; j < CR; j++ ) x += px_cvt<V>( a[j] ) * sb[w][j][q];
+    if ( i < t.n && q < nrhs ) X[t.fcolnum + i + q * ldx] = x;
+}
