@@ -14,7 +14,8 @@ error of the FP32+IR solution against the FP64 factorization's, and the
 CPU baseline: the oracle's static-scheduler FP64 Cholesky + solve of the
 same 100^3 problem on the host cores this process may use (affinity mask
 capped by the cgroup quota), after a small warm-up factorization
-(`--cpu-reps` timed repetitions, default 1, about 20 s on 16 cores).
+(`--cpu-reps` timed repetitions, default 3, about 11 s each on 16 cores;
+the line gives their median and minimum).
 
 Prints one JSON line (the driver's contract).  `--gpus` above 1 is refused:
 the engine drives one MI355X.
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--berr", type=float, default=1e-12)
     ap.add_argument("--cpu-grid", type=int, default=None,
                     help="grid edge of the CPU baseline (default: the workload's own, --n)")
-    ap.add_argument("--cpu-reps", type=int, default=1,
+    ap.add_argument("--cpu-reps", type=int, default=3,
                     help="timed repetitions of the CPU baseline (after a small warm-up factorization)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
