@@ -87,6 +87,10 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *   "panel_mw"      1: multi-workgroup Cholesky panels (default)
  *   "upd_big_rows"  stacked rows from which updates use the tall tiles
  *   "upd_small"     1: one-wavefront small updates (default)
+ *   "upd_small_m", "upd_small_n"  largest stacked rows (up to 256, default
+ *                   64) and facing columns (up to 64, default 16) of an
+ *                   update that goes to the one-wavefront kernel (one
+ *                   wavefront per 64 rows; at most 32 source columns)
  *   "upd_dma"       1: single-precision LL^T / LU tall tiles load their
  *                   operands straight into LDS (default); 0: register-staged
  *   "upd_dma_pad"   extra LDS bytes per workgroup of that kernel (0..65536,

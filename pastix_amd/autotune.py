@@ -8,8 +8,9 @@ rounding:
 
 - ``upd_big_rows``: stacked rows from which a blok update runs on the tall
   256/128-row tiles instead of 64 x 64 tiles;
-- ``upd_small``: the one-wavefront kernel for updates of at most 64 rows,
-  16 columns and 32 k values;
+- ``upd_small``, ``upd_small_m``, ``upd_small_n``: the one-wavefront kernel
+  for updates of at most upd_small_m rows, upd_small_n columns and 32 k
+  values;
 - ``solve_wide_max``: level-solve launches from which the tiles load 16
   columns per batch instead of all of them;
 - ``panel_mw``: the multi-workgroup Cholesky panel;
@@ -52,11 +53,14 @@ SPACE = {
     "panel_mw": [0, 1],
     "upd_dma": [0, 1],
     "upd_dma_pad": [0, 2048, 4096, 6144, 8192],
+    "upd_small_m": [64, 128, 256],
+    "upd_small_n": [16, 32, 64],
     "solve_split_max": [1, 2, 4, 8],
     "solve_split_wg": [256, 512, 1024, 2048],
 }
 DEFAULTS = {"upd_big_rows": 192, "upd_small": 1, "solve_wide_max": 128, "panel_mw": 1, "upd_dma": 1,
-            "upd_dma_pad": 4096, "solve_split_max": 4, "solve_split_wg": 512}
+            "upd_dma_pad": 4096, "upd_small_m": 64, "upd_small_n": 16, "solve_split_max": 4,
+            "solve_split_wg": 512}
 
 # problems: (label, generator, grid edge, factorization, mixed precision, RHS)
 PROBLEMS = {

@@ -172,6 +172,12 @@ struct pxd_ctx_s {
     // 1: updates of at most 64 rows, PXD_SMALL_N columns and PXD_SMALL_K
     // source columns take one wavefront each (k_update_small)
     int          upd_small = 1;
+    // the cut-off rule gives the one-wavefront kernel the updates of at most
+    // upd_small_m stacked rows and upd_small_n facing columns (and at most
+    // PXD_SMALL_K source columns); a caller's choice may use the kernel's
+    // whole range (PXD_SMALL_M, PXD_SMALL_N)
+    int          upd_small_m = 64;
+    int          upd_small_n = 16;
     // stacked rows per tall tile (upd_rows<T>, set with the factor precision)
     int          upd_bm = 256;
     // 1: single-precision LL^T / LU tall tiles on the kernel that loads its
@@ -454,15 +460,18 @@ build_schedule( pxd_ctx_t *ctx )
                         const int32_t K  = c.lcolnum - c.fcolnum + 1;
                         // kernel class: the caller's choice when it fits the
                         // update, else the cut-offs
-                        const bool small_fits = big_ok && M <= 64 && N <= PXD_SMALL_N && K <= PXD_SMALL_K;
+                        const bool small_fits = big_ok && M <= PXD_SMALL_M && N <= PXD_SMALL_N && K <= PXD_SMALL_K;
+                        const bool small_cut  = small_fits && M <= ctx->upd_small_m && N <= ctx->upd_small_n;
                         int        cls        = ctx->upd_cls.empty() ? -1 : ctx->upd_cls[2 * (size_t)b + side];
                         if ( ( cls == 2 && !small_fits ) || ( cls == 1 && !big_ok ) || cls < 0 || cls > 2 ) {
-                            cls = ( small_fits && ctx->upd_small ) ? 2 : ( big_ok && M >= ctx->upd_big_rows ) ? 1 : 0;
+                            cls = ( small_cut && ctx->upd_small ) ? 2 : ( big_ok && M >= ctx->upd_big_rows ) ? 1 : 0;
                         }
                         if ( cls == 2 ) {
-                            // one wavefront for the whole update
+                            // one wavefront per 64 stacked rows
                             int64_t first = upds.size();
-                            upds.push_back( upd_task{ k, b, 0, 0, fl, 0 } );
+                            for ( int32_t tm = 0; tm * 64 < M; tm++ ) {
+                                upds.push_back( upd_task{ k, b, tm, 0, fl, 0 } );
+                            }
                             ivs.push_back( iv{ side, bb.fcblknm, bb.frownum, bb.lrownum, &upds, first,
                                                (int64_t)upds.size() } );
                             ctx->groups.push_back( upd_group{ k, b, M, N, K, 2, side, first, (int64_t)upds.size() } );
@@ -1845,6 +1854,13 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     }
     if ( k == "upd_dma" ) {
         ctx->upd_dma     = ( value != 0. );
+        ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "upd_small_m" || k == "upd_small_n" ) {
+        const int v = (int)std::max( 0., std::min( value, 1e6 ) );
+        if ( k == "upd_small_m" ) ctx->upd_small_m = std::min( v, PXD_SMALL_M );
+        else ctx->upd_small_n = std::min( v, PXD_SMALL_N );
         ctx->tasks_dirty = 1;
         return 0;
     }

@@ -1591,20 +1591,28 @@ k_update_mfma( const upd_task *__restrict__ tasks, const dcblk *__restrict__ cb,
 }
 
 // ---------------------------------------------------------------------------
-// Small supernodal updates, one wavefront per blok update (the lower tree
-// has hundreds of thousands of them with a few rows and columns): update
-// {cblk, blok, flags} with at most 64 stacked rows (Mr), PXD_SMALL_N facing
-// columns (N) and PXD_SMALL_K source columns (K).  Lane r owns stacked row
-// r: C[f(r), :] -= A_r B^T, B (the rows of blok b, D-scaled for LDL^T) staged
-// in LDS; plain read-modify-write, or atomic adds when the task is flagged.
-#define PXD_SMALL_N 16
-#define PXD_SMALL_K 32
+// Small supernodal updates, one wavefront per 64 stacked rows of a blok
+// update (the lower tree has hundreds of thousands of them with a few rows
+// and columns): task {cblk, blok, tm, flags} covers the stacked rows
+// [64 tm, 64 tm + 64) of an update with at most PXD_SMALL_M stacked rows
+// (Mr), PXD_SMALL_N facing columns (N) and PXD_SMALL_K source columns (K).
+// Lane r owns stacked row 64 tm + r and keeps its K values of A in
+// registers; the facing columns go by chunks of PXD_SMALL_NC: the chunk of
+// B (the rows of blok b, D-scaled for LDL^T) is staged in the wavefront's
+// LDS area, C[f(r), chunk] -= A_r B_chunk^T, plain read-modify-write or
+// atomic adds when the task is flagged.  The four wavefronts of a workgroup
+// are independent (no workgroup barrier).
+#define PXD_SMALL_M  256
+#define PXD_SMALL_N  64
+#define PXD_SMALL_NC 16
+#define PXD_SMALL_K  32
 template <typename T>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_update_small( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk *__restrict__ cb,
                 const dblok *__restrict__ bk, T *L, T *U, int ldlt, uint64_t *__restrict__ stamps, int prio )
 {
-    __shared__ T sB[PXD_THREADS / 64][PXD_SMALL_K][PXD_SMALL_N + 1];
+    constexpr int NC = PXD_SMALL_NC;
+    __shared__ T sB[PXD_THREADS / 64][PXD_SMALL_K][NC + 1];
     const uint64_t t_stamp = stamp_begin( stamps );
     wave_priority( prio );
     const int     lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1628,56 +1636,70 @@ k_update_small( const upd_task *__restrict__ tasks, int32_t ntasks, const dcblk 
         const int32_t  flb = cb[F + 1].fblok;
         const int32_t  colF0 = bb.frownum - cf.fcolnum;
         const int64_t  bbase = c.coefoff + (int64_t)bb.rowoff * K;
-        // B[j][k] = LB[bbase + j + k N], times D_k for LDL^T
-        for ( int idx = lane; idx < N * K; idx += 64 ) {
-            const int j = idx % N, kk = idx / N;
-            const T   v = LB[bbase + j + (int64_t)kk * N];
-            sB[w][kk][j] = ldlt ? v * L[c.coefoff + (int64_t)kk * ( K + 1 )] : v;
-        }
-        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        if ( lane < Mr ) {
-            const int32_t s   = r0 + lane;
+        const int32_t  row   = 64 * t.tm + lane;   // stacked row of the lane, from r0
+        // the lane's row: its K values of A and its place in F
+        T       a[PXD_SMALL_K];
+        T      *dst = nullptr;
+        int32_t dld = 0;
+        if ( row < Mr ) {
+            const int32_t s   = r0 + row;
             const int32_t bi  = find_blok_by_row( bk, t.blok, lb, s );
             const dblok   ib  = bk[bi];
             const int64_t src = c.coefoff + (int64_t)ib.rowoff * K + ( s - ib.rowoff );
             const int32_t sld = ib.lrownum - ib.frownum + 1;
             const int32_t g   = ib.frownum + ( s - ib.rowoff );
-            const int32_t fi  = find_blok_by_grow( bk, cf.fblok, flb, g );
-            const dblok   fb  = bk[fi];
-            T a[PXD_SMALL_K];
 #pragma unroll
             for ( int kk = 0; kk < PXD_SMALL_K; kk++ ) a[kk] = ( kk < K ) ? LA[src + (int64_t)kk * sld] : (T)0;
-            T acc[PXD_SMALL_N];
-#pragma unroll
-            for ( int j = 0; j < PXD_SMALL_N; j++ ) acc[j] = (T)0;
-#pragma unroll
-            for ( int kk = 0; kk < PXD_SMALL_K; kk++ ) {
-                if ( kk < K ) {
-#pragma unroll
-                    for ( int j = 0; j < PXD_SMALL_N; j++ ) acc[j] += a[kk] * sB[w][kk][j];
-                }
-            }
+            const int32_t fi = find_blok_by_grow( bk, cf.fblok, flb, g );
+            const dblok   fb = bk[fi];
             if ( g >= fb.frownum && g <= fb.lrownum ) {
-                const int32_t dld = fb.lrownum - fb.frownum + 1;
-                T *dst = LC + cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum ) + (int64_t)colF0 * dld;
+                dld = fb.lrownum - fb.frownum + 1;
+                dst = LC + cf.coefoff + (int64_t)fb.rowoff * Fn + ( g - fb.frownum ) + (int64_t)colF0 * dld;
+            }
+        }
+        for ( int32_t j0 = 0; j0 < N; j0 += NC ) {
+            const int32_t nc = min( NC, N - j0 );
+            // B[j][k] = LB[bbase + j + k N] for the chunk's columns, times D_k for LDL^T
+            for ( int idx = lane; idx < nc * K; idx += 64 ) {
+                const int j = idx % nc, kk = idx / nc;
+                const T   v = LB[bbase + j0 + j + (int64_t)kk * N];
+                sB[w][kk][j] = ldlt ? v * L[c.coefoff + (int64_t)kk * ( K + 1 )] : v;
+            }
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+            if ( dst != nullptr ) {
+                T acc[NC];
+#pragma unroll
+                for ( int j = 0; j < NC; j++ ) acc[j] = (T)0;
+#pragma unroll
+                for ( int kk = 0; kk < PXD_SMALL_K; kk++ ) {
+                    if ( kk < K ) {
+#pragma unroll
+                        for ( int j = 0; j < NC; j++ ) acc[j] += a[kk] * sB[w][kk][j];
+                    }
+                }
+                T *d0 = dst + (int64_t)j0 * dld;
                 if ( t.flags & 1 ) {
 #pragma unroll
-                    for ( int j = 0; j < PXD_SMALL_N; j++ ) {
-                        if ( j < N ) px_atomic_add( dst + (int64_t)j * dld, -acc[j] );
+                    for ( int j = 0; j < NC; j++ ) {
+                        if ( j < nc ) px_atomic_add( d0 + (int64_t)j * dld, -acc[j] );
                     }
                 }
                 else {
-                    T old[PXD_SMALL_N];
+                    T old[NC];
 #pragma unroll
-                    for ( int j = 0; j < PXD_SMALL_N; j++ ) old[j] = ( j < N ) ? dst[(int64_t)j * dld] : (T)0;
+                    for ( int j = 0; j < NC; j++ ) old[j] = ( j < nc ) ? d0[(int64_t)j * dld] : (T)0;
 #pragma unroll
-                    for ( int j = 0; j < PXD_SMALL_N; j++ ) {
-                        if ( j < N ) dst[(int64_t)j * dld] = old[j] - acc[j];
+                    for ( int j = 0; j < NC; j++ ) {
+                        if ( j < nc ) d0[(int64_t)j * dld] = old[j] - acc[j];
                     }
                 }
             }
+            // every lane has read the chunk before the next one is staged
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
         }
     }
     stamp_end( stamps, blockIdx.x, t_stamp );

@@ -728,11 +728,11 @@ pastix_amd_learned_predict( pastix_amd_learned_t *lm, int arith, const pxd_model
  * the share of the device an update holds over that time: the predicted
  * time times the fraction of the CUs its workgroups fill (64 x 64 tiles:
  * one workgroup per tile; tall tiles: one per PX_TALL_ROWS(arith) x 64
- * tile; one wavefront: a quarter of a workgroup).  The updates and their sizes are those the engine
+ * tile; one wavefront: a quarter of a workgroup per 64 rows).  The updates and their sizes are those the engine
  * schedules (kernels/core_zgemmsp.c:401-472 sizes: M the stacked rows from
  * the facing blok on, less the blok itself on an LU factor's U side, N the
  * blok's rows, K the cblk's width); the one-wavefront class is offered only
- * where it fits (M <= 64, N <= 16, K <= 32).  *predicted, if given, gets the
+ * where it fits (M <= 256, N <= 64, K <= 32).  *predicted, if given, gets the
  * predicted total update time of the choice.  Needs the structure on the
  * device (after pastix_subtask_bcsc2ctab); takes effect at the next
  * factorization.
@@ -776,7 +776,7 @@ pastix_amd_choose_update_kernels( pastix_data_t *pd, pastix_amd_learned_t *lm, p
                 const pastix_int_t M = c->stride - bb->coefind / K - ( side ? N : 0 );
                 const int          ok = ( b > c->fblok ) && ( side == 0 || lu ) && M > 0;
                 for ( int v = 0; v < 3; v++ ) {
-                    const int fits = ok && ( v < 2 || ( M <= 64 && N <= 16 && K <= 32 ) );
+                    const int fits = ok && ( v < 2 || ( M <= 256 && N <= 64 && K <= 32 ) );
                     e[v] = (pxd_model_entry_t){ PastixKernelGEMMCblk2d2d, fits ? (int32_t)M : 0, (int32_t)N,
                                                 (int32_t)K, 0., v, 0 };
                 }
@@ -805,7 +805,7 @@ pastix_amd_choose_update_kernels( pastix_data_t *pd, pastix_amd_learned_t *lm, p
             wg = ceil( M / PX_TALL_ROWS( arith ) ) * ceil( N / 64. );
         }
         else {
-            wg = 0.25;
+            wg = 0.25 * ceil( M / 64. );   /* one wavefront per 64 rows */
         }
         t[r] *= fmin( 1., wg / PX_DEVICE_CUS );
     }

@@ -378,9 +378,11 @@ def _logged_factorization(px, kind, nx, dof, facto, mixed, opts=None):
 
 
 # Engine options under which every blok update runs on one kernel class
-# (where it fits): the built-in cut-offs, all tall tiles, all 64 x 64 tiles.
+# (where it fits): the built-in cut-offs, all tall tiles, all 64 x 64 tiles,
+# the one-wavefront kernel over its whole range.
 REGIMES = {"cut-offs": {}, "tall": {"upd_small": 0, "upd_big_rows": 1},
-           "tiles64": {"upd_small": 0, "upd_big_rows": 2 ** 30}}
+           "tiles64": {"upd_small": 0, "upd_big_rows": 2 ** 30},
+           "wavefront": {"upd_small": 1, "upd_small_m": 256, "upd_small_n": 64}}
 
 
 def class_logs(px, runs, mixed, log=print):

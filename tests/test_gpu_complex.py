@@ -70,6 +70,32 @@ def test_complex_tall_tile_kernel_matches_oracle(dtype, facto):
 
 
 @pytest.mark.parametrize("facto", [px.PastixFactLDLT, px.PastixFactLU])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_complex_one_wavefront_updates_over_their_whole_range(dtype, facto):
+    """Every update of at most 256 stacked rows, 64 facing columns and 32
+    source columns on the one-wavefront kernel (upd_small_m = 256,
+    upd_small_n = 64)."""
+    nx = 11
+    spm = px.Spm.helmholtz7(nx, nx, nx, 0.5, 0.1)
+    pd, iparm, dparm = analyzed(spm, nx, facto=facto, mixed=int(dtype == np.complex64))
+    px.set_option(pd, "upd_small_m", 256)
+    px.set_option(pd, "upd_small_n", 64)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    bc = px.bcsc_export(pd, complex_=True)
+    tau = static_pivot_threshold(dparm, dtype)
+    L = px.get_coeftab(pd, s["coefnbr"], dtype, side=0)
+    if facto == px.PastixFactLDLT:
+        F = orc.OracleLDLT(s, bc, tau, dtype)
+    else:
+        F = orc.OracleLU(s, bc, tau, dtype)
+        U = px.get_coeftab(pd, s["coefnbr"], dtype, side=1)
+        assert cblk_diff(s, U, F.U, dtype, full_diag=True) <= 10.0
+    assert cblk_diff(s, L, F.L, dtype, full_diag=True) <= 10.0
+    px.finalize(pd)
+
+
+@pytest.mark.parametrize("facto", [px.PastixFactLDLT, px.PastixFactLU])
 def test_complex128_cblk_wider_than_lds_rows(facto):
     """17^3: the top separator is one 289-column cblk.  In complex double the
     panel's 257-row L21 strip and the inverse's 257-row R block exceed the

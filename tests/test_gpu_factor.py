@@ -122,6 +122,40 @@ def test_tall_tile_kernel_option_values():
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("kind,facto,dtype", [("lap7", None, np.float64), ("lap7", None, np.float32),
+                                               ("convdiff27", px.PastixFactGETRF, np.float64),
+                                               ("convdiff27", px.PastixFactGETRF, np.float32),
+                                               ("lap7", px.PastixFactLDLT, np.float64)])
+def test_one_wavefront_updates_over_their_whole_range_match_oracle(kind, facto, dtype):
+    """Every blok update of at most 256 stacked rows, 64 facing columns and
+    32 source columns on the one-wavefront kernel (one wavefront per 64
+    rows, the facing columns by chunks of 16): the factor matches the
+    oracle (L and U for LU; LDL^T with its D scaling)."""
+    nx = 14
+    spm = make_problem(kind, nx)
+    mixed = int(dtype == np.float32)
+    pd, iparm, dparm = analyzed(spm, nx, facto=facto, mixed=mixed)
+    px.set_option(pd, "upd_small", 1)
+    px.set_option(pd, "upd_small_m", 256)
+    px.set_option(pd, "upd_small_n", 64)
+    px.task_numfact(pd, spm)
+    s = px.solver_export(pd)
+    L = px.get_coeftab(pd, s["coefnbr"], dtype)
+    tau = static_pivot_threshold(dparm, dtype)
+    if facto == px.PastixFactGETRF:
+        F = orc.OracleLU(s, px.bcsc_export(pd), tau, dtype)
+        U = px.get_coeftab(pd, s["coefnbr"], dtype, side=1)
+        assert cblk_diff(s, L, F.L, dtype, full_diag=True) <= 10.0
+        assert cblk_diff(s, U, F.U, dtype, full_diag=True) <= 10.0
+    elif facto == px.PastixFactLDLT:
+        F = orc.OracleLDLT(s, px.bcsc_export(pd), tau, dtype)
+        assert cblk_diff(s, L, F.L, dtype, full_diag=True) <= 10.0
+    else:
+        F = orc.OracleFactor(s, px.bcsc_export(pd), tau, dtype)
+        assert cblk_diff(s, L, F.L, dtype) <= 10.0
+    px.finalize(pd)
+
+
 def test_device_solve_matches_oracle_solve():
     nx = 12
     spm, pd, iparm, dparm = _factor_case(nx)

@@ -32,7 +32,8 @@ def test_records_carry_the_update_class():
     assert (other["variant"] == -1).all()
     # the one-wavefront class only where it fits
     small = upd[upd["variant"] == 2]
-    assert (small["m"] <= 64).all() and (small["n"] <= 16).all() and (small["k"] <= 32).all()
+    assert (small["m"] <= 256).all() and (small["n"] <= 64).all() and (small["k"] <= 32).all()
+    assert (small["m"] > 64).any() and (small["n"] > 16).any()
 
 
 def test_learned_choice_factor_matches_oracle_and_beats_polynomials():
