@@ -105,6 +105,10 @@ int  pxd_get_stats( const pxd_ctx_t *ctx, pxd_stats_t *stats );
  *                   16 columns per batch (default 128)
  *   "solve_small"   1: levels of cblks of at most 16 columns solve with one
  *                   wavefront per cblk, one launch per level (default)
+ *   "solve_spread"  cblks with at least this many 64-row tiles (default 16;
+ *                   0: none) gather the backward sweep's sums in 8 partial
+ *                   copies instead of atomics into one; the records follow
+ *                   at the next factorization, 0 takes effect at once
  *   "solve_split_max", "solve_split_wg"  smaller launches share each tile
  *                   among up to solve_split_max workgroups (default 4),
  *                   solve_split_wg workgroups in all (default 512)

@@ -83,7 +83,7 @@ struct fact_graph_sig {
 
 struct solve_graph {
     int            nq, tag;
-    const void    *sig[6];   // vectors, Y, L, U and the two inverse sets
+    const void    *sig[7];   // vectors, Y, L, U, the two inverse sets and the partial sums
     uint64_t       gen;      // task-list generation
     int64_t        wide;     // solve_wide_max at capture
     hipGraphExec_t exec;
@@ -124,6 +124,12 @@ struct pxd_ctx_s {
     sv_row   *dsvr  = nullptr;   // records of the off-diagonal stacked rows
     sv_chunk *dsvc  = nullptr;   // the diag tasks with their cblk sizes
     sv_small *dsvs  = nullptr;   // the one-wavefront solve tasks
+    // backward partial sums of the cblks with many tiles (PXD_SV_SPREAD
+    // copies of spread_cols columns x PXD_MAXRHS, factor precision)
+    void     *dspread       = nullptr;
+    size_t    dspread_bytes = 0;
+    int64_t   spread_cols   = 0;
+    int       solve_spread  = PXD_SV_SPREAD_MIN;   // fewest tiles of a cblk that spreads; 0: none
     // 1: levels of cblks of at most PXD_DIAG_ROWS columns solve with one
     // wavefront per cblk (diagonal and off-diagonal rows in one launch)
     int       solve_small = 1;
@@ -580,16 +586,29 @@ build_schedule( pxd_ctx_t *ctx )
                 }
             }
         }
+        // columns in the backward partial sums for the cblks with at least
+        // solve_spread off-diagonal tiles (PXD_SV_SPREAD_MIN by default)
+        std::vector<int32_t> soff( nc, -1 );
+        ctx->spread_cols = 0;
+        for ( int64_t k = 0; k < nc; k++ ) {
+            const dcblk  &c = ctx->hcb[k];
+            const int32_t n = c.lcolnum - c.fcolnum + 1;
+            if ( ctx->solve_spread > 0 && ( c.stride - n + 63 ) / 64 >= ctx->solve_spread ) {
+                soff[k] = (int32_t)ctx->spread_cols;
+                ctx->spread_cols += n;
+            }
+        }
         std::vector<sv_tile>  svt( solv.size() );
         std::vector<sv_chunk> svc( diag.size() );
         for ( size_t i = 0; i < solv.size(); i++ ) {
             const dcblk  &c = ctx->hcb[solv[i].cblk];
             const int32_t n = c.lcolnum - c.fcolnum + 1;
-            svt[i] = sv_tile{ n, c.fcolnum, solv[i].r0, c.stride, rowstart[solv[i].cblk] + solv[i].r0 - n };
+            svt[i] = sv_tile{ n, c.fcolnum, solv[i].r0, c.stride, rowstart[solv[i].cblk] + solv[i].r0 - n,
+                              soff[solv[i].cblk], 0 };
         }
         for ( size_t i = 0; i < diag.size(); i++ ) {
             const dcblk &c = ctx->hcb[diag[i].cblk];
-            svc[i] = sv_chunk{ c.lcolnum - c.fcolnum + 1, c.fcolnum, diag[i].r0, 0, c.dinvoff };
+            svc[i] = sv_chunk{ c.lcolnum - c.fcolnum + 1, c.fcolnum, diag[i].r0, soff[diag[i].cblk], c.dinvoff };
         }
         std::vector<sv_small> svs( smallk.size() );
         for ( size_t i = 0; i < smallk.size(); i++ ) {
@@ -1106,8 +1125,9 @@ enum { SOLVE_FWD = 1, SOLVE_DIAG = 2, SOLVE_BWD = 4, SOLVE_ALL = 7 };
 template <typename T, typename V, int NR>
 void
 sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned nf, const T *L, const T *DI,
-          V *X, int64_t ldx, V *Y, int64_t ldy, int nrhs )
+          V *X, int64_t ldx, V *Y, int64_t ldy, int nrhs, V *S )
 {
+    const int64_t lds = ctx->spread_cols;   // the backward partial sums S (or none)
     constexpr int WB = solve_wide<T>::value;   // column batch of launches with few workgroups
     hipStream_t   s  = ctx->stream;
     const size_t  sh = (size_t)lv.maxn * NR * sizeof( V );
@@ -1138,7 +1158,7 @@ sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned
     if ( forward ) {
         if ( nd ) {
             hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
-                                DI, (const V *)X, ldx, Y, ldy, nrhs, 0 );
+                                DI, (const V *)X, ldx, Y, ldy, nrhs, 0, (const V *)nullptr, (int64_t)0 );
         }
         if ( nf && many ) {
             hipLaunchKernelGGL( ( k_sv_fwd_tile<T, V, NR, 16> ), dim3( nf ), dim3( PXD_THREADS ), sh, s,
@@ -1152,15 +1172,16 @@ sv_level( pxd_ctx_t *ctx, bool forward, const level_t &lv, unsigned nd, unsigned
     else {
         if ( nf && many ) {
             hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, 1> ), dim3( nf ), dim3( PXD_THREADS ), 0, s,
-                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, 1 );
+                                ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, 1, S, lds );
         }
         else if ( nf ) {
             hipLaunchKernelGGL( ( k_sv_bwd_tile<T, V, NR, WB / 16> ), dim3( nf * split_b ), dim3( PXD_THREADS ), 0,
-                                s, ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, split_b );
+                                s, ctx->dsvt + lv.f0, ctx->dsvr, L, (const V *)X, ldx, Y, ldy, nrhs, split_b, S,
+                                lds );
         }
         if ( nd ) {
             hipLaunchKernelGGL( ( k_sv_diag<T, V, NR> ), dim3( nd ), dim3( PXD_THREADS ), sh, s, ctx->dsvc + lv.d0,
-                                DI, (const V *)Y, ldy, X, ldx, nrhs, 1 );
+                                DI, (const V *)Y, ldy, X, ldx, nrhs, 1, (const V *)S, lds );
         }
     }
 }
@@ -1234,6 +1255,19 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
         }
         V            *Y   = Yscr ? Yscr : (V *)ctx->dsolvy;
         const int64_t ldy = ctx->n;
+        // the backward partial sums: not with the caller's scratch (nothing
+        // is allocated then), nor where the kernels keep the direct atomics
+        V         *S     = nullptr;
+        const bool sp_ok = nrhs <= 4 ? sv_spread_ok<V, 4>::value : sv_spread_ok<V, PXD_MAXRHS>::value;
+        if ( Yscr == nullptr && ctx->solve_spread && ctx->spread_cols > 0 && sp_ok ) {
+            const size_t sneed = (size_t)PXD_SV_SPREAD * PXD_MAXRHS * ctx->spread_cols * sizeof( V );
+            if ( ctx->dspread_bytes < sneed ) {
+                if ( ctx->dspread ) PXD_CHECK( hipFree( ctx->dspread ) );
+                PXD_CHECK( hipMalloc( &ctx->dspread, sneed ) );
+                ctx->dspread_bytes = sneed;
+            }
+            S = (V *)ctx->dspread;
+        }
         const int     nlv = nl;
         // the widest cblk's right-hand-side block must fit next to each
         // kernel's static LDS
@@ -1258,10 +1292,10 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
             unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
             unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
             if ( nrhs <= 4 ) {
-                sv_level<T, V, 4>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs );
+                sv_level<T, V, 4>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs, S );
             }
             else {
-                sv_level<T, V, PXD_MAXRHS>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs );
+                sv_level<T, V, PXD_MAXRHS>( ctx, true, lv, nd, nf, L, DI, X, ldx, Y, ldy, nrhs, S );
             }
         }
         if ( ( what & SOLVE_FWD ) && schur && iface ) {
@@ -1287,15 +1321,20 @@ solve_sweeps( pxd_ctx_t *ctx, V *X, int64_t ldx, int nrhs, int what = SOLVE_ALL,
                 schur_zero( X, ldx );
             }
         }
+        if ( S ) {
+            // cleared by a kernel (the sweeps may be captured in a graph)
+            hipLaunchKernelGGL( ( k_zero_cols<V> ), dim3( nblocks( ctx->spread_cols, 256 ), PXD_SV_SPREAD * nrhs ),
+                                dim3( 256 ), 0, s, ctx->spread_cols, S, ctx->spread_cols );
+        }
         for ( int li = nlv - 1; li >= 0; li-- ) {
             const level_t &lv = ctx->levels[li];
             unsigned       nd = (unsigned)( ( schur ? lv.ds : lv.d1 ) - lv.d0 );
             unsigned       nf = (unsigned)( ( schur ? lv.fs : lv.f1 ) - lv.f0 );
             if ( nrhs <= 4 ) {
-                sv_level<T, V, 4>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs );
+                sv_level<T, V, 4>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs, S );
             }
             else {
-                sv_level<T, V, PXD_MAXRHS>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs );
+                sv_level<T, V, PXD_MAXRHS>( ctx, false, lv, nd, nf, LB, DB, X, ldx, Y, ldy, nrhs, S );
             }
         }
         if ( schur && !iface ) {
@@ -1427,6 +1466,13 @@ graph_buffers( pxd_ctx_t *ctx )
         PXD_CHECK( hipMalloc( &ctx->dsolvy, need ) );
         ctx->dsolvy_bytes = need;
     }
+    // the backward partial sums, so that no allocation happens under capture
+    const size_t sneed = (size_t)PXD_SV_SPREAD * PXD_MAXRHS * ctx->spread_cols * sizeof( T );
+    if ( sv_spread_ok<T, 4>::value && ctx->solve_spread && ctx->dspread_bytes < sneed ) {
+        if ( ctx->dspread ) PXD_CHECK( hipFree( ctx->dspread ) );
+        PXD_CHECK( hipMalloc( &ctx->dspread, sneed ) );
+        ctx->dspread_bytes = sneed;
+    }
     return 0;
 }
 
@@ -1436,7 +1482,8 @@ template <typename T>
 int
 graph_pass( pxd_ctx_t *ctx, int nq )
 {
-    const void *sig[6] = { ctx->dvecs, ctx->dsolvy, ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv };
+    const void *sig[7] = { ctx->dvecs, ctx->dsolvy, ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv,
+                           ctx->solve_spread ? ctx->dspread : nullptr };
     solve_graph *hit  = nullptr;
     for ( size_t i = 0; i < ctx->sgraphs.size(); i++ ) {
         solve_graph &g = ctx->sgraphs[i];
@@ -1776,7 +1823,7 @@ pxd_destroy( pxd_ctx_t *ctx )
     if ( ctx->fgraph ) {
         (void)hipGraphExecDestroy( ctx->fgraph );
     }
-    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->dsvs, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
+    void *ptrs[] = { ctx->dcb, ctx->dbk, ctx->dcol2cblk, ctx->dlist, ctx->dpan, ctx->dflags, ctx->dtrsm, ctx->dupd, ctx->dupdb, ctx->dupds, ctx->dsolv, ctx->dsvt, ctx->dsvr, ctx->dsvc, ctx->dsvs, ctx->dspread, ctx->ddiag, ctx->dsolvy, ctx->dinvt,
                      ctx->dcolptr, ctx->drowtab, ctx->dtcolptr, ctx->dtrowtab, ctx->dval, ctx->dtval,
                      ctx->dL, ctx->dU, ctx->ddinv, ctx->dudinv, ctx->dscratch, ctx->dvecs, ctx->dred,
                      };
@@ -1828,6 +1875,14 @@ pxd_set_option( pxd_ctx_t *ctx, const char *key, double value )
     if ( k == "upd_small" ) {
         ctx->upd_small   = ( value != 0. );
         ctx->tasks_dirty = 1;
+        return 0;
+    }
+    if ( k == "solve_spread" ) {
+        if ( value < 0. || value > 1e6 ) {
+            return 7;
+        }
+        ctx->solve_spread = (int)value;
+        ctx->tasks_dirty  = 1;   // the records take it at the next factorization
         return 0;
     }
     if ( k == "solve_small" ) {

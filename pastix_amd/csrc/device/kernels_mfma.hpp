@@ -757,7 +757,10 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
     typedef typename px_real<T>::type R;
     const R tau = (R)thresh;
 
-    // the block row: T(p, j) for j = 0..p (lower part only for j = p)
+    // the block row: T(p, j) for j = 0..p (lower part only for j = p).
+    // Every load is issued, the entries outside read a clamped in-range
+    // address and are masked: a load under its own branch made the compiler
+    // wait for the previous one.
     for ( int j0 = 0; j0 <= p; j0 += 4 ) {
         T v[4][NU];
 #pragma unroll
@@ -765,8 +768,10 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
 #pragma unroll
             for ( int u = 0; u < NU; u++ ) {
                 const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB, j = j0 + jj;
-                const bool ok = j <= p && r < bs && ( j < p || cc <= r );
-                v[jj][u] = ok ? A[( R0 + r ) + (int64_t)( PXD_NB * j + cc ) * n] : (T)0;
+                const bool    ok  = j <= p && r < bs && ( j < p || cc <= r );
+                const int32_t col = min( PXD_NB * min( j, p ) + cc, n - 1 );
+                const T       x   = A[( R0 + min( r, bs - 1 ) ) + (int64_t)col * n];
+                v[jj][u] = ok ? x : (T)0;
             }
 #pragma unroll
         for ( int jj = 0; jj < 4; jj++ )

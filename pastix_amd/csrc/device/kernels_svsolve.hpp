@@ -25,6 +25,8 @@ struct sv_tile {
     int32_t r0;        // first stacked row of the tile
     int32_t stride;    // stacked rows of the cblk
     int64_t row0;      // record of stacked row r0
+    int32_t soff;      // the cblk's columns in the backward partial sums, or -1
+    int32_t pad;
 };
 // ... one record per off-diagonal stacked row
 struct sv_row {
@@ -38,8 +40,27 @@ struct sv_chunk {
     int32_t n;
     int32_t fcolnum;
     int32_t R0;
-    int32_t pad;
+    int32_t soff;      // the cblk's columns in the backward partial sums, or -1
     int64_t dinvoff;
+};
+
+// Backward partial sums.  The backward tiles of one cblk all subtract into
+// the same n x nrhs entries of Y, so on the levels of one or two cblks (150
+// to 230 tiles per cblk at 100^3) each entry took that many atomics in a
+// row.  A cblk with at least PXD_SV_SPREAD_MIN tiles has instead its own
+// columns soff .. soff + n - 1 in PXD_SV_SPREAD copies of a partial-sum
+// array (copy c at S + c * nrhs * lds, right-hand side q at + q * lds); tile
+// t adds into copy t % PXD_SV_SPREAD, and the cblk's diagonal solve adds the
+// copies to Y as it loads it.  The copies are cleared before each backward
+// sweep.
+// Only where the copies' loads fit in registers: NR * sizeof(V) <= 32 bytes
+// (4 right-hand sides up to complex float, 8 in float); elsewhere the tiles
+// add into Y directly.
+#define PXD_SV_SPREAD 8
+#define PXD_SV_SPREAD_MIN 16
+template <typename V, int NR>
+struct sv_spread_ok {
+    static constexpr bool value = NR * sizeof( V ) <= 32;
 };
 
 #define PXD_SV_MAXN ( PXD_TRSM_NT * 16 )   // widest cblk (352 columns)
@@ -63,6 +84,45 @@ sv_stage( V *s, const V *__restrict__ v, int64_t ld, int32_t n, int nrhs, int ti
     for ( int e = 0; e < MAXE; e++ ) {
         const int idx = tid + PXD_THREADS * e;
         if ( idx < n * NR ) s[( idx % n ) * NR + idx / n] = t[e];
+    }
+}
+
+// sv_stage of v plus the PXD_SV_SPREAD partial-sum copies of S (copy c at
+// S + c * cst, right-hand side q at + q * lds).  The entries past n * NR
+// redo the last one (same loads, same sums, so the same bits to the same LDS
+// slot): no load or store sits under a branch, which had the compiler sink
+// the copies' loads into the store's branch and wait for each in turn.  One
+// 32-bit offset per entry against the copies' uniform bases lets the loads
+// share their address registers.
+template <typename V, int NR>
+__device__ inline void
+sv_stage_sum( V *s, const V *__restrict__ v, int64_t ld, const V *__restrict__ S, int64_t cst, int64_t lds,
+              int32_t n, int nrhs, int tid )
+{
+    constexpr int MAXE = ( PXD_SV_MAXN * NR + PXD_THREADS - 1 ) / PXD_THREADS;
+    V             t[MAXE];
+    V             p[MAXE][PXD_SV_SPREAD];
+    int32_t       off[MAXE];
+#pragma unroll
+    for ( int e = 0; e < MAXE; e++ ) {
+        const int idx = min( tid + PXD_THREADS * e, n * NR - 1 );
+        const int j = idx % n, q = min( idx / n, nrhs - 1 );
+        off[e] = j + q * (int32_t)lds;
+        t[e]   = v[j + q * ld];
+    }
+#pragma unroll
+    for ( int c = 0; c < PXD_SV_SPREAD; c++ ) {
+        const V *__restrict__ Sc = S + c * cst;
+#pragma unroll
+        for ( int e = 0; e < MAXE; e++ ) p[e][c] = Sc[off[e]];
+    }
+#pragma unroll
+    for ( int e = 0; e < MAXE; e++ ) {
+        const int idx = min( tid + PXD_THREADS * e, n * NR - 1 );
+        V         a   = t[e];
+#pragma unroll
+        for ( int c = 0; c < PXD_SV_SPREAD; c++ ) a += p[e][c];
+        s[( idx % n ) * NR + idx / n] = ( idx / n < nrhs ) ? a : (V)0;
     }
 }
 
@@ -99,9 +159,12 @@ k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
     __syncthreads();
     if ( valid ) {
         for ( int32_t cb0 = c0; cb0 < c1; cb0 += B ) {
+            // every load of the batch is issued, the columns past c1 reading
+            // column c1 - 1 again: a load under its own branch made the
+            // compiler wait for the previous one (one round trip per column)
             T a[B];
 #pragma unroll
-            for ( int u = 0; u < B; u++ ) a[u] = ( cb0 + u < c1 ) ? L[ri.base + (int64_t)( cb0 + u ) * ri.ld] : (T)0;
+            for ( int u = 0; u < B; u++ ) a[u] = L[ri.base + (int64_t)min( cb0 + u, c1 - 1 ) * ri.ld];
 #pragma unroll
             for ( int u = 0; u < B; u++ ) {
                 if ( cb0 + u < c1 ) {
@@ -135,7 +198,7 @@ k_sv_fwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
 template <typename T, typename V, int NR, int NCH>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ rows, const T *__restrict__ L,
-               const V *__restrict__ X, int64_t ldx, V *Y, int64_t ldy, int nrhs, int split )
+               const V *__restrict__ X, int64_t ldx, V *Y, int64_t ldy, int nrhs, int split, V *S, int64_t lds )
 {
     __shared__ T sA[64][64 + 1];   // sA[col][row]
     __shared__ V sx[64][NR];
@@ -151,6 +214,11 @@ k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
     if ( valid ) {
         ri = rows[t.row0 + rr];
     }
+    // the sums go to Y, or to the tile's copy of the cblk's partial sums
+    const bool spread = sv_spread_ok<V, NR>::value && S != nullptr && t.soff >= 0;
+    V *const   dst    = spread ? S + (int64_t)( ( idx / split ) % PXD_SV_SPREAD ) * nrhs * lds + t.soff
+                               : Y + t.fcolnum;
+    const int64_t ldd = spread ? lds : ldy;
     // X of the tile's rows (wave 0) and the first NCH chunks of the tile
     if ( part == 0 ) {
         V xv[NR];
@@ -199,7 +267,7 @@ k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
             for ( int q = 0; q < NR; q++ ) {
                 if ( q < nrhs ) {
                     const V s = red[0][col][q] + red[1][col][q] + red[2][col][q] + red[3][col][q];
-                    px_atomic_add( Y + t.fcolnum + c0 + col + q * ldy, -s );
+                    px_atomic_add( dst + c0 + col + q * ldd, -s );
                 }
             }
         }
@@ -214,7 +282,8 @@ k_sv_bwd_tile( const sv_tile *__restrict__ tiles, const sv_row *__restrict__ row
 template <typename T, typename V, int NR>
 __global__ void __launch_bounds__( PXD_THREADS )
 k_sv_diag( const sv_chunk *__restrict__ chunks, const T *__restrict__ Linv, const V *__restrict__ Xin,
-           int64_t ldx, V *__restrict__ Yout, int64_t ldy, int nrhs, int trans )
+           int64_t ldx, V *__restrict__ Yout, int64_t ldy, int nrhs, int trans, const V *__restrict__ S,
+           int64_t lds )
 {
     constexpr int CR   = PXD_DIAG_ROWS;
     constexpr int NG   = PXD_THREADS / CR;   // column phases
@@ -238,7 +307,17 @@ k_sv_diag( const sv_chunk *__restrict__ chunks, const T *__restrict__ Linv, cons
         const bool    ok = i < n && j < n && ( trans ? j >= i : j <= i );
         a[u] = ok ? LI[trans ? j + (int64_t)i * n : i + (int64_t)j * n] : (T)0;
     }
-    sv_stage<V, NR>( sb, Xin + ch.fcolnum, ldx, n, nrhs, tid );
+    bool summed = false;
+    if constexpr ( sv_spread_ok<V, NR>::value ) {
+        if ( S != nullptr && ch.soff >= 0 ) {
+            // backward: Y plus the cblk's partial sums
+            sv_stage_sum<V, NR>( sb, Xin + ch.fcolnum, ldx, S + ch.soff, (int64_t)nrhs * lds, lds, n, nrhs, tid );
+            summed = true;
+        }
+    }
+    if ( !summed ) {
+        sv_stage<V, NR>( sb, Xin + ch.fcolnum, ldx, n, nrhs, tid );
+    }
     __syncthreads();
 
     V acc[NR];

@@ -171,6 +171,46 @@ def test_device_solve_matches_oracle_solve():
     px.finalize(pd)
 
 
+@pytest.mark.parametrize("spread", [1, 2, 0])
+@pytest.mark.parametrize("facto,mixed,nrhs", [(None, 0, 2), (None, 1, 4), (px.PastixFactGETRF, 0, 3),
+                                              (px.PastixFactLDLT, 0, 7)])
+def test_backward_partial_sums_match_oracle_solve(spread, facto, mixed, nrhs):
+    """The backward sweep's gather into partial-sum copies (solve_spread = 1:
+    every cblk with an off-diagonal tile, 2: those with two or more, 0: the
+    direct atomics): the solve matches the oracle's, for LL^T, LU and LDL^T,
+    single and double factors, 2 to 7 right-hand sides (7 takes the 8-column
+    launches, where the double factor keeps the direct atomics)."""
+    nx = 14
+    kind = "convdiff27" if facto == px.PastixFactGETRF else "lap7"
+    spm = make_problem(kind, nx)
+    pd, iparm, dparm = analyzed(spm, nx, facto=facto, mixed=mixed)
+    px.set_option(pd, "solve_spread", spread)
+    px.task_numfact(pd, spm)
+    dtype = np.float32 if mixed else np.float64
+    s = px.solver_export(pd)
+    tau = static_pivot_threshold(dparm, dtype)
+    Or = {None: orc.OracleFactor, px.PastixFactGETRF: orc.OracleLU, px.PastixFactLDLT: orc.OracleLDLT}[facto]
+    F = Or(s, px.bcsc_export(pd), tau, dtype)
+    x0, b = spm.gen_rhs(nrhs, seed=5)
+    bp = permute_rhs(pd, b)
+    xo = F.solve(bp)
+    for rep in range(2):   # the copies are cleared again for the second solve
+        x = np.array(b, order="F", copy=True)
+        px.task_solve(pd, spm, x)
+        xp = permute_rhs(pd, x)
+        tol = (1e-4 if mixed else 1e-12) * max(1.0, np.abs(xo).max())
+        assert np.abs(xp - xo).max() <= tol, rep
+    px.finalize(pd)
+
+
+def test_backward_partial_sums_option_values():
+    spm = make_problem("lap7", 6)
+    pd, iparm, dparm = analyzed(spm, 6)
+    with pytest.raises(px.PastixError):
+        px.set_option(pd, "solve_spread", -1)
+    px.finalize(pd)
+
+
 @pytest.mark.parametrize("nx,nrhs", [(8, 1), (16, 4)])
 def test_mixed_fp32_gmres_reaches_1e12(nx, nrhs):
     spm, pd, iparm, dparm = _factor_case(nx, mixed=1)
