@@ -387,10 +387,22 @@ k_sv_small_fwd( const sv_small *__restrict__ tasks, int64_t ntasks, const sv_row
     const sv_small t = tasks[ti];
     const int      i = lane & ( CR - 1 ), q = lane >> 4;
     const T       *LI = Linv + t.dinvoff;
-    T              a[CR];
+    // the record of the lane's first off-diagonal row travels with the
+    // inverse and the right-hand side, and that row's factor entries are in
+    // flight while the diagonal block is solved: three dependent trips to
+    // memory instead of four
+    const bool has0 = lane < t.nrows;
+    sv_row     ri0{ 0, 0, 0 };
+    if ( has0 ) {
+        ri0 = rows[t.row0 + lane];
+    }
+    T a[CR];
 #pragma unroll
     for ( int j = 0; j < CR; j++ ) a[j] = ( i < t.n && j <= i ) ? LI[i + (int64_t)j * t.n] : (T)0;
     const V b = ( i < t.n && q < nrhs ) ? X[t.fcolnum + i + q * ldx] : (V)0;
+    T       av0[CR];
+#pragma unroll
+    for ( int j = 0; j < CR; j++ ) av0[j] = ( has0 && j < t.n ) ? L[ri0.base + (int64_t)j * ri0.ld] : (T)0;
     sb[w][i][q] = b;
     sv_wave_sync();
     V y = (V)0;
@@ -401,10 +413,15 @@ k_sv_small_fwd( const sv_small *__restrict__ tasks, int64_t ntasks, const sv_row
     if ( i < t.n && q < nrhs ) Y[t.fcolnum + i + q * ldy] = y;
     sv_wave_sync();
     for ( int32_t s = lane; s < t.nrows; s += 64 ) {
-        const sv_row ri = rows[t.row0 + s];
-        T            av[CR];
+        sv_row ri = ri0;
+        T      av[CR];
 #pragma unroll
-        for ( int j = 0; j < CR; j++ ) av[j] = ( j < t.n ) ? L[ri.base + (int64_t)j * ri.ld] : (T)0;
+        for ( int j = 0; j < CR; j++ ) av[j] = av0[j];
+        if ( s != lane ) {
+            ri = rows[t.row0 + s];
+#pragma unroll
+            for ( int j = 0; j < CR; j++ ) av[j] = ( j < t.n ) ? L[ri.base + (int64_t)j * ri.ld] : (T)0;
+        }
 #pragma unroll
         for ( int qq = 0; qq < 4; qq++ ) {
             if ( qq < nrhs ) {
