@@ -659,9 +659,15 @@ k_potrf2( const int32_t *__restrict__ list, const dcblk *__restrict__ cb, T *__r
 // flag[p] = p+1.  The critical path is the chain of diagonal tiles instead
 // of the n/32 sequential block steps of a single workgroup.
 //
-// Publication: a workgroup-scope release by every thread, a barrier, then a
-// device-scope release and the flag store by thread 0.  Waits: thread 0
-// polls the flag, then a device-scope acquire and a barrier.  Every wait is
+// Exchange between the workgroups: the L(p,k) tiles and the diagonal
+// inverses that other blocks read are written and read with agent-scope
+// relaxed atomic stores and loads (panel_st, panel_ld), which are coherent
+// across the XCDs by themselves.  Publication: every thread waits until its
+// own stores are complete, a barrier, then thread 0 stores the flag.  Waits:
+// thread 0 polls the flag, then a barrier.  No agent-scope fence is needed,
+// and none is issued: such a fence writes back (release) or invalidates
+// (acquire) the XCD's whole L2, which the update kernels of the other
+// stream keep filling (100^3: FP32 numfact 86.3 -> 85.4 ms).  Every wait is
 // bounded by a wall-clock deadline; on expiry the kernel records it in *err
 // and goes on, so every wave reaches the end.
 struct pan_task {
@@ -689,21 +695,36 @@ panel_wait( const int *f, int target, int *err, int tid )
             }
             __builtin_amdgcn_s_sleep( 1 );
         }
-        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
     }
     __syncthreads();
-    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "workgroup" );
+    __atomic_signal_fence( __ATOMIC_SEQ_CST );   // the block's loads stay after the wait
 }
 
 __device__ inline void
 panel_publish( int *f, int value, int tid )
 {
-    __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup" );
+    // each thread's panel_st stores complete (vmcnt 0) before the barrier,
+    // so the flag is stored after all of them
+    __atomic_signal_fence( __ATOMIC_SEQ_CST );
+    __builtin_amdgcn_s_waitcnt( 0 );
     __syncthreads();
     if ( tid == 0 ) {
-        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "agent" );
         __hip_atomic_store( f, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
     }
+}
+
+// Loads and stores of the data that the panel's workgroups exchange.
+template <typename T>
+__device__ inline T
+panel_ld( const T *p )
+{
+    return __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+template <typename T>
+__device__ inline void
+panel_st( T *p, T v )
+{
+    __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
 }
 
 // P[r][c] = sum_l X[r][l] Y[c][l] for 32 x 32 tiles held as s[col][row]:
@@ -790,7 +811,7 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
 #pragma unroll
             for ( int u = 0; u < NU; u++ ) {
                 const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
-                v[u] = LI[( PXD_NB * k + r ) + (int64_t)( PXD_NB * k + cc ) * n];
+                v[u] = panel_ld( LI + ( PXD_NB * k + r ) + (int64_t)( PXD_NB * k + cc ) * n );
             }
 #pragma unroll
             for ( int u = 0; u < NU; u++ ) {
@@ -808,7 +829,7 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
                 const int r = 16 * qr + M::row( lane, v ), cc = 16 * qc + ( lane & 15 );
                 const T   x = ( r < bs ) ? M::get( acc, v ) : (T)0;
                 sT[k][cc][r] = x;
-                if ( r < bs ) A[( R0 + r ) + (int64_t)( PXD_NB * k + cc ) * n] = x;
+                if ( r < bs ) panel_st( A + ( R0 + r ) + (int64_t)( PXD_NB * k + cc ) * n, x );
             }
         }
         if ( !last ) {
@@ -824,7 +845,7 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
 #pragma unroll
                 for ( int u = 0; u < NU; u++ ) {
                     const int idx = tid + u * PXD_THREADS, r = idx % PXD_NB, cc = idx / PXD_NB;
-                    v[u] = A[( PXD_NB * j + r ) + (int64_t)( PXD_NB * k + cc ) * n];
+                    v[u] = panel_ld( A + ( PXD_NB * j + r ) + (int64_t)( PXD_NB * k + cc ) * n );
                 }
 #pragma unroll
                 for ( int u = 0; u < NU; u++ ) {
@@ -875,7 +896,7 @@ k_potrf_mw( const pan_task *__restrict__ tasks, const dcblk *__restrict__ cb, T 
         if ( rr >= cc ) {
             A[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sA[cc][rr];
         }
-        LI[( R0 + rr ) + (int64_t)( R0 + cc ) * n] = sI[cc][rr];
+        panel_st( LI + ( R0 + rr ) + (int64_t)( R0 + cc ) * n, sI[cc][rr] );
     }
     if ( !last ) {
         panel_publish( flag + p, p + 1, tid );
