@@ -19,7 +19,9 @@ rounding:
 - ``upd_dma_pad``: extra LDS per workgroup of that kernel, which sets how
   many of its workgroups share a CU with the lookahead stream's panels;
 - ``solve_split_max``, ``solve_split_wg``: how many workgroups share a tile
-  of a level-solve launch with few tiles.
+  of a level-solve launch with few tiles;
+- ``solve_spread``: the fewest tiles of a cblk whose backward gather goes
+  through partial-sum copies (0: none).
 
 The search is coordinate descent from the current values (each option in
 turn over its candidates, the others fixed, two passes); the objective is
@@ -57,10 +59,11 @@ SPACE = {
     "upd_small_n": [16, 32, 64],
     "solve_split_max": [1, 2, 4, 8],
     "solve_split_wg": [256, 512, 1024, 2048],
+    "solve_spread": [0, 8, 16, 32],
 }
 DEFAULTS = {"upd_big_rows": 192, "upd_small": 1, "solve_wide_max": 128, "panel_mw": 1, "upd_dma": 1,
             "upd_dma_pad": 4096, "upd_small_m": 64, "upd_small_n": 16, "solve_split_max": 4,
-            "solve_split_wg": 512}
+            "solve_split_wg": 512, "solve_spread": 16}
 
 # problems: (label, generator, grid edge, factorization, mixed precision, RHS)
 PROBLEMS = {
